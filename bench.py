@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Benchmark: ray-primitive tests/s and rays/s of the view-factor engine on one MI355X.
+
+Workload (default): the batched design sweep -- 256 coaxial furnace
+geometries (34-605 primitives each) with 1e6 rays per emitter, solved in the
+FP32 + FP64-refined form by one persistent launch per step.  A step is one
+full solve of all 256 geometries (every emitter, every ray).  Beside it the
+same math runs in the plain one-geometry-per-launch form on a sample of the
+geometries, and the CPU oracle runs a bounded sample on the host cores.
+
+Prints ONE JSON line (see the driver contract).  --gpus N > 1 is refused:
+this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector (packed) / dense FP32 peak
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(args):
+    from rayfactorcl_amd import scenes
+    if args.workload == "sweep":
+        geoms = scenes.sweep(args.geoms, seed=args.seed)
+        desc = f"{args.geoms}-geometry furnace design sweep, {args.rays:.0e} rays/emitter"
+    elif args.workload == "furnace":
+        geoms = [scenes.parse_xml(scenes.furnace_xml())]
+        desc = f"furnace scene (244 primitives), {args.rays:.0e} rays/emitter"
+    elif args.workload == "soup":
+        geoms = [scenes.parse_xml(scenes.soup_xml(1000, seed=args.seed))]
+        desc = f"random primitive soup (1000 primitives), {args.rays:.0e} rays/emitter"
+    else:
+        raise SystemExit(f"unknown workload {args.workload}")
+    return geoms, desc
+
+
+def cpu_baseline(geoms, seconds: float, threads: int):
+    """Oracle (compat FP32 restatement of the reference kernel) on host cores,
+    a bounded sample: every emitter of the median-size geometry, ray budget
+    scaled so the run takes about `seconds`."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    sizes = [g.n for g in geoms]
+    g = geoms[int(np.argsort(sizes)[len(sizes) // 2])]
+    n = g.n
+    # calibrate on one emitter
+    wi = 256
+    t0 = time.perf_counter()
+    O.solve(g.prims, None, n_wi=wi, mode="compat", rng="seeded", threads=threads, emitters=(0, min(n, 4)))
+    dt = time.perf_counter() - t0
+    tests_per_s = 4 * wi * n * min(n, 4) / max(dt, 1e-6)
+    n_wi = max(16, int(seconds * tests_per_s / (4.0 * n * n)))
+    t0 = time.perf_counter()
+    hits, rays = O.solve(g.prims, None, n_wi=n_wi, mode="compat", rng="seeded", threads=threads)
+    dt = time.perf_counter() - t0
+    total_rays = int(rays.sum())
+    tests = total_rays * n
+    return {
+        "value": tests / dt, "unit": "tests/s", "rays_per_s": total_rays / dt, "cores": threads, "kind": "port",
+        "sample": f"compat FP32 oracle, all {n} emitters of one {n}-primitive sweep geometry, "
+                  f"{4 * n_wi} rays/emitter ({tests:.3e} tests, {dt:.1f} s)",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="sweep", choices=["sweep", "furnace", "soup"])
+    ap.add_argument("--geoms", type=int, default=256)
+    ap.add_argument("--rays", type=float, default=1e6, help="rays per emitter")
+    ap.add_argument("--seed", type=int, default=0x52465331)
+    ap.add_argument("--plain-geoms", type=int, default=16, help="geometries in the plain-form comparison")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-refine", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
+        return 2
+
+    import rayfactorcl_amd as rf
+
+    geoms, desc = build_workload(args)
+    log(f"workload: {desc}; primitives {min(g.n for g in geoms)}..{max(g.n for g in geoms)}")
+    ctx = rf.Context(0)
+    dev = [rf.DeviceScene(ctx, g.prims, None) for g in geoms]
+    rays = int(args.rays)
+    refine = not args.no_refine
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        rpt, blk = rf.autotune(ctx, dev, rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays,
+                                                      refine=refine))
+        log(f"autotune: {rpt} rays/thread, {blk} threads/group ({time.perf_counter() - t0:.1f} s)")
+    opts = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", refine=refine,
+                        seed=args.seed)
+    for i in range(args.warmup):
+        _, t = rf.batch_solve(ctx, dev, opts, readback=False)
+        log(f"warmup {i}: {t['kernel_ms']:.1f} ms kernel")
+    ctx.sync()
+    t_all = []
+    kernel_ms = 0.0
+    tests = flops = nrays = refined = launches = 0
+    wall0 = time.perf_counter()
+    for i in range(args.steps):
+        s0 = time.perf_counter()
+        _, t = rf.batch_solve(ctx, dev, opts, readback=False)
+        ctx.sync()
+        t_all.append(time.perf_counter() - s0)
+        kernel_ms += t["kernel_ms"]
+        tests += t["tests"]; flops += t["flops"]; nrays += t["rays"]; refined += t["refined"]
+        launches += t["launches"]
+        log(f"step {i}: {t_all[-1] * 1e3:.1f} ms wall, {t['kernel_ms']:.1f} ms kernel, "
+            f"{t['tests'] / (t['kernel_ms'] * 1e-3):.3e} tests/s")
+    wall = time.perf_counter() - wall0
+    value = tests / wall
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+
+    # plain one-geometry-per-launch form on a sample of the geometries (same math and streams)
+    plain = None
+    if args.plain_geoms > 0:
+        idx = np.linspace(0, len(dev) - 1, min(args.plain_geoms, len(dev))).astype(int)
+        sub = [dev[i] for i in idx]
+        ob = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", refine=refine,
+                          seed=args.seed)
+        op = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="per_emitter", refine=refine,
+                          seed=args.seed)
+        orf = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="reference", refine=refine,
+                           seed=args.seed)
+        res = {}
+        for name, o in [("batched", ob), ("per_emitter", op), ("reference_form", orf)]:
+            rf.batch_solve(ctx, sub, o, readback=False)
+            s0 = time.perf_counter()
+            _, t = rf.batch_solve(ctx, sub, o, readback=False)
+            dt = time.perf_counter() - s0
+            res[name] = {"tests_per_s": t["tests"] / dt, "kernel_ms": t["kernel_ms"], "wall_ms": dt * 1e3,
+                         "launches": t["launches"]}
+            log(f"sample {name}: {res[name]['tests_per_s']:.3e} tests/s, {t['launches']} launches")
+        plain = {
+            "geometries": len(sub),
+            "batched_tests_per_s": res["batched"]["tests_per_s"],
+            "plain_per_emitter_tests_per_s": res["per_emitter"]["tests_per_s"],
+            "plain_reference_form_tests_per_s": res["reference_form"]["tests_per_s"],
+            "batched_over_per_emitter": res["batched"]["tests_per_s"] / res["per_emitter"]["tests_per_s"],
+            "batched_over_reference_form": res["batched"]["tests_per_s"] / res["reference_form"]["tests_per_s"],
+        }
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: {threads} threads, ~{args.cpu_seconds:.0f} s")
+        cpu = cpu_baseline(geoms, args.cpu_seconds, threads)
+        log(f"cpu baseline: {cpu['value']:.3e} tests/s")
+
+    out = {
+        "metric": "ray-primitive tests/s (batched FP32+FP64-refined form), 1 MI355X",
+        "value": value,
+        "unit": "tests/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / max(args.steps, 1) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": desc, "geometries": len(geoms), "rays_per_emitter": rays,
+                   "primitives_total": int(sum(g.n for g in geoms)), "form": "persistent",
+                   "math": "fp32 fast + fp64 refine" if refine else "fp32 fast", "rng": "philox4x32-10 seeded",
+                   "parallelism": "single GPU"},
+        "rays_per_s": nrays / wall,
+        "refined_pairs": refined,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "rf::k_queue<fast,plain>", "avg_launch_ms": kernel_ms / max(launches, 1),
+                     "flops_per_launch": flops / max(launches, 1)},
+        "plain_comparison": plain,
+        "cpu_baseline": cpu,
+        "device": {"arch": ctx.arch, "cus": ctx.n_cu, "clock_mhz": ctx.clock_mhz},
+    }
+    print(json.dumps(out), flush=True)
+    for d in dev:
+        d.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,289 @@
+/*
+ * rayfactor.h -- C ABI of the MI355X (gfx950) Monte Carlo view-factor engine.
+ *
+ * Drop-in boundary: this ABI replaces the OpenCL kernel dispatch inside
+ * RayFactorCL's runCL_GPURNG (RayFactorCL/main.cpp:54-251).  Plain C types,
+ * pointers and sizes only; every failure is returned as an rf_status code
+ * (the reference prints and calls exit(1), util_opencl.h:123-131).
+ *
+ * Threading: an rf_ctx owns one HIP device and one stream; calls on one
+ * context must not overlap (no internal locking).  Different contexts are
+ * independent.
+ */
+#ifndef RAYFACTOR_H
+#define RAYFACTOR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RF_ABI_VERSION 1
+
+/* ------------------------------------------------------------------ */
+/* Scene ABI                                                           */
+/* ------------------------------------------------------------------ */
+
+/* Replaces: enum PrimitiveType, RayfactorConstants.h:25-33 (same values). */
+typedef enum rf_primitive_type {
+    RF_CYLINDER = 0,
+    RF_FRUSTUM = 1,
+    RF_SPHERE = 2,
+    RF_ANNULUS = 3,
+    RF_RECTANGLE = 4,
+    RF_DISC = 5,
+    RF_NONE = 6
+} rf_primitive_type;
+
+/* Replaces: struct Primitive, RayfactorConstants.h:38-45 and
+ * RayFactorGPU.cl:37-44.  Byte-identical, 112 bytes:
+ *   m     object->world transform, 3x4 row-major
+ *   invm  world->object transform, 3x4 row-major
+ *   util1 frustum top-radius ratio / annulus inner radius squared
+ *   util2 annulus outer radius squared / +-1 normal flip (cylinder, frustum, sphere) */
+typedef struct rf_primitive {
+    float m[12];
+    float invm[12];
+    int32_t type;
+    int32_t rayDensity;
+    float util1;
+    float util2;
+} rf_primitive;
+
+/* Replaces: struct BoundingVolume, RayfactorConstants.h:48-56 and
+ * RayFactorGPU.cl:46-54.  Byte-identical, 72 bytes.  Children are the
+ * primitives [objStartIndex, objEndIndex); settings == 1 tests them as soon
+ * as the bounding surface is hit ("intermediateCheck"). */
+typedef struct rf_bounding_volume {
+    float invm[12];
+    int32_t type;
+    int32_t settings;
+    int32_t objStartIndex;
+    int32_t objEndIndex;
+    float util1;
+    float util2;
+} rf_bounding_volume;
+
+/* ------------------------------------------------------------------ */
+/* Status codes                                                        */
+/* ------------------------------------------------------------------ */
+typedef enum rf_status {
+    RF_OK = 0,
+    RF_ERR_INVALID_ARG = -1,
+    RF_ERR_INVALID_SCENE = -2,   /* unknown type, bad BV child range, n == 0 */
+    RF_ERR_HIP = -3,             /* HIP runtime error (see rf_last_error) */
+    RF_ERR_OUT_OF_MEMORY = -4,
+    RF_ERR_OVERFLOW = -5,        /* int32 overflow of a reference quantity */
+    RF_ERR_IO = -6,              /* file missing / unwritable */
+    RF_ERR_PARSE = -7,           /* malformed scene XML */
+    RF_ERR_CAPACITY = -8,        /* request exceeds a configured limit */
+    RF_ERR_NO_DEVICE = -9
+} rf_status;
+
+/* ------------------------------------------------------------------ */
+/* Solve options                                                       */
+/* ------------------------------------------------------------------ */
+typedef enum rf_math {
+    RF_MATH_COMPAT = 0,   /* FP32, bit-exact reference semantics (pinned IEEE builtins) */
+    RF_MATH_FP64 = 1,     /* FP64 physics semantics: the error oracle form */
+    RF_MATH_FAST = 2      /* FP32 hardware rcp/sqrt, FP64 re-solve of ill-conditioned roots */
+} rf_math;
+
+typedef enum rf_rng {
+    RF_RNG_REFERENCE = 0, /* key {gid, 0xdecafbad}, ctr {1, 0xf00dcafe, 0xdeadbeef, 0xbeeff00d} */
+    RF_RNG_SEEDED = 1     /* key {wi_lo, seed}, ctr {wi_hi, emitter, geometry, 0xbeeff00d} */
+} rf_rng;
+
+typedef enum rf_budget {
+    RF_BUDGET_REFERENCE = 0, /* main.cpp:172-173: every emitter uses primitive 0's rayDensity */
+    RF_BUDGET_EXPLICIT = 1   /* rays_per_emitter (rounded up to whole 4-ray work items) */
+} rf_budget;
+
+typedef enum rf_form {
+    RF_FORM_PER_EMITTER = 0, /* one launch per emitter (the reference's dispatch shape) */
+    RF_FORM_PERSISTENT = 1,  /* one persistent launch fed by a device work queue */
+    RF_FORM_REFERENCE = 2    /* one launch per emitter, 128-wide groups, one global atomic per ray */
+} rf_form;
+
+typedef struct rf_solve_opts {
+    int32_t math;             /* rf_math */
+    int32_t rng;              /* rf_rng */
+    int32_t use_bvs;          /* 0: runGPU semantics, 1: runBVGPU semantics */
+    int32_t budget;           /* rf_budget */
+    int32_t n_samples;        /* RF_BUDGET_REFERENCE: the reference's n_samples */
+    int32_t form;             /* rf_form */
+    int64_t rays_per_emitter; /* RF_BUDGET_EXPLICIT */
+    uint32_t seed;            /* RF_RNG_SEEDED */
+    uint32_t geometry;        /* RF_RNG_SEEDED: geometry id (batched solves use their index) */
+    int32_t refine;           /* RF_MATH_FAST: 1 = FP64 re-solve of flagged roots */
+    int32_t rays_per_thread;  /* 0 = autotuned/default; else 1, 2, 4 or 8 */
+    int32_t block_size;       /* 0 = autotuned/default; else 64..1024, multiple of 64 */
+    int32_t reserved;
+} rf_solve_opts;
+
+/* Fills *opts with defaults: COMPAT math, reference RNG and budget,
+ * n_samples = 1000000, per-emitter form, refine = 1. */
+void rf_solve_opts_default(rf_solve_opts *opts);
+
+/* Measurements of the last solve on a context. */
+typedef struct rf_timing {
+    double kernel_ms;         /* HIP events around the device work, on the library stream */
+    double wall_ms;           /* host steady clock around the whole call */
+    int64_t rays;             /* rays traced */
+    int64_t tests;            /* ray-primitive tests executed (counted on the device for BV) */
+    int64_t flops;            /* algorithmic FP32 flops: sum over tests of F_type */
+    int64_t refined;          /* (ray, primitive) pairs re-solved in FP64 */
+    int64_t launches;         /* kernel launches issued */
+} rf_timing;
+
+/* ------------------------------------------------------------------ */
+/* Library / context                                                   */
+/* ------------------------------------------------------------------ */
+typedef struct rf_ctx rf_ctx;
+typedef struct rf_scene rf_scene;
+
+int32_t rf_abi_version(void);
+const char *rf_status_string(int32_t status);
+
+/* Replaces: opencl_init (util_opencl.h:133-334) -- one HIP device, one stream. */
+int32_t rf_ctx_create(int32_t hip_device, rf_ctx **out);
+int32_t rf_ctx_destroy(rf_ctx *ctx);
+/* Last error text of a context (empty if none).  Valid until the next call. */
+const char *rf_last_error(const rf_ctx *ctx);
+/* Device name and compute-unit count (for run reports). */
+int32_t rf_device_info(const rf_ctx *ctx, char *name, size_t name_len, int32_t *n_cu, int32_t *clock_mhz);
+
+/* Replaces: clCreateBuffer + blocking clEnqueueWriteBuffer of the objects /
+ * bvs / zeroed hits buffers (main.cpp:118-134).  The library copies the
+ * arrays to device memory it owns; the caller's arrays are not retained. */
+int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32_t n_prims,
+                        const rf_bounding_volume *bvs, int32_t n_bvs, rf_scene **out);
+int32_t rf_scene_destroy(rf_scene *scene);
+int32_t rf_scene_size(const rf_scene *scene, int32_t *n_prims, int32_t *n_bvs);
+/* Device bytes held by a scene (primitive layouts plus its n x n int64 tally). */
+int64_t rf_scene_footprint(int32_t n_prims, int32_t n_bvs);
+
+/* ------------------------------------------------------------------ */
+/* Dispatch                                                            */
+/* ------------------------------------------------------------------ */
+
+/* Replaces: main.cpp:172-173, the work items launched per emitter:
+ * round(rayDensity*n_samples/count) to a multiple of wgsize, ties down.
+ * RF_ERR_OVERFLOW when rayDensity*n_samples overflows int32. */
+int32_t rf_nthreads(int32_t ray_density, int32_t n_samples, int32_t count, int32_t wgsize,
+                    int64_t *nthreads);
+
+/* Replaces: clSetKernelArg(kernel, 0, &objNo) + clEnqueueNDRangeKernel
+ * (main.cpp:170-176) of runGPU (RayFactorGPU.cl:481) or runBVGPU (:530):
+ * work items [wi0, wi0 + n_wi) of one emitter, 4 rays each, tallied into the
+ * scene's device hit matrix.  Asynchronous on the context stream. */
+int32_t rf_launch_emitter(rf_ctx *ctx, rf_scene *scene, int32_t emitter, int64_t wi0,
+                          int64_t n_wi, const rf_solve_opts *opts);
+/* Replaces: clFinish (main.cpp:185-188). */
+int32_t rf_sync(rf_ctx *ctx);
+/* Replaces: the blocking clEnqueueReadBuffer of hits (main.cpp:199-205); int64 cells. */
+int32_t rf_hits_read(rf_ctx *ctx, rf_scene *scene, int64_t *hits);
+/* Replaces: the blocking zero write of hits (main.cpp:130-134). */
+int32_t rf_hits_reset(rf_ctx *ctx, rf_scene *scene);
+
+/* Replaces: the whole dispatch of runCL_GPURNG (main.cpp:54-214): every
+ * emitter, the launch geometry, and the readback.  hits: caller-owned n*n
+ * int64 (row = emitter); rays: caller-owned n (rays traced per emitter);
+ * timing may be NULL. */
+int32_t rf_solve(rf_ctx *ctx, rf_scene *scene, const rf_solve_opts *opts,
+                 int64_t *hits, int64_t *rays, rf_timing *timing);
+
+/* Per-ray hit index (-1 = miss) of work items [wi0, wi0+n_wi) of one
+ * emitter: idx_out holds 4*n_wi int32.  No reference counterpart: the
+ * diagnostic behind the differing-ray counts between arithmetic modes. */
+int32_t rf_trace_rays(rf_ctx *ctx, rf_scene *scene, int32_t emitter, int64_t wi0, int64_t n_wi,
+                      const rf_solve_opts *opts, int32_t *idx_out);
+
+/* Replaces: runCL_GPURNG (main.cpp:54-272) with its exact observables:
+ * int32 hits (RF_ERR_OVERFLOW past 2^31-1 instead of wrapping) and
+ * samplesForObj = nthreads*4 (main.cpp:178).  vfm (may be NULL) receives the
+ * reference's float normalisation (main.cpp:216-246). */
+int32_t rf_run_cl_gpurng(rf_ctx *ctx, const rf_primitive *prims, int32_t n_prims,
+                         const rf_bounding_volume *bvs, int32_t n_bvs, int32_t use_bvs,
+                         int32_t n_samples, int32_t *hits, double *samples_for_obj, float *vfm);
+
+/* ------------------------------------------------------------------ */
+/* Batched sweeps (no reference counterpart: furnace design sweeps)    */
+/* ------------------------------------------------------------------ */
+
+/* Solves n_geoms independent scenes.  RF_FORM_PERSISTENT runs all their
+ * (geometry, emitter, ray-chunk) work units in one persistent launch fed by
+ * a device work queue (captured in a HIP graph); RF_FORM_PER_EMITTER /
+ * RF_FORM_REFERENCE issue one launch per emitter of each geometry.
+ * hits[g] is a caller-owned n_g*n_g int64 matrix (may be NULL to skip the
+ * readback); rays[g] (may be NULL) receives n_g int64.  With RF_RNG_SEEDED,
+ * geometry g uses geometry id opts->geometry + g. */
+int32_t rf_batch_solve(rf_ctx *ctx, rf_scene *const *scenes, int32_t n_geoms,
+                       const rf_solve_opts *opts, int64_t *const *hits, int64_t *const *rays,
+                       rf_timing *timing);
+
+/* Autotuner: times the FAST kernel over {rays per thread} x {block size} on
+ * the given scenes (a bounded sample of their work) and stores the fastest
+ * choice in the context, where later solves with rays_per_thread == 0 and
+ * block_size == 0 use it.  best_rpt / best_block may be NULL. */
+int32_t rf_autotune(rf_ctx *ctx, rf_scene *const *scenes, int32_t n_geoms,
+                    const rf_solve_opts *opts, int32_t *best_rpt, int32_t *best_block);
+
+/* ------------------------------------------------------------------ */
+/* Streaming solves (resident scenes, HBM-footprint admission)         */
+/* ------------------------------------------------------------------ */
+typedef struct rf_stream rf_stream;
+
+/* A service that keeps scenes and tallies resident on the device and batches
+ * queued requests of unequal size into persistent launches.  Admission
+ * rejects (RF_ERR_CAPACITY) a request whose scene footprint would push the
+ * resident total past hbm_limit_bytes. */
+int32_t rf_stream_create(rf_ctx *ctx, int64_t hbm_limit_bytes, rf_stream **out);
+int32_t rf_stream_destroy(rf_stream *st);
+/* Enqueues one solve; the primitives are copied.  *request_id identifies it. */
+int32_t rf_stream_submit(rf_stream *st, const rf_primitive *prims, int32_t n_prims,
+                         const rf_solve_opts *opts, int64_t *request_id);
+/* Runs every queued request in one persistent launch; returns how many ran. */
+int32_t rf_stream_flush(rf_stream *st, int32_t *n_completed);
+/* Copies a completed request's n*n hits (and per-emitter rays) out and
+ * releases its device memory.  RF_ERR_INVALID_ARG if not completed. */
+int32_t rf_stream_collect(rf_stream *st, int64_t request_id, int64_t *hits, int64_t *rays,
+                          double *latency_ms);
+int64_t rf_stream_resident_bytes(const rf_stream *st);
+
+/* ------------------------------------------------------------------ */
+/* Host-side scene input and result output                             */
+/* ------------------------------------------------------------------ */
+
+/* A scene read from the reference XML format. */
+typedef struct rf_host_scene {
+    rf_primitive *prims;
+    int32_t n_prims;
+    rf_bounding_volume *bvs;
+    int32_t n_bvs;
+    float globalRayDensity;
+    int32_t useBoundingVolumes;
+} rf_host_scene;
+
+/* Replaces: Scene::readScene (RayFactorCL/Scene.cpp:24-452): settings,
+ * primitives, bounding volumes, affine transforms and per-type rayDensity,
+ * producing the same bytes (fields the reference leaves uninitialised are
+ * zero).  Free with rf_host_scene_free. */
+int32_t rf_scene_load_xml(const char *path, rf_host_scene *out);
+/* The same from an in-memory XML document. */
+int32_t rf_scene_parse_xml(const char *text, size_t len, rf_host_scene *out);
+void rf_host_scene_free(rf_host_scene *scene);
+
+/* Replaces: the output.txt writer (main.cpp:236-251): row i is
+ * float(hits[i][j]) / samples[i] printed with precision 8 and a trailing
+ * comma after every value. */
+int32_t rf_write_output(const char *path, int32_t n, const int64_t *hits, const double *samples);
+/* FP64 normalisation: vf[i][j] = hits[i][j] / rays[i] (0 where rays[i] == 0). */
+int32_t rf_normalise(int32_t n, const int64_t *hits, const int64_t *rays, double *vf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RAYFACTOR_H */

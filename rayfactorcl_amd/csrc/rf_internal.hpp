@@ -1,0 +1,104 @@
+// rf_internal.hpp -- structures shared by the kernels and the host runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rayfactor.h"
+#include "rf_device.hpp"
+
+namespace rf {
+
+// Fast-path type segments, in traversal order.
+enum SegKind { SEG_CYL = 0, SEG_FRUSTUM = 1, SEG_SPHERE = 2, SEG_RING = 3, SEG_RECT = 4, N_SEG = 5 };
+
+// Per-BV metadata of the fast / fp64 BV kernels.
+struct BvMeta {
+    int32_t settings;
+    int32_t start;
+    int32_t end;
+    int32_t type;
+};
+
+// Device-side descriptor of one geometry (all pointers are device memory).
+struct GeomDesc {
+    const rf_primitive *prims;          // caller's order (AoS, 112 B)
+    const rfd::FastPrim *fast;          // type-segmented traversal order
+    const rfd::FastPrim *fast_orig;     // caller's order (BV child scans)
+    const rf_bounding_volume *bvs;      // caller's BVs (AoS, 72 B)
+    const rfd::FastPrim *fast_bvs;      // BV hit records
+    const BvMeta *bv_meta;
+    unsigned long long *hits;           // n x n tally, row = emitter
+    int32_t n;
+    int32_t n_bvs;
+    int32_t seg_begin[N_SEG];
+    int32_t seg_end[N_SEG];
+    uint32_t geometry_id;               // seeded RNG geometry word
+    int32_t pad;
+};
+
+// One work unit: work items [wi0, wi0+n_wi) of one emitter of one geometry.
+struct WorkUnit {
+    int32_t geom;
+    int32_t emitter;
+    int64_t wi0;
+    int64_t n_wi;
+    int64_t pad;
+};
+
+struct KParams {
+    const GeomDesc *geoms;
+    const WorkUnit *units;              // nullptr: implicit unit list (below)
+    unsigned int *queue_head;           // atomic unit counter (zeroed before launch)
+    unsigned long long *stats;          // [0] tests executed (BV kernels), [1] FP64 re-solves
+    int32_t *trace_out;                 // per-ray hit index: ray 4*(wi - trace_wi0) + v
+    int64_t trace_wi0;
+    int32_t n_units;
+    int32_t rng;
+    uint32_t seed;
+    int32_t refine;
+    int32_t bins;                       // LDS tally bins (0 = global atomics per ray)
+    // Implicit unit list (units == nullptr): unit u is work items
+    // [imp_wi0 + u*imp_unit, min(imp_wi0 + (u+1)*imp_unit, imp_wi0 + imp_n_wi))
+    // of emitter imp_emitter of geoms[0], seeded with geometry word imp_geom_id.
+    int32_t imp_emitter;
+    uint32_t imp_geom_id;
+    int64_t imp_wi0;
+    int64_t imp_n_wi;
+    int64_t imp_unit;
+};
+
+enum KMath { KM_COMPAT = 0, KM_FP64 = 1, KM_FAST = 2 };
+
+struct LaunchCfg {
+    int math;        // KMath
+    int use_bv;
+    int trace;
+    int nr;          // rays per thread (fast plain): 1, 2, 4, 8
+    int refine;
+    int block;       // threads per workgroup
+    int grid;        // workgroups
+    int direct;      // 1: reference form (one work item per thread, global atomics, no queue)
+    size_t lds_bytes;
+};
+
+hipError_t launch_solve(const LaunchCfg &cfg, const KParams &kp, hipStream_t stream);
+hipError_t launch_fill_u32(unsigned int *p, unsigned int value, size_t count, hipStream_t stream);
+hipError_t launch_fill_u64(unsigned long long *p, size_t count, hipStream_t stream);
+
+// Algorithmic FP32 flops per ray-primitive test (counted from the reference
+// formulas: FMA = 2, mul/add = 1, rcp/sqrt = 1, compare/select = 0).
+inline int64_t flops_per_test(int type)
+{
+    switch (type) {
+    case RF_RECTANGLE: return 39;
+    case RF_DISC: return 42;
+    case RF_ANNULUS: return 42;
+    case RF_CYLINDER: return 56;
+    case RF_FRUSTUM: return 65;
+    case RF_SPHERE: return 58;
+    default: return 0;
+    }
+}
+
+}  // namespace rf

@@ -1,0 +1,948 @@
+// rf_runtime.cpp -- host side of the C ABI (include/rayfactor.h): contexts,
+// device scenes, the dispatch that replaces runCL_GPURNG's OpenCL launches
+// (RayFactorCL/main.cpp:54-251), batched persistent solves, the autotuner
+// and the streaming service.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "rayfactor.h"
+#include "rf_internal.hpp"
+
+using namespace rf;
+using rfd::FastPrim;
+
+// ---------------------------------------------------------------------------
+// Context and scene objects
+// ---------------------------------------------------------------------------
+struct rf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    char name[256] = {0};
+    int n_cu = 0;
+    int clock_mhz = 0;
+    int tuned_rpt = 0, tuned_block = 0;
+    // scratch device buffers
+    GeomDesc *d_geoms = nullptr; size_t geoms_cap = 0;
+    WorkUnit *d_units = nullptr; size_t units_cap = 0;
+    unsigned *d_queue = nullptr; size_t queue_cap = 0; size_t queue_next = 0;
+    unsigned long long *d_stats = nullptr;
+    // cached HIP graph of the last persistent batched solve
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uintptr_t> graph_key;
+};
+
+struct rf_scene {
+    rf_ctx *ctx = nullptr;
+    int n = 0, n_bvs = 0;
+    std::vector<rf_primitive> prims;
+    std::vector<rf_bounding_volume> bvs;
+    rf_primitive *d_prims = nullptr;
+    FastPrim *d_fast = nullptr, *d_fast_orig = nullptr, *d_fast_bvs = nullptr;
+    rf_bounding_volume *d_bvs = nullptr;
+    BvMeta *d_bv_meta = nullptr;
+    unsigned long long *d_hits = nullptr;
+    GeomDesc *d_desc = nullptr;          // single-geometry descriptor
+    GeomDesc desc{};                     // host copy
+    int64_t flops_per_ray = 0;           // sum over primitives of F_type (plain kernels)
+    double flops_per_test_avg = 0.0;     // BV accounting
+};
+
+static thread_local std::string g_err;
+
+static int set_err(rf_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg; else g_err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                      \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return set_err(ctx, RF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" int32_t rf_abi_version(void) { return RF_ABI_VERSION; }
+
+extern "C" const char *rf_status_string(int32_t s)
+{
+    switch (s) {
+    case RF_OK: return "ok";
+    case RF_ERR_INVALID_ARG: return "invalid argument";
+    case RF_ERR_INVALID_SCENE: return "invalid scene";
+    case RF_ERR_HIP: return "HIP runtime error";
+    case RF_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case RF_ERR_OVERFLOW: return "integer overflow";
+    case RF_ERR_IO: return "I/O error";
+    case RF_ERR_PARSE: return "scene parse error";
+    case RF_ERR_CAPACITY: return "capacity exceeded";
+    case RF_ERR_NO_DEVICE: return "no such HIP device";
+    default: return "unknown status";
+    }
+}
+
+extern "C" void rf_solve_opts_default(rf_solve_opts *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->math = RF_MATH_COMPAT;
+    o->rng = RF_RNG_REFERENCE;
+    o->use_bvs = 0;
+    o->budget = RF_BUDGET_REFERENCE;
+    o->n_samples = 1000000;
+    o->form = RF_FORM_PER_EMITTER;
+    o->refine = 1;
+}
+
+extern "C" const char *rf_last_error(const rf_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+extern "C" int32_t rf_ctx_create(int32_t dev, rf_ctx **out)
+{
+    if (!out) return set_err(nullptr, RF_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return set_err(nullptr, RF_ERR_NO_DEVICE, "no HIP device visible");
+    if (dev < 0 || dev >= count) return set_err(nullptr, RF_ERR_NO_DEVICE, "device index out of range");
+    rf_ctx *c = new rf_ctx;
+    c->device = dev;
+    hipError_t e = hipSetDevice(dev);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+    if (e == hipSuccess) e = hipMalloc((void **)&c->d_stats, 4 * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        g_err = std::string("context creation failed: ") + hipGetErrorString(e);
+        delete c;
+        return RF_ERR_HIP;
+    }
+    std::snprintf(c->name, sizeof c->name, "%s", prop.gcnArchName);
+    c->n_cu = prop.multiProcessorCount;
+    c->clock_mhz = prop.clockRate / 1000;
+    *out = c;
+    return RF_OK;
+}
+
+extern "C" int32_t rf_ctx_destroy(rf_ctx *c)
+{
+    if (!c) return RF_ERR_INVALID_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    (void)hipFree(c->d_geoms);
+    (void)hipFree(c->d_units);
+    (void)hipFree(c->d_queue);
+    (void)hipFree(c->d_stats);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return RF_OK;
+}
+
+extern "C" int32_t rf_device_info(const rf_ctx *c, char *name, size_t len, int32_t *n_cu, int32_t *clock_mhz)
+{
+    if (!c) return RF_ERR_INVALID_ARG;
+    if (name && len) std::snprintf(name, len, "%s", c->name);
+    if (n_cu) *n_cu = c->n_cu;
+    if (clock_mhz) *clock_mhz = c->clock_mhz;
+    return RF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Scene upload
+// ---------------------------------------------------------------------------
+static bool valid_type(int32_t t) { return t >= RF_CYLINDER && t < RF_NONE; }
+
+static FastPrim make_fast(const float invm[12], int32_t type, float u1, float u2, int32_t orig)
+{
+    FastPrim f;
+    std::memcpy(f.r, invm, sizeof f.r);
+    f.k0 = 0.f; f.k1 = 0.f; f.orig = orig; f.type = type;
+    switch (type) {
+    case RF_CYLINDER:
+        f.r[11] = invm[11] - 0.5f;                       // accept |z - 1/2| <= 1/2
+        break;
+    case RF_FRUSTUM: {
+        const float k = u1 - 1.0f;                       // w = 1 + (u1-1) z: cone radius
+        for (int j = 0; j < 4; ++j) f.r[8 + j] = k * invm[8 + j];
+        f.r[11] += 1.0f;
+        f.k0 = 0.5f * (1.0f + u1);
+        f.k1 = 0.5f * std::fabs(u1 - 1.0f);
+        break;
+    }
+    case RF_DISC: f.k0 = 0.f; f.k1 = 1.f; break;
+    case RF_ANNULUS: f.k0 = u1; f.k1 = u2; break;
+    default: break;
+    }
+    return f;
+}
+
+static int seg_of(int32_t type)
+{
+    switch (type) {
+    case RF_CYLINDER: return SEG_CYL;
+    case RF_FRUSTUM: return SEG_FRUSTUM;
+    case RF_SPHERE: return SEG_SPHERE;
+    case RF_RECTANGLE: return SEG_RECT;
+    default: return SEG_RING;
+    }
+}
+
+extern "C" int64_t rf_scene_footprint(int32_t n, int32_t nbv)
+{
+    if (n < 0 || nbv < 0) return -1;
+    return (int64_t)n * (sizeof(rf_primitive) + 2 * sizeof(FastPrim)) +
+           (int64_t)nbv * (sizeof(rf_bounding_volume) + sizeof(FastPrim) + sizeof(BvMeta)) +
+           (int64_t)n * n * 8 + (int64_t)sizeof(GeomDesc);
+}
+
+static void free_scene_mem(rf_scene *s)
+{
+    (void)hipFree(s->d_prims); (void)hipFree(s->d_fast); (void)hipFree(s->d_fast_orig);
+    (void)hipFree(s->d_fast_bvs); (void)hipFree(s->d_bvs); (void)hipFree(s->d_bv_meta);
+    (void)hipFree(s->d_hits); (void)hipFree(s->d_desc);
+}
+
+static int validate_scene(const rf_primitive *prims, int32_t n, const rf_bounding_volume *bvs, int32_t nbv,
+                          std::string &why)
+{
+    if (n <= 0 || !prims) { why = "scene has no primitives"; return RF_ERR_INVALID_SCENE; }
+    if (n > (1 << 20)) { why = "more than 2^20 primitives"; return RF_ERR_CAPACITY; }
+    if (nbv < 0 || (nbv > 0 && !bvs)) { why = "bad bounding-volume array"; return RF_ERR_INVALID_ARG; }
+    for (int32_t i = 0; i < n; ++i)
+        if (!valid_type(prims[i].type)) { why = "primitive " + std::to_string(i) + " has an unknown type"; return RF_ERR_INVALID_SCENE; }
+    for (int32_t i = 0; i < nbv; ++i) {
+        if (!valid_type(bvs[i].type)) { why = "bounding volume " + std::to_string(i) + " has an unknown type"; return RF_ERR_INVALID_SCENE; }
+        if (bvs[i].objStartIndex < 0 || bvs[i].objEndIndex > n || bvs[i].objStartIndex > bvs[i].objEndIndex) {
+            why = "bounding volume " + std::to_string(i) + " has an invalid child range";
+            return RF_ERR_INVALID_SCENE;
+        }
+    }
+    return RF_OK;
+}
+
+extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32_t n, const rf_bounding_volume *bvs,
+                                   int32_t nbv, rf_scene **out)
+{
+    if (!ctx || !out) return set_err(ctx, RF_ERR_INVALID_ARG, "NULL context or output");
+    *out = nullptr;
+    std::string why;
+    int rc = validate_scene(prims, n, bvs, nbv, why);
+    if (rc) return set_err(ctx, rc, why);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    rf_scene *s = new rf_scene;
+    s->ctx = ctx; s->n = n; s->n_bvs = nbv;
+    s->prims.assign(prims, prims + n);
+    if (nbv) s->bvs.assign(bvs, bvs + nbv);
+
+    // type-segmented traversal order (stable within a type)
+    std::vector<FastPrim> fast, fast_orig(n), fast_bvs(nbv);
+    std::vector<BvMeta> meta(nbv);
+    int32_t seg_begin[N_SEG], seg_end[N_SEG];
+    for (int k = 0; k < N_SEG; ++k) {
+        seg_begin[k] = (int32_t)fast.size();
+        for (int32_t i = 0; i < n; ++i)
+            if (seg_of(prims[i].type) == k)
+                fast.push_back(make_fast(prims[i].invm, prims[i].type, prims[i].util1, prims[i].util2, i));
+        seg_end[k] = (int32_t)fast.size();
+    }
+    int64_t fsum = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        fast_orig[i] = make_fast(prims[i].invm, prims[i].type, prims[i].util1, prims[i].util2, i);
+        fsum += flops_per_test(prims[i].type);
+    }
+    int64_t fbv = 0;
+    for (int32_t i = 0; i < nbv; ++i) {
+        fast_bvs[i] = make_fast(bvs[i].invm, bvs[i].type, bvs[i].util1, bvs[i].util2, i);
+        meta[i] = BvMeta{bvs[i].settings, bvs[i].objStartIndex, bvs[i].objEndIndex, bvs[i].type};
+        fbv += flops_per_test(bvs[i].type);
+    }
+    s->flops_per_ray = fsum;
+    s->flops_per_test_avg = (double)(fsum + fbv) / (double)(n + nbv);
+
+    hipError_t e = hipSuccess;
+    auto up = [&](void **dst, const void *src, size_t bytes) {
+        if (e != hipSuccess || bytes == 0) return;
+        e = hipMalloc(dst, bytes);
+        if (e == hipSuccess) e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+    };
+    up((void **)&s->d_prims, prims, sizeof(rf_primitive) * n);
+    up((void **)&s->d_fast, fast.data(), sizeof(FastPrim) * n);
+    up((void **)&s->d_fast_orig, fast_orig.data(), sizeof(FastPrim) * n);
+    if (nbv) {
+        up((void **)&s->d_bvs, bvs, sizeof(rf_bounding_volume) * nbv);
+        up((void **)&s->d_fast_bvs, fast_bvs.data(), sizeof(FastPrim) * nbv);
+        up((void **)&s->d_bv_meta, meta.data(), sizeof(BvMeta) * nbv);
+    }
+    if (e == hipSuccess) e = hipMalloc((void **)&s->d_hits, (size_t)n * n * sizeof(unsigned long long));
+    if (e == hipSuccess) e = launch_fill_u64(s->d_hits, (size_t)n * n, ctx->stream);
+    GeomDesc &d = s->desc;
+    d.prims = s->d_prims; d.fast = s->d_fast; d.fast_orig = s->d_fast_orig;
+    d.bvs = s->d_bvs; d.fast_bvs = s->d_fast_bvs; d.bv_meta = s->d_bv_meta;
+    d.hits = s->d_hits; d.n = n; d.n_bvs = nbv; d.geometry_id = 0;
+    for (int k = 0; k < N_SEG; ++k) { d.seg_begin[k] = seg_begin[k]; d.seg_end[k] = seg_end[k]; }
+    up((void **)&s->d_desc, &d, sizeof d);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        const int code = (e == hipErrorOutOfMemory) ? RF_ERR_OUT_OF_MEMORY : RF_ERR_HIP;
+        set_err(ctx, code, std::string("scene upload failed: ") + hipGetErrorString(e));
+        free_scene_mem(s);
+        delete s;
+        return code;
+    }
+    *out = s;
+    return RF_OK;
+}
+
+extern "C" int32_t rf_scene_destroy(rf_scene *s)
+{
+    if (!s) return RF_ERR_INVALID_ARG;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    free_scene_mem(s);
+    delete s;
+    return RF_OK;
+}
+
+extern "C" int32_t rf_scene_size(const rf_scene *s, int32_t *n, int32_t *nbv)
+{
+    if (!s) return RF_ERR_INVALID_ARG;
+    if (n) *n = s->n;
+    if (nbv) *nbv = s->n_bvs;
+    return RF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launch geometry (main.cpp:101,172-173)
+// ---------------------------------------------------------------------------
+extern "C" int32_t rf_nthreads(int32_t ray_density, int32_t n_samples, int32_t count, int32_t wgsize, int64_t *out)
+{
+    if (!out || count <= 0 || wgsize <= 0) return RF_ERR_INVALID_ARG;
+    const int64_t prod = (int64_t)ray_density * (int64_t)n_samples;
+    if (prod > INT32_MAX || prod < INT32_MIN) return RF_ERR_OVERFLOW;
+    const int64_t nt = prod / count;          // int arithmetic in the reference
+    if (nt < 0) return RF_ERR_OVERFLOW;       // would wrap through size_t
+    const int64_t rem = nt % wgsize;
+    *out = rem > wgsize / 2 ? nt + wgsize - rem : nt - rem;
+    return RF_OK;
+}
+
+static int check_opts(rf_ctx *ctx, const rf_solve_opts *o)
+{
+    if (!o) return set_err(ctx, RF_ERR_INVALID_ARG, "opts is NULL");
+    if (o->math < RF_MATH_COMPAT || o->math > RF_MATH_FAST) return set_err(ctx, RF_ERR_INVALID_ARG, "bad math mode");
+    if (o->rng != RF_RNG_REFERENCE && o->rng != RF_RNG_SEEDED) return set_err(ctx, RF_ERR_INVALID_ARG, "bad rng mode");
+    if (o->budget != RF_BUDGET_REFERENCE && o->budget != RF_BUDGET_EXPLICIT) return set_err(ctx, RF_ERR_INVALID_ARG, "bad budget mode");
+    if (o->form < RF_FORM_PER_EMITTER || o->form > RF_FORM_REFERENCE) return set_err(ctx, RF_ERR_INVALID_ARG, "bad form");
+    if (o->budget == RF_BUDGET_EXPLICIT && o->rays_per_emitter < 0) return set_err(ctx, RF_ERR_INVALID_ARG, "negative ray budget");
+    if (o->rays_per_thread && o->rays_per_thread != 1 && o->rays_per_thread != 2 && o->rays_per_thread != 4 &&
+        o->rays_per_thread != 8)
+        return set_err(ctx, RF_ERR_INVALID_ARG, "rays_per_thread must be 0, 1, 2, 4 or 8");
+    if (o->block_size && (o->block_size < 64 || o->block_size > 256 || o->block_size % 64))
+        return set_err(ctx, RF_ERR_INVALID_ARG, "block_size must be 0 or a multiple of 64 in [64, 256]");
+    return RF_OK;
+}
+
+// Work items per emitter of a scene under the options' budget.
+static int emitter_budget(rf_ctx *ctx, const rf_scene *s, const rf_solve_opts *o, std::vector<int64_t> &n_wi)
+{
+    n_wi.assign(s->n, 0);
+    if (o->budget == RF_BUDGET_REFERENCE) {
+        // every emitter uses objects[i].rayDensity with i = the device index 0 (main.cpp:172)
+        int64_t nt;
+        int rc = rf_nthreads(s->prims[0].rayDensity, o->n_samples, 4, 128, &nt);
+        if (rc) return set_err(ctx, rc, "rayDensity * n_samples overflows int32 (main.cpp:172)");
+        if (nt > INT32_MAX) return set_err(ctx, RF_ERR_OVERFLOW, "work items exceed the reference's int gid");
+        std::fill(n_wi.begin(), n_wi.end(), nt);
+    } else {
+        const int64_t wi = (o->rays_per_emitter + 3) / 4;
+        if (o->rng == RF_RNG_REFERENCE && wi > ((int64_t)1 << 32))
+            return set_err(ctx, RF_ERR_OVERFLOW, "reference RNG keys work items with 32 bits");
+        std::fill(n_wi.begin(), n_wi.end(), wi);
+    }
+    return RF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch
+// ---------------------------------------------------------------------------
+static constexpr int kMaxLdsBins = 16384;   // 64 KiB of 32-bit bins per workgroup
+
+static void pick_shape(const rf_ctx *ctx, const rf_solve_opts *o, int &nr, int &block)
+{
+    nr = 1;
+    block = 256;
+    if (o->math == RF_MATH_FAST && !o->use_bvs) {
+        nr = o->rays_per_thread ? o->rays_per_thread : (ctx->tuned_rpt ? ctx->tuned_rpt : 4);
+        block = o->block_size ? o->block_size : (ctx->tuned_block ? ctx->tuned_block : 256);
+    } else if (o->block_size) {
+        block = o->block_size;
+    }
+}
+
+static int ensure_queue(rf_ctx *ctx, size_t need)
+{
+    if (ctx->queue_cap >= need) return RF_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->d_queue);
+    size_t cap = std::max<size_t>(need, 4096);
+    HIPCHK(ctx, hipMalloc((void **)&ctx->d_queue, cap * sizeof(unsigned)));
+    ctx->queue_cap = cap;
+    ctx->queue_next = 0;
+    return RF_OK;
+}
+
+// One queue counter, zeroed on the stream right before the launch that uses it.
+static int next_counter(rf_ctx *ctx, unsigned **slot)
+{
+    int rc = ensure_queue(ctx, 4096);
+    if (rc) return rc;
+    if (ctx->queue_next >= ctx->queue_cap) ctx->queue_next = 0;
+    *slot = ctx->d_queue + ctx->queue_next++;
+    HIPCHK(ctx, launch_fill_u32(*slot, 0u, 1, ctx->stream));
+    return RF_OK;
+}
+
+static int max_groups(const rf_ctx *ctx, int block)
+{
+    // at least 4 resident groups per CU keeps every SIMD busy with 256-thread groups
+    const int per_cu = std::max(2, 2048 / block);
+    return ctx->n_cu * per_cu;
+}
+
+// Implicit-unit launch over work items [wi0, wi0+n_wi) of one emitter.
+static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int64_t n_wi,
+                        const rf_solve_opts *o, uint32_t geom_id, int32_t *trace_out, int64_t trace_wi0,
+                        int64_t *launches)
+{
+    if (n_wi <= 0) return RF_OK;
+    int nr, block;
+    pick_shape(ctx, o, nr, block);
+    KParams kp{};
+    kp.geoms = s->d_desc;
+    kp.units = nullptr;
+    kp.stats = ctx->d_stats;
+    kp.trace_out = trace_out;
+    kp.trace_wi0 = trace_wi0;
+    kp.rng = o->rng;
+    kp.seed = o->seed;
+    kp.refine = o->refine;
+    kp.bins = s->n <= kMaxLdsBins ? s->n : 0;
+    kp.imp_emitter = emitter;
+    kp.imp_geom_id = geom_id;
+    kp.imp_wi0 = wi0;
+    kp.imp_n_wi = n_wi;
+    LaunchCfg c{};
+    c.math = o->math == RF_MATH_COMPAT ? KM_COMPAT : (o->math == RF_MATH_FP64 ? KM_FP64 : KM_FAST);
+    c.use_bv = o->use_bvs && s->n_bvs > 0;
+    c.trace = trace_out != nullptr;
+    c.nr = nr;
+    c.refine = o->refine;
+    if (o->form == RF_FORM_REFERENCE && !trace_out) {
+        // main.cpp:172-176: global = nthreads, local = 128
+        c.direct = 1;
+        c.block = 128;
+        const int64_t grid = (n_wi + 127) / 128;
+        if (grid > INT32_MAX) return set_err(ctx, RF_ERR_CAPACITY, "grid too large");
+        c.grid = (int)grid;
+        c.lds_bytes = 0;
+        HIPCHK(ctx, launch_solve(c, kp, ctx->stream));
+        ++*launches;
+        return RF_OK;
+    }
+    // queue form: units of at least one pass of the workgroup, sized to give
+    // every resident group about one unit
+    const int64_t pass_wi = std::max<int64_t>(1, (int64_t)block * (nr >= 4 ? nr / 4 : 1));
+    const int groups = max_groups(ctx, block);
+    int64_t unit = (n_wi + groups - 1) / groups;
+    unit = ((unit + pass_wi - 1) / pass_wi) * pass_wi;
+    unit = std::max(unit, pass_wi);
+    const int64_t n_units = (n_wi + unit - 1) / unit;
+    kp.imp_unit = unit;
+    kp.n_units = (int32_t)n_units;
+    int rc = next_counter(ctx, &kp.queue_head);
+    if (rc) return rc;
+    c.direct = 0;
+    c.block = block;
+    c.grid = (int)std::min<int64_t>(n_units, groups);
+    c.lds_bytes = (c.trace || kp.bins == 0) ? 0 : (size_t)kp.bins * sizeof(uint32_t);
+    HIPCHK(ctx, launch_solve(c, kp, ctx->stream));
+    ++*launches;
+    return RF_OK;
+}
+
+extern "C" int32_t rf_launch_emitter(rf_ctx *ctx, rf_scene *s, int32_t emitter, int64_t wi0, int64_t n_wi,
+                                     const rf_solve_opts *o)
+{
+    if (!ctx || !s) return set_err(ctx, RF_ERR_INVALID_ARG, "NULL context or scene");
+    int rc = check_opts(ctx, o);
+    if (rc) return rc;
+    if (emitter < 0 || emitter >= s->n) return set_err(ctx, RF_ERR_INVALID_ARG, "emitter out of range");
+    if (wi0 < 0 || n_wi < 0) return set_err(ctx, RF_ERR_INVALID_ARG, "negative work-item range");
+    if (o->rng == RF_RNG_REFERENCE && wi0 + n_wi > ((int64_t)1 << 32))
+        return set_err(ctx, RF_ERR_OVERFLOW, "reference RNG keys work items with 32 bits");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int64_t launches = 0;
+    return launch_range(ctx, s, emitter, wi0, n_wi, o, o->geometry, nullptr, 0, &launches);
+}
+
+extern "C" int32_t rf_sync(rf_ctx *ctx)
+{
+    if (!ctx) return RF_ERR_INVALID_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return RF_OK;
+}
+
+static int read_hits(rf_ctx *ctx, rf_scene *s, int64_t *hits)
+{
+    HIPCHK(ctx, hipMemcpyAsync(hits, s->d_hits, (size_t)s->n * s->n * sizeof(int64_t), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return RF_OK;
+}
+
+extern "C" int32_t rf_hits_read(rf_ctx *ctx, rf_scene *s, int64_t *hits)
+{
+    if (!ctx || !s || !hits) return set_err(ctx, RF_ERR_INVALID_ARG, "NULL argument");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    return read_hits(ctx, s, hits);
+}
+
+extern "C" int32_t rf_hits_reset(rf_ctx *ctx, rf_scene *s)
+{
+    if (!ctx || !s) return set_err(ctx, RF_ERR_INVALID_ARG, "NULL argument");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, launch_fill_u64(s->d_hits, (size_t)s->n * s->n, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return RF_OK;
+}
+
+static double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static int read_stats(rf_ctx *ctx, unsigned long long st[2])
+{
+    HIPCHK(ctx, hipMemcpy(st, ctx->d_stats, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    return RF_OK;
+}
+
+// Accounting of one geometry's solve.
+static void account(const rf_scene *s, const rf_solve_opts *o, const std::vector<int64_t> &n_wi, rf_timing &t)
+{
+    int64_t rays = 0;
+    for (int64_t w : n_wi) rays += 4 * w;
+    t.rays += rays;
+    if (!(o->use_bvs && s->n_bvs > 0)) {
+        t.tests += rays * s->n;
+        t.flops += rays * s->flops_per_ray;
+    }
+}
+
+// Persistent batched solve over several scenes (and the single-scene
+// persistent form).
+static int solve_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const rf_solve_opts *o,
+                            const std::vector<std::vector<int64_t>> &budgets, int64_t *launches)
+{
+    int nr, block;
+    pick_shape(ctx, o, nr, block);
+    const bool use_bv = o->use_bvs != 0;
+    // geometry descriptors
+    std::vector<GeomDesc> descs(n_geoms);
+    int max_n = 0;
+    int64_t total_wi = 0;
+    for (int g = 0; g < n_geoms; ++g) {
+        descs[g] = scenes[g]->desc;
+        descs[g].geometry_id = o->geometry + (uint32_t)g;
+        max_n = std::max(max_n, scenes[g]->n);
+        for (int64_t w : budgets[g]) total_wi += w;
+    }
+    const int groups = max_groups(ctx, block);
+    const int64_t pass_wi = std::max<int64_t>(1, (int64_t)block * (nr >= 4 ? nr / 4 : 1));
+    // ~8 units per resident group for balance, each at most 16384 work items
+    int64_t target = total_wi / ((int64_t)groups * 8);
+    target = std::max<int64_t>(pass_wi, std::min<int64_t>(target, 16384));
+    target = ((target + pass_wi - 1) / pass_wi) * pass_wi;
+    std::vector<WorkUnit> units;
+    for (int g = 0; g < n_geoms; ++g)
+        for (int e = 0; e < scenes[g]->n; ++e) {
+            const int64_t nw = budgets[g][e];
+            for (int64_t w0 = 0; w0 < nw; w0 += target)
+                units.push_back(WorkUnit{g, e, w0, std::min(target, nw - w0), 0});
+        }
+    if (units.empty()) return RF_OK;
+    if (units.size() > (size_t)INT32_MAX) return set_err(ctx, RF_ERR_CAPACITY, "too many work units");
+    // largest units first: cost ~ work items x primitives
+    std::stable_sort(units.begin(), units.end(), [&](const WorkUnit &a, const WorkUnit &b) {
+        return a.n_wi * scenes[a.geom]->n > b.n_wi * scenes[b.geom]->n;
+    });
+    if (ctx->geoms_cap < (size_t)n_geoms) {
+        (void)hipFree(ctx->d_geoms);
+        HIPCHK(ctx, hipMalloc((void **)&ctx->d_geoms, n_geoms * sizeof(GeomDesc)));
+        ctx->geoms_cap = n_geoms;
+    }
+    if (ctx->units_cap < units.size()) {
+        (void)hipFree(ctx->d_units);
+        HIPCHK(ctx, hipMalloc((void **)&ctx->d_units, units.size() * sizeof(WorkUnit)));
+        ctx->units_cap = units.size();
+    }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_geoms, descs.data(), n_geoms * sizeof(GeomDesc), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_units, units.data(), units.size() * sizeof(WorkUnit), hipMemcpyHostToDevice, ctx->stream));
+    KParams kp{};
+    kp.geoms = ctx->d_geoms;
+    kp.units = ctx->d_units;
+    kp.stats = ctx->d_stats;
+    kp.n_units = (int32_t)units.size();
+    kp.rng = o->rng;
+    kp.seed = o->seed;
+    kp.refine = o->refine;
+    kp.bins = max_n <= kMaxLdsBins ? max_n : 0;
+    int rc = next_counter(ctx, &kp.queue_head);
+    if (rc) return rc;
+    LaunchCfg c{};
+    c.math = o->math == RF_MATH_COMPAT ? KM_COMPAT : (o->math == RF_MATH_FP64 ? KM_FP64 : KM_FAST);
+    c.use_bv = use_bv;
+    c.nr = nr;
+    c.refine = o->refine;
+    c.block = block;
+    c.grid = (int)std::min<int64_t>((int64_t)units.size(), groups);
+    c.lds_bytes = kp.bins ? (size_t)kp.bins * sizeof(uint32_t) : 0;
+    HIPCHK(ctx, launch_solve(c, kp, ctx->stream));
+    ++*launches;
+    return RF_OK;
+}
+
+static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const rf_solve_opts *o,
+                      int64_t *const *hits, int64_t *const *rays, rf_timing *timing)
+{
+    const double w0 = now_ms();
+    int rc = check_opts(ctx, o);
+    if (rc) return rc;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    std::vector<std::vector<int64_t>> budgets(n_geoms);
+    for (int g = 0; g < n_geoms; ++g) {
+        if (!scenes[g] || scenes[g]->ctx != ctx) return set_err(ctx, RF_ERR_INVALID_ARG, "scene of another context");
+        rc = emitter_budget(ctx, scenes[g], o, budgets[g]);
+        if (rc) return rc;
+    }
+    // hit matrices and statistics start at zero
+    for (int g = 0; g < n_geoms; ++g)
+        HIPCHK(ctx, launch_fill_u64(scenes[g]->d_hits, (size_t)scenes[g]->n * scenes[g]->n, ctx->stream));
+    HIPCHK(ctx, launch_fill_u64(ctx->d_stats, 2, ctx->stream));
+    rc = ensure_queue(ctx, 4096);
+    if (rc) return rc;
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    int64_t launches = 0;
+    if (o->form == RF_FORM_PERSISTENT) {
+        rc = solve_persistent(ctx, scenes, n_geoms, o, budgets, &launches);
+    } else {
+        for (int g = 0; g < n_geoms && !rc; ++g)
+            for (int e = 0; e < scenes[g]->n && !rc; ++e)
+                rc = launch_range(ctx, scenes[g], e, 0, budgets[g][e], o, o->geometry + (uint32_t)g, nullptr, 0,
+                                  &launches);
+    }
+    if (rc) { (void)hipStreamSynchronize(ctx->stream); return rc; }
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    unsigned long long st[2];
+    rc = read_stats(ctx, st);
+    if (rc) return rc;
+    rf_timing t{};
+    t.kernel_ms = ms;
+    t.launches = launches;
+    t.refined = (int64_t)st[1];
+    for (int g = 0; g < n_geoms; ++g) {
+        account(scenes[g], o, budgets[g], t);
+        if (o->use_bvs && scenes[g]->n_bvs > 0) {
+            // BV kernels count executed tests on the device (all geometries together)
+        }
+        if (hits && hits[g]) { rc = read_hits(ctx, scenes[g], hits[g]); if (rc) return rc; }
+        if (rays && rays[g])
+            for (int e = 0; e < scenes[g]->n; ++e) rays[g][e] = 4 * budgets[g][e];
+    }
+    if (o->use_bvs) {
+        double favg = 0.0;
+        int nbv_g = 0;
+        for (int g = 0; g < n_geoms; ++g)
+            if (scenes[g]->n_bvs > 0) { favg += scenes[g]->flops_per_test_avg; ++nbv_g; }
+        favg = nbv_g ? favg / nbv_g : 0.0;
+        t.tests += (int64_t)st[0];
+        t.flops += (int64_t)((double)st[0] * favg);
+    }
+    t.wall_ms = now_ms() - w0;
+    if (timing) *timing = t;
+    return RF_OK;
+}
+
+extern "C" int32_t rf_solve(rf_ctx *ctx, rf_scene *s, const rf_solve_opts *o, int64_t *hits, int64_t *rays,
+                            rf_timing *timing)
+{
+    if (!ctx || !s) return set_err(ctx, RF_ERR_INVALID_ARG, "NULL context or scene");
+    rf_scene *const sc[1] = {s};
+    int64_t *const h[1] = {hits};
+    int64_t *const r[1] = {rays};
+    return solve_impl(ctx, sc, 1, o, h, r, timing);
+}
+
+extern "C" int32_t rf_batch_solve(rf_ctx *ctx, rf_scene *const *scenes, int32_t n_geoms, const rf_solve_opts *o,
+                                  int64_t *const *hits, int64_t *const *rays, rf_timing *timing)
+{
+    if (!ctx || !scenes || n_geoms <= 0) return set_err(ctx, RF_ERR_INVALID_ARG, "empty batch");
+    return solve_impl(ctx, scenes, n_geoms, o, hits, rays, timing);
+}
+
+extern "C" int32_t rf_trace_rays(rf_ctx *ctx, rf_scene *s, int32_t emitter, int64_t wi0, int64_t n_wi,
+                                 const rf_solve_opts *o, int32_t *idx_out)
+{
+    if (!ctx || !s || !idx_out) return set_err(ctx, RF_ERR_INVALID_ARG, "NULL argument");
+    int rc = check_opts(ctx, o);
+    if (rc) return rc;
+    if (emitter < 0 || emitter >= s->n || wi0 < 0 || n_wi <= 0) return set_err(ctx, RF_ERR_INVALID_ARG, "bad range");
+    if (n_wi > ((int64_t)1 << 26)) return set_err(ctx, RF_ERR_CAPACITY, "trace window above 2^26 work items");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int32_t *d_out = nullptr;
+    HIPCHK(ctx, hipMalloc((void **)&d_out, (size_t)n_wi * 4 * sizeof(int32_t)));
+    HIPCHK(ctx, launch_fill_u32((unsigned *)d_out, 0xffffffffu, (size_t)n_wi * 4, ctx->stream));
+    int64_t launches = 0;
+    rc = launch_range(ctx, s, emitter, wi0, n_wi, o, o->geometry, d_out, wi0, &launches);
+    if (rc == RF_OK) {
+        hipError_t e = hipMemcpyAsync(idx_out, d_out, (size_t)n_wi * 4 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                      ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = set_err(ctx, RF_ERR_HIP, std::string("trace readback: ") + hipGetErrorString(e));
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_out);
+    return rc;
+}
+
+extern "C" int32_t rf_run_cl_gpurng(rf_ctx *ctx, const rf_primitive *prims, int32_t n, const rf_bounding_volume *bvs,
+                                    int32_t nbv, int32_t use_bvs, int32_t n_samples, int32_t *hits,
+                                    double *samples_for_obj, float *vfm)
+{
+    if (!ctx || !hits || !samples_for_obj) return set_err(ctx, RF_ERR_INVALID_ARG, "NULL argument");
+    rf_scene *s = nullptr;
+    int rc = rf_scene_create(ctx, prims, n, bvs, nbv, &s);
+    if (rc) return rc;
+    rf_solve_opts o;
+    rf_solve_opts_default(&o);
+    o.math = RF_MATH_COMPAT;
+    o.use_bvs = use_bvs;
+    o.budget = RF_BUDGET_REFERENCE;
+    o.n_samples = n_samples;
+    std::vector<int64_t> h((size_t)n * n), r(n);
+    rc = rf_solve(ctx, s, &o, h.data(), r.data(), nullptr);
+    rf_scene_destroy(s);
+    if (rc) return rc;
+    for (size_t i = 0; i < h.size(); ++i) {
+        if (h[i] > INT32_MAX) return set_err(ctx, RF_ERR_OVERFLOW, "a hit count exceeds int32 (main.cpp:103)");
+        hits[i] = (int32_t)h[i];
+    }
+    for (int i = 0; i < n; ++i) samples_for_obj[i] = (double)r[i];   // nthreads*count, main.cpp:178
+    if (vfm)
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                float v = 0.0f;
+                v += (float)hits[(size_t)i * n + j];                  // main.cpp:231
+                vfm[(size_t)i * n + j] = (float)((double)v / samples_for_obj[i]);   // :245
+            }
+    return RF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: fastest (rays per thread, block size) of the fast plain kernel
+// on a bounded sample of the given scenes' work.
+// ---------------------------------------------------------------------------
+extern "C" int32_t rf_autotune(rf_ctx *ctx, rf_scene *const *scenes, int32_t n_geoms, const rf_solve_opts *o,
+                               int32_t *best_rpt, int32_t *best_block)
+{
+    if (!ctx || !scenes || n_geoms <= 0) return set_err(ctx, RF_ERR_INVALID_ARG, "empty batch");
+    int rc = check_opts(ctx, o);
+    if (rc) return rc;
+    rf_solve_opts t = *o;
+    t.math = RF_MATH_FAST;
+    t.use_bvs = 0;
+    t.form = RF_FORM_PERSISTENT;
+    t.budget = RF_BUDGET_EXPLICIT;
+    // sample: about 3e11 ray-primitive tests per candidate (a few device fills)
+    double tests_per_budget_ray = 0.0;   // sum over geometries of emitters x primitives
+    for (int g = 0; g < n_geoms; ++g) tests_per_budget_ray += (double)scenes[g]->n * scenes[g]->n;
+    int64_t rays = (int64_t)(3.0e11 / std::max(1.0, tests_per_budget_ray));
+    rays = std::max<int64_t>(rays, 4096);
+    if (o->budget == RF_BUDGET_EXPLICIT && o->rays_per_emitter > 0) rays = std::min(rays, o->rays_per_emitter);
+    t.rays_per_emitter = rays;
+    const int rpts[] = {2, 4, 8};
+    const int blocks[] = {128, 256};
+    double best = 1e300;
+    int br = 4, bb = 256;
+    for (int rp : rpts)
+        for (int bl : blocks) {
+            t.rays_per_thread = rp;
+            t.block_size = bl;
+            rf_timing tm{};
+            rc = solve_impl(ctx, scenes, n_geoms, &t, nullptr, nullptr, &tm);   // warm
+            if (rc) return rc;
+            rc = solve_impl(ctx, scenes, n_geoms, &t, nullptr, nullptr, &tm);
+            if (rc) return rc;
+            if (tm.kernel_ms < best) { best = tm.kernel_ms; br = rp; bb = bl; }
+        }
+    ctx->tuned_rpt = br;
+    ctx->tuned_block = bb;
+    if (best_rpt) *best_rpt = br;
+    if (best_block) *best_block = bb;
+    return RF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Streaming service: resident scenes and tallies, HBM-footprint admission,
+// queued requests of unequal size batched into persistent launches.
+// ---------------------------------------------------------------------------
+struct StreamReq {
+    rf_scene *scene = nullptr;
+    rf_solve_opts opts{};
+    double t_submit = 0.0, t_done = 0.0;
+    bool done = false;
+    std::vector<int64_t> hits, rays;
+    int64_t footprint = 0;
+};
+
+struct rf_stream {
+    rf_ctx *ctx = nullptr;
+    int64_t limit = 0;
+    int64_t resident = 0;
+    int64_t next_id = 1;
+    std::map<int64_t, StreamReq> reqs;
+    std::vector<int64_t> queued;
+};
+
+extern "C" int32_t rf_stream_create(rf_ctx *ctx, int64_t limit, rf_stream **out)
+{
+    if (!ctx || !out || limit <= 0) return set_err(ctx, RF_ERR_INVALID_ARG, "bad stream arguments");
+    rf_stream *st = new rf_stream;
+    st->ctx = ctx;
+    st->limit = limit;
+    *out = st;
+    return RF_OK;
+}
+
+extern "C" int32_t rf_stream_destroy(rf_stream *st)
+{
+    if (!st) return RF_ERR_INVALID_ARG;
+    for (auto &kv : st->reqs)
+        if (kv.second.scene) rf_scene_destroy(kv.second.scene);
+    delete st;
+    return RF_OK;
+}
+
+extern "C" int64_t rf_stream_resident_bytes(const rf_stream *st) { return st ? st->resident : -1; }
+
+extern "C" int32_t rf_stream_submit(rf_stream *st, const rf_primitive *prims, int32_t n, const rf_solve_opts *o,
+                                    int64_t *id)
+{
+    if (!st || !id) return RF_ERR_INVALID_ARG;
+    rf_ctx *ctx = st->ctx;
+    int rc = check_opts(ctx, o);
+    if (rc) return rc;
+    if (o->use_bvs) return set_err(ctx, RF_ERR_INVALID_ARG, "streaming solves take plain scenes");
+    const int64_t fp = rf_scene_footprint(n, 0);
+    if (fp < 0) return set_err(ctx, RF_ERR_INVALID_ARG, "bad scene size");
+    if (st->resident + fp > st->limit)
+        return set_err(ctx, RF_ERR_CAPACITY, "admission: scene footprint exceeds the HBM budget");
+    StreamReq r;
+    rc = rf_scene_create(ctx, prims, n, nullptr, 0, &r.scene);
+    if (rc) return rc;
+    r.opts = *o;
+    r.footprint = fp;
+    r.t_submit = now_ms();
+    st->resident += fp;
+    const int64_t rid = st->next_id++;
+    st->reqs.emplace(rid, std::move(r));
+    st->queued.push_back(rid);
+    *id = rid;
+    return RF_OK;
+}
+
+extern "C" int32_t rf_stream_flush(rf_stream *st, int32_t *n_completed)
+{
+    if (!st) return RF_ERR_INVALID_ARG;
+    rf_ctx *ctx = st->ctx;
+    if (n_completed) *n_completed = 0;
+    if (st->queued.empty()) return RF_OK;
+    // group queued requests by their solve options; one persistent launch per group
+    std::vector<int64_t> pending = st->queued;
+    st->queued.clear();
+    int done = 0;
+    while (!pending.empty()) {
+        const rf_solve_opts ref = st->reqs[pending[0]].opts;
+        std::vector<int64_t> group, rest;
+        for (int64_t rid : pending) {
+            const rf_solve_opts &q = st->reqs[rid].opts;
+            if (q.math == ref.math && q.rng == ref.rng && q.budget == ref.budget && q.n_samples == ref.n_samples &&
+                q.rays_per_emitter == ref.rays_per_emitter && q.seed == ref.seed && q.refine == ref.refine)
+                group.push_back(rid);
+            else
+                rest.push_back(rid);
+        }
+        std::vector<rf_scene *> sc;
+        std::vector<std::vector<int64_t>> hbuf(group.size()), rbuf(group.size());
+        std::vector<int64_t *> hp, rp;
+        for (size_t i = 0; i < group.size(); ++i) {
+            StreamReq &r = st->reqs[group[i]];
+            sc.push_back(r.scene);
+            hbuf[i].resize((size_t)r.scene->n * r.scene->n);
+            rbuf[i].resize(r.scene->n);
+            hp.push_back(hbuf[i].data());
+            rp.push_back(rbuf[i].data());
+        }
+        rf_solve_opts o = ref;
+        o.form = RF_FORM_PERSISTENT;
+        int rc = solve_impl(ctx, sc.data(), (int)sc.size(), &o, hp.data(), rp.data(), nullptr);
+        if (rc) return rc;
+        const double t = now_ms();
+        for (size_t i = 0; i < group.size(); ++i) {
+            StreamReq &r = st->reqs[group[i]];
+            r.hits = std::move(hbuf[i]);
+            r.rays = std::move(rbuf[i]);
+            r.done = true;
+            r.t_done = t;
+            ++done;
+        }
+        pending.swap(rest);
+    }
+    if (n_completed) *n_completed = done;
+    return RF_OK;
+}
+
+extern "C" int32_t rf_stream_collect(rf_stream *st, int64_t id, int64_t *hits, int64_t *rays, double *latency_ms)
+{
+    if (!st) return RF_ERR_INVALID_ARG;
+    auto it = st->reqs.find(id);
+    if (it == st->reqs.end() || !it->second.done) return set_err(st->ctx, RF_ERR_INVALID_ARG, "request not completed");
+    StreamReq &r = it->second;
+    if (hits) std::memcpy(hits, r.hits.data(), r.hits.size() * sizeof(int64_t));
+    if (rays) std::memcpy(rays, r.rays.data(), r.rays.size() * sizeof(int64_t));
+    if (latency_ms) *latency_ms = r.t_done - r.t_submit;
+    st->resident -= r.footprint;
+    rf_scene_destroy(r.scene);
+    st->reqs.erase(it);
+    return RF_OK;
+}

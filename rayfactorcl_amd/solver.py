@@ -1,0 +1,261 @@
+"""Host-side mirror of RayFactorCL's dispatch (RayFactorCL/main.cpp:54-272).
+
+``run_cl_gpurng`` keeps the reference's observables (int32 hit matrix,
+``samplesForObj``, float view factors, ``output.txt``); ``Context`` /
+``DeviceScene`` / ``solve`` expose the per-emitter launch and the whole
+dispatch with int64 tallies, the batched persistent form, per-ray traces,
+the autotuner and the streaming service.  Every call goes through the C ABI
+of ``librayfactor_amd.so``.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_double, c_int32, c_int64, c_void_p
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import BV_DTYPE, FORM, MATH, PRIM_DTYPE, RNG, SolveOpts, Timing, check, lib, ptr
+
+
+def make_opts(math: str = "compat", rng: str = "reference", use_bvs: bool = False,
+              n_samples: int | None = None, rays_per_emitter: int | None = None,
+              form: str = "per_emitter", refine: bool = True, seed: int = 0, geometry: int = 0,
+              rays_per_thread: int = 0, block_size: int = 0) -> SolveOpts:
+    """Solve options.  The ray budget is the reference's launch geometry
+    (``n_samples``, main.cpp:172-173) unless ``rays_per_emitter`` is given."""
+    o = SolveOpts()
+    lib().rf_solve_opts_default(byref(o))
+    o.math = MATH[math]
+    o.rng = RNG[rng]
+    o.use_bvs = int(bool(use_bvs))
+    o.form = FORM[form]
+    o.refine = int(bool(refine))
+    o.seed = seed & 0xFFFFFFFF
+    o.geometry = geometry & 0xFFFFFFFF
+    o.rays_per_thread = rays_per_thread
+    o.block_size = block_size
+    if rays_per_emitter is not None:
+        o.budget = 1
+        o.rays_per_emitter = int(rays_per_emitter)
+    else:
+        o.budget = 0
+        o.n_samples = int(n_samples if n_samples is not None else 1000000)
+    return o
+
+
+def nthreads(ray_density: int, n_samples: int, count: int = 4, wgsize: int = 128) -> int:
+    """Work items launched per emitter (main.cpp:172-173)."""
+    out = c_int64()
+    check(lib().rf_nthreads(ray_density, n_samples, count, wgsize, byref(out)), "rf_nthreads")
+    return out.value
+
+
+class Context:
+    """One HIP device and stream (replaces opencl_init, util_opencl.h:133)."""
+
+    def __init__(self, device: int = 0):
+        h = c_void_p()
+        check(lib().rf_ctx_create(device, byref(h)), "rf_ctx_create")
+        self.handle = h
+        name = ctypes.create_string_buffer(256)
+        n_cu, clk = c_int32(), c_int32()
+        lib().rf_device_info(self.handle, name, 256, byref(n_cu), byref(clk))
+        self.arch = name.value.decode()
+        self.n_cu = n_cu.value
+        self.clock_mhz = clk.value
+
+    def close(self) -> None:
+        if self.handle:
+            lib().rf_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self) -> None:
+        check(lib().rf_sync(self.handle), "rf_sync", self.handle)
+
+
+class DeviceScene:
+    """Primitives (and bounding volumes) resident on the device."""
+
+    def __init__(self, ctx: Context, prims: np.ndarray, bvs: np.ndarray | None = None):
+        self.ctx = ctx
+        self.prims = np.ascontiguousarray(prims, dtype=PRIM_DTYPE)
+        self.bvs = None if bvs is None or len(bvs) == 0 else np.ascontiguousarray(bvs, dtype=BV_DTYPE)
+        h = c_void_p()
+        check(lib().rf_scene_create(ctx.handle, ptr(self.prims), len(self.prims), ptr(self.bvs),
+                                    0 if self.bvs is None else len(self.bvs), byref(h)),
+              "rf_scene_create", ctx.handle)
+        self.handle = h
+        self.n = len(self.prims)
+
+    def close(self) -> None:
+        if self.handle:
+            lib().rf_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- per-emitter launch (clSetKernelArg(0) + clEnqueueNDRangeKernel) --
+    def launch_emitter(self, emitter: int, wi0: int, n_wi: int, opts: SolveOpts) -> None:
+        check(lib().rf_launch_emitter(self.ctx.handle, self.handle, emitter, wi0, n_wi, byref(opts)),
+              "rf_launch_emitter", self.ctx.handle)
+
+    def hits(self) -> np.ndarray:
+        out = np.zeros((self.n, self.n), dtype=np.int64)
+        check(lib().rf_hits_read(self.ctx.handle, self.handle, ptr(out)), "rf_hits_read", self.ctx.handle)
+        return out
+
+    def reset_hits(self) -> None:
+        check(lib().rf_hits_reset(self.ctx.handle, self.handle), "rf_hits_reset", self.ctx.handle)
+
+
+@dataclass
+class SolveResult:
+    hits: np.ndarray        # (n, n) int64, row = emitter
+    rays: np.ndarray        # (n,) int64, rays traced per emitter
+    timing: dict
+
+    @property
+    def vf(self) -> np.ndarray:
+        return normalise(self.hits, self.rays)
+
+
+def solve(ctx: Context, scene: DeviceScene, opts: SolveOpts | None = None, **kw) -> SolveResult:
+    """All emitters of one scene (the dispatch of runCL_GPURNG, main.cpp:54-214)."""
+    o = opts if opts is not None else make_opts(**kw)
+    hits = np.zeros((scene.n, scene.n), dtype=np.int64)
+    rays = np.zeros(scene.n, dtype=np.int64)
+    t = Timing()
+    check(lib().rf_solve(ctx.handle, scene.handle, byref(o), ptr(hits), ptr(rays), byref(t)), "rf_solve",
+          ctx.handle)
+    return SolveResult(hits, rays, t.as_dict())
+
+
+def batch_solve(ctx: Context, scenes: list[DeviceScene], opts: SolveOpts | None = None, readback: bool = True,
+                **kw) -> tuple[list[SolveResult], dict]:
+    """Independent scenes in one persistent launch (form='persistent') or one
+    launch per emitter of each (form='per_emitter' / 'reference')."""
+    o = opts if opts is not None else make_opts(**kw)
+    n_g = len(scenes)
+    sc = (c_void_p * n_g)(*[s.handle.value for s in scenes])
+    hits = [np.zeros((s.n, s.n), dtype=np.int64) for s in scenes]
+    rays = [np.zeros(s.n, dtype=np.int64) for s in scenes]
+    hp = (c_void_p * n_g)(*[h.ctypes.data if readback else None for h in hits])
+    rp = (c_void_p * n_g)(*[r.ctypes.data for r in rays])
+    t = Timing()
+    check(lib().rf_batch_solve(ctx.handle, sc, n_g, byref(o), hp, rp, byref(t)), "rf_batch_solve", ctx.handle)
+    res = [SolveResult(h, r, {}) for h, r in zip(hits, rays)]
+    return res, t.as_dict()
+
+
+def trace(ctx: Context, scene: DeviceScene, emitter: int, wi0: int, n_wi: int, opts: SolveOpts | None = None,
+          **kw) -> np.ndarray:
+    """Hit index (-1 = miss) of each of the 4*n_wi rays of work items [wi0, wi0+n_wi)."""
+    o = opts if opts is not None else make_opts(**kw)
+    out = np.empty(4 * n_wi, dtype=np.int32)
+    check(lib().rf_trace_rays(ctx.handle, scene.handle, emitter, wi0, n_wi, byref(o), ptr(out)), "rf_trace_rays",
+          ctx.handle)
+    return out
+
+
+def autotune(ctx: Context, scenes: list[DeviceScene], opts: SolveOpts | None = None, **kw) -> tuple[int, int]:
+    """Fastest (rays per thread, block size) for the fast kernel on these scenes."""
+    o = opts if opts is not None else make_opts(**kw)
+    sc = (c_void_p * len(scenes))(*[s.handle.value for s in scenes])
+    r, b = c_int32(), c_int32()
+    check(lib().rf_autotune(ctx.handle, sc, len(scenes), byref(o), byref(r), byref(b)), "rf_autotune", ctx.handle)
+    return r.value, b.value
+
+
+def run_cl_gpurng(ctx: Context, prims: np.ndarray, bvs: np.ndarray | None, use_bvs: bool, n_samples: int):
+    """runCL_GPURNG (main.cpp:54-272): int32 hits, samplesForObj, float vfm."""
+    prims = np.ascontiguousarray(prims, dtype=PRIM_DTYPE)
+    bvs = None if bvs is None or len(bvs) == 0 else np.ascontiguousarray(bvs, dtype=BV_DTYPE)
+    n = len(prims)
+    hits = np.zeros((n, n), dtype=np.int32)
+    samples = np.zeros(n, dtype=np.float64)
+    vfm = np.zeros((n, n), dtype=np.float32)
+    check(lib().rf_run_cl_gpurng(ctx.handle, ptr(prims), n, ptr(bvs), 0 if bvs is None else len(bvs),
+                                 int(bool(use_bvs)), int(n_samples), ptr(hits), ptr(samples), ptr(vfm)),
+          "rf_run_cl_gpurng", ctx.handle)
+    return hits, samples, vfm
+
+
+def normalise(hits: np.ndarray, rays: np.ndarray) -> np.ndarray:
+    """FP64 view factors hits[i, j] / rays[i]."""
+    hits = np.ascontiguousarray(hits, dtype=np.int64)
+    rays = np.ascontiguousarray(rays, dtype=np.int64)
+    n = len(rays)
+    vf = np.zeros((n, n), dtype=np.float64)
+    check(lib().rf_normalise(n, ptr(hits), ptr(rays), ptr(vf)), "rf_normalise")
+    return vf
+
+
+def write_output(path: str, hits: np.ndarray, samples: np.ndarray) -> None:
+    """output.txt in the reference format (main.cpp:236-251)."""
+    hits = np.ascontiguousarray(hits, dtype=np.int64)
+    samples = np.ascontiguousarray(samples, dtype=np.float64)
+    check(lib().rf_write_output(str(path).encode(), len(samples), ptr(hits), ptr(samples)), "rf_write_output")
+
+
+class Stream:
+    """Streaming solves: resident scenes, HBM-footprint admission, batched flushes."""
+
+    def __init__(self, ctx: Context, hbm_limit_bytes: int):
+        h = c_void_p()
+        check(lib().rf_stream_create(ctx.handle, int(hbm_limit_bytes), byref(h)), "rf_stream_create", ctx.handle)
+        self.ctx = ctx
+        self.handle = h
+        self._n = {}
+
+    def submit(self, prims: np.ndarray, opts: SolveOpts) -> int:
+        prims = np.ascontiguousarray(prims, dtype=PRIM_DTYPE)
+        rid = c_int64()
+        check(lib().rf_stream_submit(self.handle, ptr(prims), len(prims), byref(opts), byref(rid)),
+              "rf_stream_submit", self.ctx.handle)
+        self._n[rid.value] = len(prims)
+        return rid.value
+
+    def flush(self) -> int:
+        n = c_int32()
+        check(lib().rf_stream_flush(self.handle, byref(n)), "rf_stream_flush", self.ctx.handle)
+        return n.value
+
+    def collect(self, rid: int) -> tuple[np.ndarray, np.ndarray, float]:
+        n = self._n[rid]
+        hits = np.zeros((n, n), dtype=np.int64)
+        rays = np.zeros(n, dtype=np.int64)
+        lat = c_double()
+        check(lib().rf_stream_collect(self.handle, rid, ptr(hits), ptr(rays), byref(lat)), "rf_stream_collect",
+              self.ctx.handle)
+        del self._n[rid]
+        return hits, rays, lat.value
+
+    def resident_bytes(self) -> int:
+        return lib().rf_stream_resident_bytes(self.handle)
+
+    def close(self) -> None:
+        if self.handle:
+            lib().rf_stream_destroy(self.handle)
+            self.handle = None
+
+
+def scene_footprint(n_prims: int, n_bvs: int = 0) -> int:
+    return lib().rf_scene_footprint(n_prims, n_bvs)
