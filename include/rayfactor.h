@@ -119,7 +119,7 @@ typedef struct rf_solve_opts {
     uint32_t geometry;        /* RF_RNG_SEEDED: geometry id (batched solves use their index) */
     int32_t refine;           /* RF_MATH_FAST: 1 = FP64 re-solve of flagged roots */
     int32_t rays_per_thread;  /* 0 = autotuned/default; else 1, 2, 4 or 8 */
-    int32_t block_size;       /* 0 = autotuned/default; else 64..1024, multiple of 64 */
+    int32_t block_size;       /* 0 = autotuned/default; else 64..256, multiple of 64 */
     int32_t reserved;
 } rf_solve_opts;
 
@@ -149,6 +149,8 @@ const char *rf_status_string(int32_t status);
 
 /* Replaces: opencl_init (util_opencl.h:133-334) -- one HIP device, one stream. */
 int32_t rf_ctx_create(int32_t hip_device, rf_ctx **out);
+/* Also releases the context's remaining scenes and streams; handles to them
+ * are invalid afterwards. */
 int32_t rf_ctx_destroy(rf_ctx *ctx);
 /* Last error text of a context (empty if none).  Valid until the next call. */
 const char *rf_last_error(const rf_ctx *ctx);

@@ -240,22 +240,38 @@ __device__ __forceinline__ Ray64 widen(const Ray32 &r)
 
 // FP64 re-solve of one flagged (ray, primitive) pair from the caller's
 // record; the FP64 root is rounded back for the nearest-hit comparison.
-__device__ __forceinline__ float refine_prim(CPTR(rf_primitive) P, int orig, const Ray32 &R)
+// Kept out of line: it runs for a small fraction of pairs, and inlining it
+// into the unrolled multi-ray loop would size the whole kernel's register
+// allocation for FP64 temporaries of every ray at once.
+__device__ __attribute__((noinline)) float refine_prim(CPTR(rf_primitive) P, int orig, const Ray32 &R)
 {
     const rf_primitive p = ldc(P + orig);
     return (float)hit_d(p.type, p.invm, p.util1, p.util2, widen(R));
 }
 
-__device__ __forceinline__ float refine_bv(CPTR(rf_bounding_volume) B, int w, const Ray32 &R)
+__device__ __attribute__((noinline)) float refine_bv(CPTR(rf_bounding_volume) B, int w, const Ray32 &R)
 {
     const rf_bounding_volume b = ldc(B + w);
     return (float)hit_d(b.type, b.invm, b.util1, b.util2, widen(R));
 }
 
+// Deferred FP64 re-solves of the plain fast kernel: a flagged (ray, primitive)
+// pair is kept out of the FP32 minimum and queued in a per-lane list in LDS
+// (PEND_K entries per lane, strided by the workgroup size); after the scan the
+// list is re-solved in FP64 and merged into the nearest hit.  A lane whose
+// list overflows re-traces that ray entirely in FP64.  No FP64 code runs
+// inside the primitive loop.
+constexpr int PEND_K = 4;
+
+struct Pending {
+    uint32_t *buf;     // LDS, PEND_K * blockDim entries: (orig << 3) | ray slot
+    int count;
+    uint32_t overflow; // bit i: ray slot i needs a full FP64 re-trace
+};
+
 template <int KIND, int NR, bool REFINE>
-__device__ __forceinline__ void seg_loop(CPTR(FastPrim) F, CPTR(rf_primitive) P, int begin, int end,
-                                         const Ray32 (&R)[NR], float (&tb)[NR], int (&ib)[NR],
-                                         uint32_t &nref)
+__device__ __forceinline__ void seg_loop(CPTR(FastPrim) F, int begin, int end, const Ray32 (&R)[NR],
+                                         float (&tb)[NR], int (&ib)[NR], Pending &pd)
 {
     for (int w = begin; w < end; ++w) {
         const FastPrim q = ldc(F + w);
@@ -270,8 +286,17 @@ __device__ __forceinline__ void seg_loop(CPTR(FastPrim) F, CPTR(rf_primitive) P,
         if constexpr (REFINE && KIND <= SEG_SPHERE) {
             if (__builtin_amdgcn_ballot_w64(any)) {
 #pragma unroll
-                for (int i = 0; i < NR; ++i)
-                    if (f[i]) { t[i] = refine_prim(P, q.orig, R[i]); ++nref; }
+                for (int i = 0; i < NR; ++i) {
+                    if (f[i]) {
+                        if (pd.count < PEND_K) {
+                            pd.buf[pd.count * blockDim.x + threadIdx.x] = ((uint32_t)q.orig << 3) | (uint32_t)i;
+                            ++pd.count;
+                        } else {
+                            pd.overflow |= 1u << i;
+                        }
+                        t[i] = finf();
+                    }
+                }
             }
         }
 #pragma unroll
@@ -280,17 +305,65 @@ __device__ __forceinline__ void seg_loop(CPTR(FastPrim) F, CPTR(rf_primitive) P,
     }
 }
 
+// FP64 stage of the deferred re-solves (see Pending).  The flagged ray is
+// regenerated from its index (emission is a pure function of the random
+// block), so the scan's ray registers are dead here.
+template <int NR>
+__device__ __forceinline__ void resolve_pending(const KParams &kp, CPTR(GeomDesc) G, const rf_primitive &E,
+                                                uint32_t gid, int emitter, int64_t wi0, int64_t base,
+                                                float (&tb)[NR], int (&ib)[NR], Pending &pd, uint32_t &nref)
+{
+    CPTR(rf_primitive) P = cst(G->prims);
+    const int n = G->n;
+    for (int e = 0; e < pd.count + __builtin_popcount(pd.overflow); ++e) {
+        int prim_lo, prim_hi, k;
+        if (e < pd.count) {
+            const uint32_t ent = pd.buf[e * blockDim.x + threadIdx.x];
+            prim_lo = (int)(ent >> 3);
+            prim_hi = prim_lo + 1;
+            k = (int)(ent & 7u);
+        } else {
+            // a slot whose list overflowed: re-trace the whole ray in FP64
+            int m = e - pd.count;
+            uint32_t ov = pd.overflow;
+            while (m--) ov &= ov - 1;
+            k = __builtin_ctz(ov);
+            prim_lo = 0;
+            prim_hi = n;
+        }
+        const int64_t rl = base + k;
+        float f[4], r4[4];
+        workitem_floats(kp, gid, emitter, wi0 + (rl >> 2), f);
+        rotate_rands(f, (int)(rl & 3), r4);
+        const Ray64 r64 = widen(emit_fast(&E, r4));
+        float tk = finf();
+        int ik = -1;
+#pragma unroll
+        for (int i = 0; i < NR; ++i)
+            if (k == i) { tk = tb[i]; ik = ib[i]; }
+        if (prim_hi - prim_lo > 1) { tk = finf(); ik = -1; }
+        for (int w = prim_lo; w < prim_hi; ++w) {
+            const rf_primitive p = ldc(P + w);
+            const float t = (float)hit_d(p.type, p.invm, p.util1, p.util2, r64);
+            if (t < tk || (t == tk && w < ik)) { tk = t; ik = w; }
+        }
+#pragma unroll
+        for (int i = 0; i < NR; ++i)
+            if (k == i) { tb[i] = tk; ib[i] = ik; }
+        nref += (uint32_t)(prim_hi - prim_lo);
+    }
+}
+
 template <int NR, bool REFINE>
 __device__ __forceinline__ void scan_fast(CPTR(GeomDesc) G, const Ray32 (&R)[NR], float (&tb)[NR],
-                                          int (&ib)[NR], uint32_t &nref)
+                                          int (&ib)[NR], Pending &pd)
 {
     CPTR(FastPrim) F = cst(G->fast);
-    CPTR(rf_primitive) P = cst(G->prims);
-    seg_loop<SEG_CYL, NR, REFINE>(F, P, G->seg_begin[SEG_CYL], G->seg_end[SEG_CYL], R, tb, ib, nref);
-    seg_loop<SEG_FRUSTUM, NR, REFINE>(F, P, G->seg_begin[SEG_FRUSTUM], G->seg_end[SEG_FRUSTUM], R, tb, ib, nref);
-    seg_loop<SEG_SPHERE, NR, REFINE>(F, P, G->seg_begin[SEG_SPHERE], G->seg_end[SEG_SPHERE], R, tb, ib, nref);
-    seg_loop<SEG_RING, NR, REFINE>(F, P, G->seg_begin[SEG_RING], G->seg_end[SEG_RING], R, tb, ib, nref);
-    seg_loop<SEG_RECT, NR, REFINE>(F, P, G->seg_begin[SEG_RECT], G->seg_end[SEG_RECT], R, tb, ib, nref);
+    seg_loop<SEG_CYL, NR, REFINE>(F, G->seg_begin[SEG_CYL], G->seg_end[SEG_CYL], R, tb, ib, pd);
+    seg_loop<SEG_FRUSTUM, NR, REFINE>(F, G->seg_begin[SEG_FRUSTUM], G->seg_end[SEG_FRUSTUM], R, tb, ib, pd);
+    seg_loop<SEG_SPHERE, NR, REFINE>(F, G->seg_begin[SEG_SPHERE], G->seg_end[SEG_SPHERE], R, tb, ib, pd);
+    seg_loop<SEG_RING, NR, REFINE>(F, G->seg_begin[SEG_RING], G->seg_end[SEG_RING], R, tb, ib, pd);
+    seg_loop<SEG_RECT, NR, REFINE>(F, G->seg_begin[SEG_RECT], G->seg_end[SEG_RECT], R, tb, ib, pd);
 }
 
 // Child scan of the fast BV kernel over the caller's order [start, end).
@@ -387,12 +460,12 @@ __device__ __forceinline__ void tally(const KParams &kp, CPTR(GeomDesc) G, int e
 template <int MATH, bool BV, bool TRACE, int NR, bool REFINE, bool DIRECT>
 __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, uint32_t gid, int emitter,
                                          int64_t wi0, int64_t n_wi, int64_t first, int64_t stride,
-                                         uint32_t *bins, uint64_t &tests, uint32_t &nref)
+                                         uint32_t *bins, uint32_t *pend, uint64_t &tests, uint32_t &nref)
 {
     const rf_primitive E = ldc(cst(G->prims) + emitter);
     const int64_t ray0 = (wi0 - kp.trace_wi0) * 4;   // absolute ray index base (trace launches)
-    if constexpr (MATH == KM_FAST && !BV) {
-        // NR rays per thread; rays [4*first*.. ) in units of NR
+    if constexpr (MATH == KM_FAST && !BV && !DIRECT) {
+        // NR rays per thread: ray groups first, first+stride, ... of NR consecutive rays
         const int64_t n_rays = n_wi * 4;
         for (int64_t base = first * NR; base < n_rays; base += stride * NR) {
             Ray32 R[NR];
@@ -411,7 +484,12 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                 tb[i] = finf();
                 ib[i] = -1;
             }
-            scan_fast<NR, REFINE>(G, R, tb, ib, nref);
+            Pending pd{pend, 0, 0u};
+            scan_fast<NR, REFINE>(G, R, tb, ib, pd);
+            if constexpr (REFINE) {
+                if (__builtin_amdgcn_ballot_w64(pd.count > 0 || pd.overflow != 0))
+                    resolve_pending<NR>(kp, G, E, gid, emitter, wi0, base, tb, ib, pd, nref);
+            }
 #pragma unroll
             for (int i = 0; i < NR; ++i)
                 if (base + i < n_rays) tally<TRACE, DIRECT>(kp, G, emitter, ray0 + base + i, ib[i], bins);
@@ -442,7 +520,12 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                         Ray32 R1[1] = {ray};
                         float tb1[1] = {finf()};
                         int ib1[1] = {-1};
-                        scan_fast<1, REFINE>(G, R1, tb1, ib1, nref);
+                        Pending pd{pend, 0, 0u};
+                        scan_fast<1, REFINE>(G, R1, tb1, ib1, pd);
+                        if constexpr (REFINE) {
+                            if (__builtin_amdgcn_ballot_w64(pd.count > 0 || pd.overflow != 0))
+                                resolve_pending<1>(kp, G, E, gid, emitter, wi0, wl * 4 + v, tb1, ib1, pd, nref);
+                        }
                         idx = ib1[0];
                     }
                 }
@@ -471,7 +554,10 @@ __device__ __forceinline__ void flush_stats(const KParams &kp, uint64_t tests, u
 template <int MATH, bool BV, bool TRACE, int NR, bool REFINE>
 __global__ void __launch_bounds__(256) k_queue(KParams kp)
 {
-    extern __shared__ uint32_t bins[];
+    // dynamic LDS: [bins: kp.bins x u32][pending re-solves: PEND_K x blockDim x u32]
+    extern __shared__ uint32_t smem[];
+    uint32_t *bins = smem;
+    uint32_t *pend = smem + (TRACE ? 0 : kp.bins);
     __shared__ int s_unit;
     uint64_t tests = 0;
     uint32_t nref = 0;
@@ -504,7 +590,7 @@ __global__ void __launch_bounds__(256) k_queue(KParams kp)
             __syncthreads();
         }
         run_unit<MATH, BV, TRACE, NR, REFINE, false>(kp, G, gid, emitter, wi0, n_wi, threadIdx.x,
-                                                     blockDim.x, bins, tests, nref);
+                                                     blockDim.x, bins, pend, tests, nref);
         if (lds_tally) {
             __syncthreads();
             unsigned long long *row = G->hits + (int64_t)emitter * n;
@@ -524,12 +610,13 @@ __global__ void __launch_bounds__(256) k_queue(KParams kp)
 template <int MATH, bool BV>
 __global__ void __launch_bounds__(256) k_direct(KParams kp)
 {
+    extern __shared__ uint32_t smem[];   // pending re-solves: PEND_K x blockDim x u32
     CPTR(GeomDesc) G = cst(kp.geoms);
     uint64_t tests = 0;
     uint32_t nref = 0;
     const int64_t wl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     run_unit<MATH, BV, false, 1, true, true>(kp, G, kp.imp_geom_id, kp.imp_emitter, kp.imp_wi0, kp.imp_n_wi,
-                                             wl, kp.imp_n_wi + 1, nullptr, tests, nref);
+                                             wl, kp.imp_n_wi + 1, nullptr, smem, tests, nref);
     flush_stats(kp, tests, nref);
 }
 
@@ -591,8 +678,8 @@ template <int MATH>
 static hipError_t go_math(const LaunchCfg &c, const KParams &kp, hipStream_t s)
 {
     if (c.direct) {
-        if (c.use_bv) hipLaunchKernelGGL((k_direct<MATH, true>), dim3(c.grid), dim3(c.block), 0, s, kp);
-        else hipLaunchKernelGGL((k_direct<MATH, false>), dim3(c.grid), dim3(c.block), 0, s, kp);
+        if (c.use_bv) hipLaunchKernelGGL((k_direct<MATH, true>), dim3(c.grid), dim3(c.block), c.lds_bytes, s, kp);
+        else hipLaunchKernelGGL((k_direct<MATH, false>), dim3(c.grid), dim3(c.block), c.lds_bytes, s, kp);
         return hipGetLastError();
     }
     if (c.use_bv) return c.trace ? go_queue_math<MATH, true, true>(c, kp, s) : go_queue_math<MATH, true, false>(c, kp, s);

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -40,6 +41,9 @@ struct rf_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<uintptr_t> graph_key;
+    // live objects owned by the context (released by rf_ctx_destroy)
+    std::set<rf_scene *> scenes;
+    std::set<rf_stream *> streams;
 };
 
 struct rf_scene {
@@ -141,6 +145,9 @@ extern "C" int32_t rf_ctx_destroy(rf_ctx *c)
     if (!c) return RF_ERR_INVALID_ARG;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    // streams own scenes of their own; release them first, then remaining scenes
+    while (!c->streams.empty()) rf_stream_destroy(*c->streams.begin());
+    while (!c->scenes.empty()) rf_scene_destroy(*c->scenes.begin());
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     (void)hipFree(c->d_geoms);
@@ -305,6 +312,7 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
         delete s;
         return code;
     }
+    ctx->scenes.insert(s);
     *out = s;
     return RF_OK;
 }
@@ -314,6 +322,7 @@ extern "C" int32_t rf_scene_destroy(rf_scene *s)
     if (!s) return RF_ERR_INVALID_ARG;
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
+    s->ctx->scenes.erase(s);
     free_scene_mem(s);
     delete s;
     return RF_OK;
@@ -359,7 +368,8 @@ static int check_opts(rf_ctx *ctx, const rf_solve_opts *o)
 }
 
 // Work items per emitter of a scene under the options' budget.
-static int emitter_budget(rf_ctx *ctx, const rf_scene *s, const rf_solve_opts *o, std::vector<int64_t> &n_wi)
+static int emitter_budget(rf_ctx *ctx, const rf_scene *s, const rf_solve_opts *o, std::vector<int64_t> &n_wi,
+                          int64_t rays_override = -1)
 {
     n_wi.assign(s->n, 0);
     if (o->budget == RF_BUDGET_REFERENCE) {
@@ -370,7 +380,8 @@ static int emitter_budget(rf_ctx *ctx, const rf_scene *s, const rf_solve_opts *o
         if (nt > INT32_MAX) return set_err(ctx, RF_ERR_OVERFLOW, "work items exceed the reference's int gid");
         std::fill(n_wi.begin(), n_wi.end(), nt);
     } else {
-        const int64_t wi = (o->rays_per_emitter + 3) / 4;
+        const int64_t r = rays_override >= 0 ? rays_override : o->rays_per_emitter;
+        const int64_t wi = (r + 3) / 4;
         if (o->rng == RF_RNG_REFERENCE && wi > ((int64_t)1 << 32))
             return set_err(ctx, RF_ERR_OVERFLOW, "reference RNG keys work items with 32 bits");
         std::fill(n_wi.begin(), n_wi.end(), wi);
@@ -381,7 +392,8 @@ static int emitter_budget(rf_ctx *ctx, const rf_scene *s, const rf_solve_opts *o
 // ---------------------------------------------------------------------------
 // Dispatch
 // ---------------------------------------------------------------------------
-static constexpr int kMaxLdsBins = 16384;   // 64 KiB of 32-bit bins per workgroup
+static constexpr int kPendK = 4;            // deferred FP64 re-solves per lane (PEND_K in the kernels)
+static constexpr int kMaxLdsBins = 15000;   // 32-bit bins per workgroup (< 64 KiB with the re-solve list)
 
 static void pick_shape(const rf_ctx *ctx, const rf_solve_opts *o, int &nr, int &block)
 {
@@ -460,7 +472,7 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
         const int64_t grid = (n_wi + 127) / 128;
         if (grid > INT32_MAX) return set_err(ctx, RF_ERR_CAPACITY, "grid too large");
         c.grid = (int)grid;
-        c.lds_bytes = 0;
+        c.lds_bytes = (size_t)kPendK * c.block * sizeof(uint32_t);
         HIPCHK(ctx, launch_solve(c, kp, ctx->stream));
         ++*launches;
         return RF_OK;
@@ -480,7 +492,8 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
     c.direct = 0;
     c.block = block;
     c.grid = (int)std::min<int64_t>(n_units, groups);
-    c.lds_bytes = (c.trace || kp.bins == 0) ? 0 : (size_t)kp.bins * sizeof(uint32_t);
+    c.lds_bytes = ((c.trace || kp.bins == 0) ? 0 : (size_t)kp.bins * sizeof(uint32_t)) +
+                  (size_t)kPendK * block * sizeof(uint32_t);
     HIPCHK(ctx, launch_solve(c, kp, ctx->stream));
     ++*launches;
     return RF_OK;
@@ -559,7 +572,8 @@ static void account(const rf_scene *s, const rf_solve_opts *o, const std::vector
 // Persistent batched solve over several scenes (and the single-scene
 // persistent form).
 static int solve_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const rf_solve_opts *o,
-                            const std::vector<std::vector<int64_t>> &budgets, int64_t *launches)
+                            const std::vector<std::vector<int64_t>> &budgets, const std::vector<uint32_t> &ids,
+                            int64_t *launches)
 {
     int nr, block;
     pick_shape(ctx, o, nr, block);
@@ -570,7 +584,7 @@ static int solve_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, c
     int64_t total_wi = 0;
     for (int g = 0; g < n_geoms; ++g) {
         descs[g] = scenes[g]->desc;
-        descs[g].geometry_id = o->geometry + (uint32_t)g;
+        descs[g].geometry_id = ids[g];
         max_n = std::max(max_n, scenes[g]->n);
         for (int64_t w : budgets[g]) total_wi += w;
     }
@@ -623,23 +637,28 @@ static int solve_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, c
     c.refine = o->refine;
     c.block = block;
     c.grid = (int)std::min<int64_t>((int64_t)units.size(), groups);
-    c.lds_bytes = kp.bins ? (size_t)kp.bins * sizeof(uint32_t) : 0;
+    c.lds_bytes = (kp.bins ? (size_t)kp.bins * sizeof(uint32_t) : 0) + (size_t)kPendK * block * sizeof(uint32_t);
     HIPCHK(ctx, launch_solve(c, kp, ctx->stream));
     ++*launches;
     return RF_OK;
 }
 
+// geom_ids: seeded-RNG geometry word of each scene (NULL: opts->geometry + index);
+// geom_rays: explicit rays per emitter of each scene (NULL: opts->rays_per_emitter)
 static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const rf_solve_opts *o,
-                      int64_t *const *hits, int64_t *const *rays, rf_timing *timing)
+                      int64_t *const *hits, int64_t *const *rays, rf_timing *timing,
+                      const uint32_t *geom_ids = nullptr, const int64_t *geom_rays = nullptr)
 {
     const double w0 = now_ms();
     int rc = check_opts(ctx, o);
     if (rc) return rc;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     std::vector<std::vector<int64_t>> budgets(n_geoms);
+    std::vector<uint32_t> ids(n_geoms);
+    for (int g = 0; g < n_geoms; ++g) ids[g] = geom_ids ? geom_ids[g] : o->geometry + (uint32_t)g;
     for (int g = 0; g < n_geoms; ++g) {
         if (!scenes[g] || scenes[g]->ctx != ctx) return set_err(ctx, RF_ERR_INVALID_ARG, "scene of another context");
-        rc = emitter_budget(ctx, scenes[g], o, budgets[g]);
+        rc = emitter_budget(ctx, scenes[g], o, budgets[g], geom_rays ? geom_rays[g] : -1);
         if (rc) return rc;
     }
     // hit matrices and statistics start at zero
@@ -651,12 +670,11 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
     HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     int64_t launches = 0;
     if (o->form == RF_FORM_PERSISTENT) {
-        rc = solve_persistent(ctx, scenes, n_geoms, o, budgets, &launches);
+        rc = solve_persistent(ctx, scenes, n_geoms, o, budgets, ids, &launches);
     } else {
         for (int g = 0; g < n_geoms && !rc; ++g)
             for (int e = 0; e < scenes[g]->n && !rc; ++e)
-                rc = launch_range(ctx, scenes[g], e, 0, budgets[g][e], o, o->geometry + (uint32_t)g, nullptr, 0,
-                                  &launches);
+                rc = launch_range(ctx, scenes[g], e, 0, budgets[g][e], o, ids[g], nullptr, 0, &launches);
     }
     if (rc) { (void)hipStreamSynchronize(ctx->stream); return rc; }
     HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -840,6 +858,7 @@ extern "C" int32_t rf_stream_create(rf_ctx *ctx, int64_t limit, rf_stream **out)
     rf_stream *st = new rf_stream;
     st->ctx = ctx;
     st->limit = limit;
+    ctx->streams.insert(st);
     *out = st;
     return RF_OK;
 }
@@ -849,6 +868,7 @@ extern "C" int32_t rf_stream_destroy(rf_stream *st)
     if (!st) return RF_ERR_INVALID_ARG;
     for (auto &kv : st->reqs)
         if (kv.second.scene) rf_scene_destroy(kv.second.scene);
+    st->ctx->streams.erase(st);
     delete st;
     return RF_OK;
 }
@@ -897,17 +917,21 @@ extern "C" int32_t rf_stream_flush(rf_stream *st, int32_t *n_completed)
         for (int64_t rid : pending) {
             const rf_solve_opts &q = st->reqs[rid].opts;
             if (q.math == ref.math && q.rng == ref.rng && q.budget == ref.budget && q.n_samples == ref.n_samples &&
-                q.rays_per_emitter == ref.rays_per_emitter && q.seed == ref.seed && q.refine == ref.refine)
+                q.seed == ref.seed && q.refine == ref.refine)
                 group.push_back(rid);
             else
                 rest.push_back(rid);
         }
         std::vector<rf_scene *> sc;
+        std::vector<uint32_t> gids;
+        std::vector<int64_t> grays;
         std::vector<std::vector<int64_t>> hbuf(group.size()), rbuf(group.size());
         std::vector<int64_t *> hp, rp;
         for (size_t i = 0; i < group.size(); ++i) {
             StreamReq &r = st->reqs[group[i]];
             sc.push_back(r.scene);
+            gids.push_back(r.opts.geometry);
+            grays.push_back(r.opts.rays_per_emitter);
             hbuf[i].resize((size_t)r.scene->n * r.scene->n);
             rbuf[i].resize(r.scene->n);
             hp.push_back(hbuf[i].data());
@@ -915,7 +939,8 @@ extern "C" int32_t rf_stream_flush(rf_stream *st, int32_t *n_completed)
         }
         rf_solve_opts o = ref;
         o.form = RF_FORM_PERSISTENT;
-        int rc = solve_impl(ctx, sc.data(), (int)sc.size(), &o, hp.data(), rp.data(), nullptr);
+        int rc = solve_impl(ctx, sc.data(), (int)sc.size(), &o, hp.data(), rp.data(), nullptr, gids.data(),
+                            grays.data());
         if (rc) return rc;
         const double t = now_ms();
         for (size_t i = 0; i < group.size(); ++i) {

@@ -10,6 +10,7 @@ of ``librayfactor_amd.so``.
 from __future__ import annotations
 
 import ctypes
+import weakref
 from ctypes import byref, c_double, c_int32, c_int64, c_void_p
 from dataclasses import dataclass
 
@@ -65,9 +66,12 @@ class Context:
         self.arch = name.value.decode()
         self.n_cu = n_cu.value
         self.clock_mhz = clk.value
+        self._children = weakref.WeakSet()   # scenes and streams, closed before the context
 
     def close(self) -> None:
         if self.handle:
+            for child in list(self._children):
+                child.close()
             lib().rf_ctx_destroy(self.handle)
             self.handle = None
 
@@ -100,11 +104,13 @@ class DeviceScene:
               "rf_scene_create", ctx.handle)
         self.handle = h
         self.n = len(self.prims)
+        ctx._children.add(self)
 
     def close(self) -> None:
-        if self.handle:
+        # the context releases its scenes when it closes first
+        if self.handle and self.ctx.handle:
             lib().rf_scene_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
@@ -224,6 +230,7 @@ class Stream:
         self.ctx = ctx
         self.handle = h
         self._n = {}
+        ctx._children.add(self)
 
     def submit(self, prims: np.ndarray, opts: SolveOpts) -> int:
         prims = np.ascontiguousarray(prims, dtype=PRIM_DTYPE)
@@ -252,9 +259,15 @@ class Stream:
         return lib().rf_stream_resident_bytes(self.handle)
 
     def close(self) -> None:
-        if self.handle:
+        if self.handle and self.ctx.handle:
             lib().rf_stream_destroy(self.handle)
-            self.handle = None
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def scene_footprint(n_prims: int, n_bvs: int = 0) -> int:

@@ -1,0 +1,95 @@
+"""Edge cases and error paths of the GPU dispatch."""
+import numpy as np
+import pytest
+
+import rayfactorcl_amd as rf
+from rayfactorcl_amd import _lib, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def test_single_primitive_scene(ctx, oracle, golden_scene):
+    sc = golden_scene("input-uni")
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    r = rf.solve(ctx, dev, math="compat", rays_per_emitter=4096)
+    want, _ = oracle.solve(sc.prims, None, n_wi=1024, mode="compat")
+    assert np.array_equal(r.hits, want)
+
+
+def test_zero_work_items(ctx, golden_scene):
+    sc = golden_scene("input-C14")
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    r = rf.solve(ctx, dev, math="compat", n_samples=64)     # 4*64/4 = 64 -> rounds down to 0 (Q5)
+    assert list(r.rays) == [0, 0] and r.hits.sum() == 0
+
+
+def test_invalid_scenes_are_rejected(ctx, golden_scene):
+    sc = golden_scene("input-C14")
+    bad = sc.prims.copy()
+    bad[1]["type"] = rf.NONE
+    with pytest.raises(rf.RayFactorError) as e:
+        rf.DeviceScene(ctx, bad, None)
+    assert e.value.code == _lib.RF_ERR_INVALID_SCENE
+    bvs = golden_scene("input-C14-BV").bvs.copy()
+    bvs[1]["objEndIndex"] = 5
+    with pytest.raises(rf.RayFactorError) as e:
+        rf.DeviceScene(ctx, sc.prims, bvs)
+    assert e.value.code == _lib.RF_ERR_INVALID_SCENE
+    with pytest.raises(rf.RayFactorError):
+        rf.DeviceScene(ctx, sc.prims[:0], None)
+
+
+def test_bad_options_are_rejected(ctx, golden_scene):
+    sc = golden_scene("input-C14")
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    with pytest.raises(rf.RayFactorError) as e:
+        rf.solve(ctx, dev, math="fast", rays_per_emitter=4096, rays_per_thread=3)
+    assert e.value.code == _lib.RF_ERR_INVALID_ARG
+    with pytest.raises(rf.RayFactorError) as e:
+        rf.solve(ctx, dev, math="compat", rays_per_emitter=4 * (1 << 32) + 8)   # reference RNG: 32-bit gid
+    assert e.value.code == _lib.RF_ERR_OVERFLOW
+    with pytest.raises(rf.RayFactorError) as e:
+        rf.solve(ctx, dev, math="compat", n_samples=2 ** 30)                    # rayDensity*n_samples > int32
+    assert e.value.code == _lib.RF_ERR_OVERFLOW
+
+
+def test_large_scene_global_atomic_tally(ctx):
+    # more primitives than LDS bins: per-ray global atomics; rows agree with traces
+    sc = scenes.parse_xml(scenes.furnace_xml(wall_bands=16500, preform_bands=10))
+    assert sc.n > 16000
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    r = rf.solve(ctx, dev, math="fast", rays_per_emitter=256, rng="seeded", form="persistent")
+    for e in [0, 8000, sc.n - 1]:
+        tr = rf.trace(ctx, dev, e, 0, 64, math="fast", rng="seeded")
+        row = np.bincount(tr[tr >= 0], minlength=sc.n)
+        assert np.array_equal(row, r.hits[e])
+
+
+def test_launch_emitter_accumulates(ctx, oracle, golden_scene):
+    sc = golden_scene("config1")
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    o = rf.make_opts(math="compat", rays_per_emitter=0)
+    dev.reset_hits()
+    dev.launch_emitter(2, 0, 1000, o)
+    dev.launch_emitter(2, 1000, 3000, o)
+    dev.launch_emitter(0, 500, 700, o)
+    ctx.sync()
+    h = dev.hits()
+    row2 = oracle.solve_range(sc.prims, None, 2, 0, 4000, mode="compat")
+    row0 = oracle.solve_range(sc.prims, None, 0, 500, 700, mode="compat")
+    assert np.array_equal(h[2], row2) and np.array_equal(h[0], row0) and h[[1, 3, 4]].sum() == 0
+
+
+def test_config1_closed_forms_at_scale(ctx, golden_scene):
+    sc = golden_scene("config1")
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    rays = 10_000_000
+    r = rf.solve(ctx, dev, math="fast", rays_per_emitter=rays, rng="seeded", form="persistent")
+    F = r.vf
+    sig = lambda f: np.sqrt(f * (1 - f) / rays) * 2.0   # 4-ray work items share a block: x2 margin
+    for (i, j, exact, allow) in [(1, 2, 0.1715729, 0), (1, 0, 0.8284271, 1.5e-3), (0, 1, 0.2071068, 1.5e-3),
+                                 (0, 0, 0.5857864, 1.5e-3), (3, 4, 0.3819660, 0)]:
+        assert abs(F[i, j] - exact) <= 4 * sig(exact) + allow, (i, j, F[i, j])
+    assert np.all(F[:3].sum(axis=1) > 0.998)
+    # reciprocity A_i F_ij = A_j F_ji (wall area 4 pi, discs pi)
+    assert 4 * F[0, 1] == pytest.approx(F[1, 0], rel=2e-3)
