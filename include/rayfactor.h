@@ -120,8 +120,16 @@ typedef struct rf_solve_opts {
     int32_t refine;           /* RF_MATH_FAST: 1 = FP64 re-solve of flagged roots */
     int32_t rays_per_thread;  /* 0 = autotuned/default; else 1, 2, 4 or 8 */
     int32_t block_size;       /* 0 = autotuned/default; else 64..256, multiple of 64 */
-    int32_t reserved;
+    int32_t transform;        /* RF_MATH_FAST plain form: rf_transform */
+    float refine_margin;      /* RF_MATH_FAST: relative margin of the FP64 re-solve flags, 0 = 2^-14;
+                                 larger re-solves more pairs (at most 0.25) */
 } rf_solve_opts;
+
+typedef enum rf_transform {
+    RF_TRANSFORM_AUTO = 0,    /* autotuned choice (VALU until tuned) */
+    RF_TRANSFORM_VALU = 1,    /* ray -> object transform on the vector ALUs */
+    RF_TRANSFORM_MFMA = 2     /* ray -> object transform on the matrix cores (v_mfma_f32_16x16x4f32) */
+} rf_transform;
 
 /* Fills *opts with defaults: COMPAT math, reference RNG and budget,
  * n_samples = 1000000, per-emitter form, refine = 1. */
@@ -136,6 +144,7 @@ typedef struct rf_timing {
     int64_t flops;            /* algorithmic FP32 flops: sum over tests of F_type */
     int64_t refined;          /* (ray, primitive) pairs re-solved in FP64 */
     int64_t launches;         /* kernel launches issued */
+    int64_t unresolved;       /* flagged pairs kept at their FP32 root (re-solve list full) */
 } rf_timing;
 
 /* ------------------------------------------------------------------ */
@@ -226,10 +235,11 @@ int32_t rf_batch_solve(rf_ctx *ctx, rf_scene *const *scenes, int32_t n_geoms,
                        const rf_solve_opts *opts, int64_t *const *hits, int64_t *const *rays,
                        rf_timing *timing);
 
-/* Autotuner: times the FAST kernel over {rays per thread} x {block size} on
- * the given scenes (a bounded sample of their work) and stores the fastest
- * choice in the context, where later solves with rays_per_thread == 0 and
- * block_size == 0 use it.  best_rpt / best_block may be NULL. */
+/* Autotuner: times the FAST kernel over {VALU transform x rays per thread,
+ * MFMA transform} x {block size} on the given scenes (a bounded sample of
+ * their work) and stores the fastest choice in the context, where later
+ * solves with rays_per_thread == 0, block_size == 0 and transform == AUTO use
+ * it.  best_rpt (0 for the MFMA transform) / best_block may be NULL. */
 int32_t rf_autotune(rf_ctx *ctx, rf_scene *const *scenes, int32_t n_geoms,
                     const rf_solve_opts *opts, int32_t *best_rpt, int32_t *best_block);
 

@@ -41,6 +41,7 @@ RF_ERR_NO_DEVICE = -9
 MATH = {"compat": 0, "fp64": 1, "fast": 2}
 RNG = {"reference": 0, "seeded": 1}
 FORM = {"per_emitter": 0, "persistent": 1, "reference": 2}
+TRANSFORM = {"auto": 0, "valu": 1, "mfma": 2}
 
 
 class SolveOpts(ctypes.Structure):
@@ -48,14 +49,15 @@ class SolveOpts(ctypes.Structure):
         ("math", c_int32), ("rng", c_int32), ("use_bvs", c_int32), ("budget", c_int32),
         ("n_samples", c_int32), ("form", c_int32), ("rays_per_emitter", c_int64),
         ("seed", c_uint32), ("geometry", c_uint32), ("refine", c_int32),
-        ("rays_per_thread", c_int32), ("block_size", c_int32), ("reserved", c_int32),
+        ("rays_per_thread", c_int32), ("block_size", c_int32), ("transform", c_int32),
+        ("refine_margin", c_float),
     ]
 
 
 class Timing(ctypes.Structure):
     _fields_ = [
         ("kernel_ms", c_double), ("wall_ms", c_double), ("rays", c_int64), ("tests", c_int64),
-        ("flops", c_int64), ("refined", c_int64), ("launches", c_int64),
+        ("flops", c_int64), ("refined", c_int64), ("launches", c_int64), ("unresolved", c_int64),
     ]
 
     def as_dict(self) -> dict:

@@ -30,6 +30,8 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]
+# kernels: MFMA results written straight to VGPRs (no accumulator-register copies)
+KERNEL_FLAGS = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 SOURCES = ["rf_kernels.hip", "rf_runtime.cpp", "rf_xml.cpp"]
 
@@ -57,7 +59,8 @@ def build(verbose: bool = False) -> Path:
     for src in SOURCES:
         obj = BUILD_DIR / (src + ".o")
         objs.append(obj)
-        cmd = [HIPCC, f"--offload-arch={ARCH}", *COMMON, "-c", str(CSRC / src), "-o", str(obj)]
+        extra = KERNEL_FLAGS if src.endswith(".hip") else []
+        cmd = [HIPCC, f"--offload-arch={ARCH}", *COMMON, *extra, "-c", str(CSRC / src), "-o", str(obj)]
         cmds.append(cmd)
     if verbose:
         for c in cmds:

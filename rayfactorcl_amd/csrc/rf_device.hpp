@@ -391,6 +391,25 @@ __device__ __forceinline__ double hit_d(int type, const float *__restrict__ iv, 
     }
 }
 
+// FP64 re-solve of a flagged quadric pair (cylinder, frustum, sphere) in one
+// straight-line form: the same A, B, C and root selection as hit_d for these
+// types, without the plane branches (fewer live registers where it is inlined).
+__device__ __forceinline__ double hit_d_quadric(int type, const float *__restrict__ iv, float u1, const Ray64 &w)
+{
+    const Ray64 o = xfrm_d(iv, w);
+    const bool fr = type == RF_FRUSTUM, sp = type == RF_SPHERE;
+    const double k = fr ? (double)u1 - 1.0 : 0.0;
+    const double ws = 1.0 + k * o.sz;
+    const double wc = k * o.cz;
+    double A = o.cx * o.cx + o.cy * o.cy;
+    double B = o.sx * o.cx + o.sy * o.cy;
+    double C = o.sx * o.sx + o.sy * o.sy;
+    if (sp) { A = A + o.cz * o.cz; B = B + o.sz * o.cz; C = (C + o.sz * o.sz) - 1.0; }
+    else if (fr) { A = A - wc * wc; B = B - ws * wc; C = C - ws * ws; }
+    else C = C - 1.0;
+    return quad_select_d(A, B, C, o, !sp);
+}
+
 // Physics emission (FP64).  Point parametrisation as RayFactorGPU.cl:298-443
 // (frustum z in the cancellation-free form); frame: n = normalize(invm^T
 // n_os), tangent = the world image of a fixed object axis made orthogonal to
@@ -550,9 +569,13 @@ struct FastPrim {
     int32_t type;
 };
 
-// Thresholds of the FP32 conditioning flags (relative).
-constexpr float TAU_D = 0x1p-10f;   // |D| <= TAU_D * b^2: near-tangent, root separation ill-conditioned
-constexpr float TAU_Z = 0x1p-12f;   // root within TAU_Z of a rim (normalised coordinate)
-constexpr float TAU_A = 0x1p-10f;   // frustum: ray nearly along a generator (|A| small)
+// Default relative margin tau of the FP32 conditioning flags; a flagged pair
+// is re-solved in FP64.  The one margin guards three quantities, each several
+// times (2^10) its FP32 error (transform + quadric in single precision):
+//   |D| <= tau * b^2 with the tangent point ahead of the ray and on the band
+//       (near-tangent: the root separation is ill-conditioned);
+//   a root ahead of the ray within tau of a rim (band-height units);
+//   frustum: |A| <= tau * (cx^2 + cy^2) (ray nearly along a generator).
+constexpr float TAU_DEFAULT = 0x1p-14f;
 
 }  // namespace rfd

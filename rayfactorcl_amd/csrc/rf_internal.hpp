@@ -35,6 +35,14 @@ struct GeomDesc {
     int32_t seg_end[N_SEG];
     uint32_t geometry_id;               // seeded RNG geometry word
     int32_t pad;
+    // MFMA form: primitive quads (64 floats each: the 16x4 A operand of
+    // v_mfma_f32_16x16x4f32, element A[4p + row][k] at a[16k + 4p + row]),
+    // type segments padded to whole quads, and traversal position -> index.
+    const float *quads;
+    const int32_t *pos2orig;
+    const int32_t *orig2pos;
+    int32_t qseg_begin[N_SEG];
+    int32_t qseg_end[N_SEG];
 };
 
 // One work unit: work items [wi0, wi0+n_wi) of one emitter of one geometry.
@@ -66,6 +74,7 @@ struct KParams {
     int64_t imp_wi0;
     int64_t imp_n_wi;
     int64_t imp_unit;
+    float tau;                          // fast form: relative margin of the FP64 re-solve flags
 };
 
 enum KMath { KM_COMPAT = 0, KM_FP64 = 1, KM_FAST = 2 };
@@ -79,6 +88,7 @@ struct LaunchCfg {
     int block;       // threads per workgroup
     int grid;        // workgroups
     int direct;      // 1: reference form (one work item per thread, global atomics, no queue)
+    int mfma;        // 1: fast plain form with the transform on the matrix cores
     size_t lds_bytes;
 };
 

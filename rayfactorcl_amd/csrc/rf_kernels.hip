@@ -142,7 +142,7 @@ __device__ __forceinline__ int find_bv_cd(CPTR(rf_primitive) P, CPTR(rf_bounding
 #define ROW_C(P, i, R) __builtin_fmaf(P.r[4 * i], R.cx, __builtin_fmaf(P.r[4 * i + 1], R.cy, P.r[4 * i + 2] * R.cz))
 
 template <int KIND>
-__device__ __forceinline__ float fast_test(const FastPrim &P, const Ray32 &R, bool &flag)
+__device__ __forceinline__ float fast_test(const FastPrim &P, const Ray32 &R, bool &flag, float tau)
 {
     if constexpr (KIND == SEG_CYL) {
         const float sx = ROW_S(P, 0, R), sy = ROW_S(P, 1, R), sz = ROW_S(P, 2, R);
@@ -159,9 +159,13 @@ __device__ __forceinline__ float fast_test(const FastPrim &P, const Ray32 &R, bo
         const float z2 = __builtin_fmaf(cz, t2, sz);
         const bool v1 = (t1 > EPS_F) & (__builtin_fabsf(z1) <= 0.5f);
         const bool v2 = (t2 > EPS_F) & (__builtin_fabsf(z2) <= 0.5f);
-        flag = (__builtin_fabsf(D) <= TAU_D * (b * b)) |
-               (__builtin_fabsf(__builtin_fabsf(z1) - 0.5f) <= TAU_Z) |
-               (__builtin_fabsf(__builtin_fabsf(z2) - 0.5f) <= TAU_Z);
+        // near-tangent only matters where the tangent point lies ahead of the
+        // ray and on this band; a rim only for a root ahead of the ray
+        const float tm = -b * rA;
+        const float zm = __builtin_fmaf(cz, tm, sz);
+        flag = ((__builtin_fabsf(D) <= tau * (b * b)) & (tm > 0.0f) & (__builtin_fabsf(zm) <= 0.5f + tau)) |
+               ((t1 > 0.0f) & (__builtin_fabsf(__builtin_fabsf(z1) - 0.5f) <= tau)) |
+               ((t2 > 0.0f) & (__builtin_fabsf(__builtin_fabsf(z2) - 0.5f) <= tau));
         return v1 ? t1 : (v2 ? t2 : finf());
     } else if constexpr (KIND == SEG_FRUSTUM) {
         const float sx = ROW_S(P, 0, R), sy = ROW_S(P, 1, R), sw = ROW_S(P, 2, R);
@@ -181,10 +185,13 @@ __device__ __forceinline__ float fast_test(const FastPrim &P, const Ray32 &R, bo
         const float w2 = __builtin_fmaf(cw, t2, sw) - P.k0;
         const bool v1 = (t1 > EPS_F) & (__builtin_fabsf(w1) <= P.k1);
         const bool v2 = (t2 > EPS_F) & (__builtin_fabsf(w2) <= P.k1);
-        const float rim = (2.0f * TAU_Z) * P.k1;
-        flag = (__builtin_fabsf(D) <= TAU_D * (b * b)) | (__builtin_fabsf(A) <= TAU_A * Axy) |
-               (__builtin_fabsf(__builtin_fabsf(w1) - P.k1) <= rim) |
-               (__builtin_fabsf(__builtin_fabsf(w2) - P.k1) <= rim);
+        const float rim = (2.0f * tau) * P.k1;
+        const float tm = -b * rA;
+        const float wm = __builtin_fmaf(cw, tm, sw) - P.k0;
+        flag = ((__builtin_fabsf(D) <= tau * (b * b)) & (tm > 0.0f) & (__builtin_fabsf(wm) <= P.k1 + rim)) |
+               (__builtin_fabsf(A) < tau * Axy) |
+               ((t1 > 0.0f) & (__builtin_fabsf(__builtin_fabsf(w1) - P.k1) <= rim)) |
+               ((t2 > 0.0f) & (__builtin_fabsf(__builtin_fabsf(w2) - P.k1) <= rim));
         return v1 ? t1 : (v2 ? t2 : finf());
     } else if constexpr (KIND == SEG_SPHERE) {
         const float sx = ROW_S(P, 0, R), sy = ROW_S(P, 1, R), sz = ROW_S(P, 2, R);
@@ -197,7 +204,7 @@ __device__ __forceinline__ float fast_test(const FastPrim &P, const Ray32 &R, bo
         const float rA = rcp_fast(A);
         const float t1 = (-b - sq) * rA;
         const float t2 = (sq - b) * rA;
-        flag = (__builtin_fabsf(D) <= TAU_D * (b * b));
+        flag = (__builtin_fabsf(D) <= tau * (b * b)) & (b < 0.0f);
         return (t1 > EPS_F) ? t1 : ((t2 > EPS_F) ? t2 : finf());
     } else {
         // planes: object z = 0; hit point found in world space, then mapped
@@ -220,14 +227,14 @@ __device__ __forceinline__ float fast_test(const FastPrim &P, const Ray32 &R, bo
     }
 }
 
-__device__ __forceinline__ float fast_test_any(const FastPrim &P, const Ray32 &R, bool &flag)
+__device__ __forceinline__ float fast_test_any(const FastPrim &P, const Ray32 &R, bool &flag, float tau)
 {
     switch (P.type) {
-    case RF_CYLINDER: return fast_test<SEG_CYL>(P, R, flag);
-    case RF_FRUSTUM: return fast_test<SEG_FRUSTUM>(P, R, flag);
-    case RF_SPHERE: return fast_test<SEG_SPHERE>(P, R, flag);
-    case RF_RECTANGLE: return fast_test<SEG_RECT>(P, R, flag);
-    default: return fast_test<SEG_RING>(P, R, flag);
+    case RF_CYLINDER: return fast_test<SEG_CYL>(P, R, flag, tau);
+    case RF_FRUSTUM: return fast_test<SEG_FRUSTUM>(P, R, flag, tau);
+    case RF_SPHERE: return fast_test<SEG_SPHERE>(P, R, flag, tau);
+    case RF_RECTANGLE: return fast_test<SEG_RECT>(P, R, flag, tau);
+    default: return fast_test<SEG_RING>(P, R, flag, tau);
     }
 }
 
@@ -258,20 +265,50 @@ __device__ __attribute__((noinline)) float refine_bv(CPTR(rf_bounding_volume) B,
 // Deferred FP64 re-solves of the plain fast kernel: a flagged (ray, primitive)
 // pair is kept out of the FP32 minimum and queued in a per-lane list in LDS
 // (PEND_K entries per lane, strided by the workgroup size); after the scan the
-// list is re-solved in FP64 and merged into the nearest hit.  A lane whose
-// list overflows re-traces that ray entirely in FP64.  No FP64 code runs
-// inside the primitive loop.
+// list is re-solved in FP64 and merged into the nearest hit.  No FP64 code
+// runs inside the primitive loop.  Once the list is full, flagged pairs are
+// dropped and the first dropped key (position x 8 + ray slot) is kept in
+// `resume`; the lane then re-scans the quadrics one ray slot at a time from
+// there (pairs below `from` masked), re-solving as it goes, so every flagged
+// pair is re-solved exactly once whatever the number of rays per thread: the
+// result is the same for every launch shape.
 constexpr int PEND_K = 4;
+constexpr int NO_KEY = 0x7fffffff;
 
 struct Pending {
-    uint32_t *buf;     // LDS, PEND_K * blockDim entries: (orig << 3) | ray slot
+    uint32_t *buf;     // LDS, PEND_K * blockDim entries: (index << 3) | ray slot
     int count;
-    uint32_t overflow; // bit i: ray slot i needs a full FP64 re-trace
+    int resume;        // first dropped pair key, NO_KEY if none
+    int from;          // re-scan: pairs with key < from are skipped
+    float tau;         // relative flag margin
 };
 
-template <int KIND, int NR, bool REFINE>
+// Wave-wide minimum of a lane value (re-scan start).
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const int o = __shfl_xor(v, off);
+        v = o < v ? o : v;
+    }
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// An integer zero the optimiser cannot see through: keeps a recomputation
+// from being merged with the first computation (which would keep the first
+// result's registers live across the FP64 stage).
+__device__ __forceinline__ int opaque_zero()
+{
+    int z = 0;
+    asm volatile("" : "+v"(z));
+    return z;
+}
+
+// RESCAN: one ray (NR = 1) of lane slot `slot`, positions below pd.from
+// skipped, `resume` records a position instead of a key.
+template <int KIND, int NR, bool REFINE, bool RESCAN>
 __device__ __forceinline__ void seg_loop(CPTR(FastPrim) F, int begin, int end, const Ray32 (&R)[NR],
-                                         float (&tb)[NR], int (&ib)[NR], Pending &pd)
+                                         float (&tb)[NR], int (&ib)[NR], Pending &pd, int slot = 0)
 {
     for (int w = begin; w < end; ++w) {
         const FastPrim q = ldc(F + w);
@@ -280,7 +317,12 @@ __device__ __forceinline__ void seg_loop(CPTR(FastPrim) F, int begin, int end, c
         bool any = false;
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
-            t[i] = fast_test<KIND>(q, R[i], f[i]);
+            t[i] = fast_test<KIND>(q, R[i], f[i], pd.tau);
+            if constexpr (RESCAN) {
+                const bool skip = w < pd.from;
+                f[i] = f[i] & !skip;
+                t[i] = skip ? finf() : t[i];
+            }
             any |= f[i];
         }
         if constexpr (REFINE && KIND <= SEG_SPHERE) {
@@ -289,10 +331,11 @@ __device__ __forceinline__ void seg_loop(CPTR(FastPrim) F, int begin, int end, c
                 for (int i = 0; i < NR; ++i) {
                     if (f[i]) {
                         if (pd.count < PEND_K) {
-                            pd.buf[pd.count * blockDim.x + threadIdx.x] = ((uint32_t)q.orig << 3) | (uint32_t)i;
+                            pd.buf[pd.count * blockDim.x + threadIdx.x] =
+                                ((uint32_t)q.orig << 3) | (uint32_t)(RESCAN ? slot : i);
                             ++pd.count;
-                        } else {
-                            pd.overflow |= 1u << i;
+                        } else if (pd.resume == NO_KEY) {
+                            pd.resume = RESCAN ? w : w * 8 + i;
                         }
                         t[i] = finf();
                     }
@@ -308,50 +351,75 @@ __device__ __forceinline__ void seg_loop(CPTR(FastPrim) F, int begin, int end, c
 // FP64 stage of the deferred re-solves (see Pending).  The flagged ray is
 // regenerated from its index (emission is a pure function of the random
 // block), so the scan's ray registers are dead here.
-template <int NR>
+// ONE: every entry belongs to one ray, merged into tb[0] / ib[0].
+template <int NR, bool ONE = false>
 __device__ __forceinline__ void resolve_pending(const KParams &kp, CPTR(GeomDesc) G, const rf_primitive &E,
                                                 uint32_t gid, int emitter, int64_t wi0, int64_t base,
                                                 float (&tb)[NR], int (&ib)[NR], Pending &pd, uint32_t &nref)
 {
     CPTR(rf_primitive) P = cst(G->prims);
-    const int n = G->n;
-    for (int e = 0; e < pd.count + __builtin_popcount(pd.overflow); ++e) {
-        int prim_lo, prim_hi, k;
-        if (e < pd.count) {
-            const uint32_t ent = pd.buf[e * blockDim.x + threadIdx.x];
-            prim_lo = (int)(ent >> 3);
-            prim_hi = prim_lo + 1;
-            k = (int)(ent & 7u);
-        } else {
-            // a slot whose list overflowed: re-trace the whole ray in FP64
-            int m = e - pd.count;
-            uint32_t ov = pd.overflow;
-            while (m--) ov &= ov - 1;
-            k = __builtin_ctz(ov);
-            prim_lo = 0;
-            prim_hi = n;
-        }
+    for (int e = 0; e < pd.count; ++e) {
+        const uint32_t ent = pd.buf[e * blockDim.x + threadIdx.x];
+        const int prim = (int)(ent >> 3);
+        const int k = (int)(ent & 7u);
         const int64_t rl = base + k;
         float f[4], r4[4];
         workitem_floats(kp, gid, emitter, wi0 + (rl >> 2), f);
         rotate_rands(f, (int)(rl & 3), r4);
         const Ray64 r64 = widen(emit_fast(&E, r4));
-        float tk = finf();
-        int ik = -1;
+        const rf_primitive p = ldc(P + prim);
+        const float t = (float)hit_d_quadric(p.type, p.invm, p.util1, r64);
 #pragma unroll
         for (int i = 0; i < NR; ++i)
-            if (k == i) { tk = tb[i]; ik = ib[i]; }
-        if (prim_hi - prim_lo > 1) { tk = finf(); ik = -1; }
-        for (int w = prim_lo; w < prim_hi; ++w) {
-            const rf_primitive p = ldc(P + w);
-            const float t = (float)hit_d(p.type, p.invm, p.util1, p.util2, r64);
-            if (t < tk || (t == tk && w < ik)) { tk = t; ik = w; }
+            if ((ONE || k == i) && (t < tb[i] || (t == tb[i] && prim < ib[i]))) { tb[i] = t; ib[i] = prim; }
+    }
+    nref += (uint32_t)pd.count;
+    pd.count = 0;
+}
+
+// Pairs dropped from a full list (first dropped key pd.resume): re-scan the
+// quadrics for one ray slot at a time, from the first position whose pair of
+// that slot may have been dropped, re-solving each time the list fills.
+template <int NR>
+__device__ __forceinline__ void finish_dropped(const KParams &kp, CPTR(GeomDesc) G, const rf_primitive &E,
+                                               uint32_t gid, int emitter, int64_t wi0, int64_t base,
+                                               float (&tb)[NR], int (&ib)[NR], Pending &pd, uint32_t &nref,
+                                               uint32_t &nover)
+{
+    const int K = pd.resume;
+    CPTR(FastPrim) F = cst(G->fast);
+#pragma unroll 1
+    for (int i = 0; i < NR; ++i) {
+        int from = K == NO_KEY ? NO_KEY : (K >> 3) + (i < (K & 7) ? 1 : 0);
+        float tr[1] = {finf()};
+        int ir[1] = {-1};
+        while (__builtin_amdgcn_ballot_w64(from != NO_KEY)) {
+            nover += from != NO_KEY;
+            Ray32 R1[1];
+            {
+                float f[4], r4[4];
+                const int64_t rl = base + i + opaque_zero();
+                workitem_floats(kp, gid, emitter, wi0 + (rl >> 2), f);
+                rotate_rands(f, (int)(rl & 3), r4);
+                R1[0] = emit_fast(&E, r4);
+            }
+            pd.count = 0;
+            pd.resume = NO_KEY;
+            pd.from = from;
+            const int lo = wave_min(from);
+            const auto beg = [&](int sg) { return G->seg_begin[sg] > lo ? G->seg_begin[sg] : lo; };
+            seg_loop<SEG_CYL, 1, true, true>(F, beg(SEG_CYL), G->seg_end[SEG_CYL], R1, tr, ir, pd, i);
+            seg_loop<SEG_FRUSTUM, 1, true, true>(F, beg(SEG_FRUSTUM), G->seg_end[SEG_FRUSTUM], R1, tr, ir, pd, i);
+            seg_loop<SEG_SPHERE, 1, true, true>(F, beg(SEG_SPHERE), G->seg_end[SEG_SPHERE], R1, tr, ir, pd, i);
+            if (__builtin_amdgcn_ballot_w64(pd.count > 0))
+                resolve_pending<1, true>(kp, G, E, gid, emitter, wi0, base, tr, ir, pd, nref);
+            from = pd.resume;
         }
 #pragma unroll
-        for (int i = 0; i < NR; ++i)
-            if (k == i) { tb[i] = tk; ib[i] = ik; }
-        nref += (uint32_t)(prim_hi - prim_lo);
+        for (int k = 0; k < NR; ++k)
+            if (k == i && (tr[0] < tb[k] || (tr[0] == tb[k] && ir[0] < ib[k]))) { tb[k] = tr[0]; ib[k] = ir[0]; }
     }
+    pd.resume = NO_KEY;
 }
 
 template <int NR, bool REFINE>
@@ -359,24 +427,42 @@ __device__ __forceinline__ void scan_fast(CPTR(GeomDesc) G, const Ray32 (&R)[NR]
                                           int (&ib)[NR], Pending &pd)
 {
     CPTR(FastPrim) F = cst(G->fast);
-    seg_loop<SEG_CYL, NR, REFINE>(F, G->seg_begin[SEG_CYL], G->seg_end[SEG_CYL], R, tb, ib, pd);
-    seg_loop<SEG_FRUSTUM, NR, REFINE>(F, G->seg_begin[SEG_FRUSTUM], G->seg_end[SEG_FRUSTUM], R, tb, ib, pd);
-    seg_loop<SEG_SPHERE, NR, REFINE>(F, G->seg_begin[SEG_SPHERE], G->seg_end[SEG_SPHERE], R, tb, ib, pd);
-    seg_loop<SEG_RING, NR, REFINE>(F, G->seg_begin[SEG_RING], G->seg_end[SEG_RING], R, tb, ib, pd);
-    seg_loop<SEG_RECT, NR, REFINE>(F, G->seg_begin[SEG_RECT], G->seg_end[SEG_RECT], R, tb, ib, pd);
+    seg_loop<SEG_CYL, NR, REFINE, false>(F, G->seg_begin[SEG_CYL], G->seg_end[SEG_CYL], R, tb, ib, pd);
+    seg_loop<SEG_FRUSTUM, NR, REFINE, false>(F, G->seg_begin[SEG_FRUSTUM], G->seg_end[SEG_FRUSTUM], R, tb, ib, pd);
+    seg_loop<SEG_SPHERE, NR, REFINE, false>(F, G->seg_begin[SEG_SPHERE], G->seg_end[SEG_SPHERE], R, tb, ib, pd);
+    seg_loop<SEG_RING, NR, REFINE, false>(F, G->seg_begin[SEG_RING], G->seg_end[SEG_RING], R, tb, ib, pd);
+    seg_loop<SEG_RECT, NR, REFINE, false>(F, G->seg_begin[SEG_RECT], G->seg_end[SEG_RECT], R, tb, ib, pd);
+}
+
+// The NR rays of ray group `base` (unit-local ray index) of one emitter.
+template <int NR>
+__device__ __forceinline__ void emit_group(const KParams &kp, uint32_t gid, int emitter, int64_t wi0, int64_t base,
+                                           const rf_primitive &E, Ray32 (&R)[NR])
+{
+    float f[4];
+    int64_t cur_wi = -1;
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        const int64_t rl = base + i;
+        const int64_t wl = rl >> 2;
+        if (wl != cur_wi) { workitem_floats(kp, gid, emitter, wi0 + wl, f); cur_wi = wl; }
+        float r4[4];
+        rotate_rands(f, (int)(rl & 3), r4);
+        R[i] = emit_fast(&E, r4);
+    }
 }
 
 // Child scan of the fast BV kernel over the caller's order [start, end).
 template <bool REFINE>
 __device__ __forceinline__ int find_fast_range(CPTR(FastPrim) F, CPTR(rf_primitive) P, int start, int end,
-                                               const Ray32 &R, uint32_t &nref)
+                                               const Ray32 &R, uint32_t &nref, float tau)
 {
     int obj = -1;
     float tbest = finf();
     for (int w = start; w < end; ++w) {
         const FastPrim q = ldc(F + w);
         bool f;
-        float t = fast_test_any(q, R, f);
+        float t = fast_test_any(q, R, f, tau);
         if constexpr (REFINE) {
             if (__builtin_amdgcn_ballot_w64(f)) {
                 if (f) { t = refine_prim(P, w, R); ++nref; }
@@ -388,7 +474,8 @@ __device__ __forceinline__ int find_fast_range(CPTR(FastPrim) F, CPTR(rf_primiti
 }
 
 template <bool REFINE>
-__device__ __forceinline__ int find_bv_fast(CPTR(GeomDesc) G, const Ray32 &R, uint64_t &tests, uint32_t &nref)
+__device__ __forceinline__ int find_bv_fast(CPTR(GeomDesc) G, const Ray32 &R, uint64_t &tests, uint32_t &nref,
+                                            float tau)
 {
     CPTR(FastPrim) F = cst(G->fast_orig);
     CPTR(FastPrim) FB = cst(G->fast_bvs);
@@ -402,7 +489,7 @@ __device__ __forceinline__ int find_bv_fast(CPTR(GeomDesc) G, const Ray32 &R, ui
         const FastPrim q = ldc(FB + w);
         const BvMeta meta = ldc(M + w);
         bool f;
-        float t = fast_test_any(q, R, f);
+        float t = fast_test_any(q, R, f, tau);
         if constexpr (REFINE) {
             if (__builtin_amdgcn_ballot_w64(f)) {
                 if (f) { t = refine_bv(B, w, R); ++nref; }
@@ -411,7 +498,7 @@ __device__ __forceinline__ int find_bv_fast(CPTR(GeomDesc) G, const Ray32 &R, ui
         ++tests;
         if (t < tbest) {
             if (meta.settings == 1) {
-                objectIndex = find_fast_range<REFINE>(F, P, meta.start, meta.end, R, nref);
+                objectIndex = find_fast_range<REFINE>(F, P, meta.start, meta.end, R, nref, tau);
                 tests += (uint64_t)(meta.end - meta.start);
                 if (objectIndex != -1) { tbest = t; bvIndex = w; }
             } else {
@@ -427,7 +514,7 @@ __device__ __forceinline__ int find_bv_fast(CPTR(GeomDesc) G, const Ray32 &R, ui
         const int bsel = __builtin_amdgcn_readlane(bvIndex, leader);
         if (pending && bvIndex == bsel) {
             const BvMeta meta = ldc(M + bsel);
-            objectIndex = find_fast_range<REFINE>(F, P, meta.start, meta.end, R, nref);
+            objectIndex = find_fast_range<REFINE>(F, P, meta.start, meta.end, R, nref, tau);
             tests += (uint64_t)(meta.end - meta.start);
             pending = false;
         }
@@ -460,7 +547,8 @@ __device__ __forceinline__ void tally(const KParams &kp, CPTR(GeomDesc) G, int e
 template <int MATH, bool BV, bool TRACE, int NR, bool REFINE, bool DIRECT>
 __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, uint32_t gid, int emitter,
                                          int64_t wi0, int64_t n_wi, int64_t first, int64_t stride,
-                                         uint32_t *bins, uint32_t *pend, uint64_t &tests, uint32_t &nref)
+                                         uint32_t *bins, uint32_t *pend, uint64_t &tests, uint32_t &nref,
+                                         uint32_t &nover)
 {
     const rf_primitive E = ldc(cst(G->prims) + emitter);
     const int64_t ray0 = (wi0 - kp.trace_wi0) * 4;   // absolute ray index base (trace launches)
@@ -471,24 +559,16 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
             Ray32 R[NR];
             float tb[NR];
             int ib[NR];
-            float f[4];
-            int64_t cur_wi = -1;
+            emit_group<NR>(kp, gid, emitter, wi0, base, E, R);
 #pragma unroll
-            for (int i = 0; i < NR; ++i) {
-                const int64_t rl = base + i;
-                const int64_t wl = rl >> 2;
-                if (wl != cur_wi) { workitem_floats(kp, gid, emitter, wi0 + wl, f); cur_wi = wl; }
-                float r4[4];
-                rotate_rands(f, (int)(rl & 3), r4);
-                R[i] = emit_fast(&E, r4);
-                tb[i] = finf();
-                ib[i] = -1;
-            }
-            Pending pd{pend, 0, 0u};
+            for (int i = 0; i < NR; ++i) { tb[i] = finf(); ib[i] = -1; }
+            Pending pd{pend, 0, NO_KEY, 0, kp.tau};
             scan_fast<NR, REFINE>(G, R, tb, ib, pd);
             if constexpr (REFINE) {
-                if (__builtin_amdgcn_ballot_w64(pd.count > 0 || pd.overflow != 0))
+                if (__builtin_amdgcn_ballot_w64(pd.count > 0))
                     resolve_pending<NR>(kp, G, E, gid, emitter, wi0, base, tb, ib, pd, nref);
+                if (__builtin_amdgcn_ballot_w64(pd.resume != NO_KEY))
+                    finish_dropped<NR>(kp, G, E, gid, emitter, wi0, base, tb, ib, pd, nref, nover);
             }
 #pragma unroll
             for (int i = 0; i < NR; ++i)
@@ -515,16 +595,18 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                 } else {
                     const Ray32 ray = emit_fast(&E, r4);
                     if constexpr (BV) {
-                        idx = find_bv_fast<REFINE>(G, ray, tests, nref);
+                        idx = find_bv_fast<REFINE>(G, ray, tests, nref, kp.tau);
                     } else {
                         Ray32 R1[1] = {ray};
                         float tb1[1] = {finf()};
                         int ib1[1] = {-1};
-                        Pending pd{pend, 0, 0u};
+                        Pending pd{pend, 0, NO_KEY, 0, kp.tau};
                         scan_fast<1, REFINE>(G, R1, tb1, ib1, pd);
                         if constexpr (REFINE) {
-                            if (__builtin_amdgcn_ballot_w64(pd.count > 0 || pd.overflow != 0))
+                            if (__builtin_amdgcn_ballot_w64(pd.count > 0))
                                 resolve_pending<1>(kp, G, E, gid, emitter, wi0, wl * 4 + v, tb1, ib1, pd, nref);
+                            if (__builtin_amdgcn_ballot_w64(pd.resume != NO_KEY))
+                                finish_dropped<1>(kp, G, E, gid, emitter, wi0, wl * 4 + v, tb1, ib1, pd, nref, nover);
                         }
                         idx = ib1[0];
                     }
@@ -535,24 +617,346 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
     }
 }
 
-__device__ __forceinline__ void flush_stats(const KParams &kp, uint64_t tests, uint32_t nref)
+// stats[0]: tests counted on the device (BV forms); stats[1]: pairs re-solved
+// in FP64; stats[2]: flagged pairs left at their FP32 root (full list).
+__device__ __forceinline__ void flush_stats(const KParams &kp, uint64_t tests, uint32_t nref, uint32_t nover)
 {
-    __shared__ unsigned long long s_stats[2];
-    if (threadIdx.x == 0) { s_stats[0] = 0; s_stats[1] = 0; }
+    __shared__ unsigned long long s_stats[3];
+    if (threadIdx.x < 3) s_stats[threadIdx.x] = 0;
     __syncthreads();
     if (tests) atomicAdd(&s_stats[0], (unsigned long long)tests);
     if (nref) atomicAdd(&s_stats[1], (unsigned long long)nref);
+    if (nover) atomicAdd(&s_stats[2], (unsigned long long)nover);
     __syncthreads();
-    if (threadIdx.x == 0 && kp.stats) {
-        if (s_stats[0]) atomicAdd(&kp.stats[0], s_stats[0]);
-        if (s_stats[1]) atomicAdd(&kp.stats[1], s_stats[1]);
+    if (threadIdx.x < 3 && kp.stats && s_stats[threadIdx.x]) atomicAdd(&kp.stats[threadIdx.x], s_stats[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+// fast plain form with the ray transform on the matrix cores
+//
+// A wave traces 64 rays per pass as 4 sets of 16.  Primitives come in quads;
+// one coalesced 256-byte load per quad gives every lane one element of the
+// 16x4 A operand (4 primitives x 4 output rows x [x, y, z, 1]).  For each
+// set, two v_mfma_f32_16x16x4f32 transform the set's origins (B column
+// [s; 1]) and directions ([c; 0]); lane l then holds the four object-space
+// rows of ray (set, l % 16) for primitive slot l / 16 and runs only the
+// quadric / plane arithmetic on the VALU.  Each lane keeps the nearest hit
+// of its slot; a cross-lane minimum over the 4 slots finishes each ray.
+// Output rows per type (quads built on the host):
+//   cylinder  x, y, z-1/2, 0          frustum  x, y, w = 1+(u1-1)z, z-1/2
+//   sphere    x, y, z, 0              disc / annulus  x/ro, y/ro, z, (ri/ro)^2
+//   rectangle x, y, z, 0              padding  never hits or flags
+// ---------------------------------------------------------------------------
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int KIND>
+__device__ __forceinline__ float quad_test(const f32x4 &o, const f32x4 &d, bool &flag, float tau)
+{
+    if constexpr (KIND == SEG_CYL) {
+        const float sx = o.x, sy = o.y, sz = o.z, cx = d.x, cy = d.y, cz = d.z;
+        const float A = __builtin_fmaf(cx, cx, cy * cy);
+        const float b = __builtin_fmaf(sx, cx, sy * cy);
+        const float C = __builtin_fmaf(sx, sx, __builtin_fmaf(sy, sy, -1.0f));
+        const float D = __builtin_fmaf(b, b, -(A * C));
+        const float sq = sqrt_fast(D);
+        const float rA = rcp_fast(A);
+        const float t1 = (-b - sq) * rA;
+        const float t2 = (sq - b) * rA;
+        const float z1 = __builtin_fmaf(cz, t1, sz);
+        const float z2 = __builtin_fmaf(cz, t2, sz);
+        const bool v1 = (t1 > EPS_F) & (__builtin_fabsf(z1) <= 0.5f);
+        const bool v2 = (t2 > EPS_F) & (__builtin_fabsf(z2) <= 0.5f);
+        const float tm = -b * rA;
+        const float zm = __builtin_fmaf(cz, tm, sz);
+        flag = ((__builtin_fabsf(D) <= tau * (b * b)) & (tm > 0.0f) & (__builtin_fabsf(zm) <= 0.5f + tau)) |
+               ((t1 > 0.0f) & (__builtin_fabsf(__builtin_fabsf(z1) - 0.5f) <= tau)) |
+               ((t2 > 0.0f) & (__builtin_fabsf(__builtin_fabsf(z2) - 0.5f) <= tau));
+        return v1 ? t1 : (v2 ? t2 : finf());
+    } else if constexpr (KIND == SEG_FRUSTUM) {
+        const float sx = o.x, sy = o.y, sw = o.z, sz = o.w, cx = d.x, cy = d.y, cw = d.z, cz = d.w;
+        const float Axy = __builtin_fmaf(cx, cx, cy * cy);
+        const float A = __builtin_fmaf(-cw, cw, Axy);
+        const float b = __builtin_fmaf(sx, cx, __builtin_fmaf(sy, cy, -(sw * cw)));
+        const float C = __builtin_fmaf(sx, sx, __builtin_fmaf(sy, sy, -(sw * sw)));
+        const float D = __builtin_fmaf(b, b, -(A * C));
+        const float sq = sqrt_fast(D);
+        const float rA = rcp_fast(A);
+        const float ta = (-b - sq) * rA;
+        const float tb = (sq - b) * rA;
+        const float t1 = __builtin_fminf(ta, tb);
+        const float t2 = __builtin_fmaxf(ta, tb);
+        const float z1 = __builtin_fmaf(cz, t1, sz);
+        const float z2 = __builtin_fmaf(cz, t2, sz);
+        const bool v1 = (t1 > EPS_F) & (__builtin_fabsf(z1) <= 0.5f);
+        const bool v2 = (t2 > EPS_F) & (__builtin_fabsf(z2) <= 0.5f);
+        const float tm = -b * rA;
+        const float zm = __builtin_fmaf(cz, tm, sz);
+        flag = ((__builtin_fabsf(D) <= tau * (b * b)) & (tm > 0.0f) & (__builtin_fabsf(zm) <= 0.5f + tau)) |
+               (__builtin_fabsf(A) < tau * Axy) |
+               ((t1 > 0.0f) & (__builtin_fabsf(__builtin_fabsf(z1) - 0.5f) <= tau)) |
+               ((t2 > 0.0f) & (__builtin_fabsf(__builtin_fabsf(z2) - 0.5f) <= tau));
+        return v1 ? t1 : (v2 ? t2 : finf());
+    } else if constexpr (KIND == SEG_SPHERE) {
+        const float sx = o.x, sy = o.y, sz = o.z, cx = d.x, cy = d.y, cz = d.z;
+        const float A = __builtin_fmaf(cx, cx, __builtin_fmaf(cy, cy, cz * cz));
+        const float b = __builtin_fmaf(sx, cx, __builtin_fmaf(sy, cy, sz * cz));
+        const float C = __builtin_fmaf(sx, sx, __builtin_fmaf(sy, sy, __builtin_fmaf(sz, sz, -1.0f)));
+        const float D = __builtin_fmaf(b, b, -(A * C));
+        const float sq = sqrt_fast(D);
+        const float rA = rcp_fast(A);
+        const float t1 = (-b - sq) * rA;
+        const float t2 = (sq - b) * rA;
+        flag = (__builtin_fabsf(D) <= tau * (b * b)) & (b < 0.0f);
+        return (t1 > EPS_F) ? t1 : ((t2 > EPS_F) ? t2 : finf());
+    } else {
+        const float t = -o.z * rcp_fast(d.z);
+        const float hx = __builtin_fmaf(d.x, t, o.x);
+        const float hy = __builtin_fmaf(d.y, t, o.y);
+        flag = false;
+        bool v;
+        if constexpr (KIND == SEG_RING) {
+            const float rr = __builtin_fmaf(hx, hx, hy * hy);
+            v = (t > EPS_F) & (rr <= 1.0f) & (rr >= o.w);
+        } else {
+            v = (t > EPS_F) & (__builtin_fabsf(hx) <= 1.0f) & (__builtin_fabsf(hy) <= 1.0f);
+        }
+        return v ? t : finf();
     }
+}
+
+template <int KIND, bool REFINE, bool RESCAN>
+__device__ __forceinline__ void quad_loop(const float *__restrict__ Q, int qb, int qe, const float (&Bo)[4],
+                                          const float (&Bd)[4], float (&tb)[4], int (&ib)[4], Pending &pd,
+                                          int lane)
+{
+    if (qb >= qe) return;
+    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+    float a_next = Q[(size_t)qb * 64 + lane];
+    for (int q = qb; q < qe; ++q) {
+        const float a = a_next;
+        if (q + 1 < qe) a_next = Q[(size_t)(q + 1) * 64 + lane];
+        const int pos = 4 * q + (lane >> 4);
+        f32x4 Do[4], Dd[4];
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            Do[st] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bo[st], zero, 0, 0, 0);
+            Dd[st] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bd[st], zero, 0, 0, 0);
+        }
+        float t[4];
+        bool f[4];
+        bool any = false;
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            t[st] = quad_test<KIND>(Do[st], Dd[st], f[st], pd.tau);
+            if constexpr (RESCAN) {
+                const bool skip = q * 4 + st < pd.from;
+                f[st] = f[st] & !skip;
+                t[st] = skip ? finf() : t[st];
+            }
+            any |= f[st];
+        }
+        if constexpr (REFINE && KIND <= SEG_SPHERE) {
+            if (__builtin_amdgcn_ballot_w64(any)) {
+#pragma unroll
+                for (int st = 0; st < 4; ++st) {
+                    if (f[st]) {
+                        if (pd.count < PEND_K) {
+                            pd.buf[pd.count * blockDim.x + threadIdx.x] = ((uint32_t)pos << 2) | (uint32_t)st;
+                            ++pd.count;
+                        } else if (pd.resume == NO_KEY) {
+                            pd.resume = q * 4 + st;
+                        }
+                        t[st] = finf();
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int st = 0; st < 4; ++st)
+            if (t[st] < tb[st]) { tb[st] = t[st]; ib[st] = pos; }
+    }
+}
+
+// FP64 stage of the MFMA form: entries (pos << 2) | set; the pair's ray is
+// (set, lane % 16), regenerated from its index.
+__device__ __forceinline__ void resolve_pending_mfma(const KParams &kp, CPTR(GeomDesc) G, const rf_primitive &E,
+                                                     uint32_t gid, int emitter, int64_t wi0, int64_t base,
+                                                     float (&tb)[4], int (&ib)[4], Pending &pd, uint32_t &nref)
+{
+    CPTR(rf_primitive) P = cst(G->prims);
+    const int32_t *p2o = G->pos2orig;
+    const int lane = threadIdx.x & 63;
+    for (int e = 0; e < pd.count; ++e) {
+        const uint32_t ent = pd.buf[e * blockDim.x + threadIdx.x];
+        const int pos = (int)(ent >> 2);
+        const int st = (int)(ent & 3u);
+        const int orig = p2o[pos];
+        if (orig < 0) continue;          // padding slot (never flags; kept out regardless)
+        const int64_t rl = base + st * 16 + (lane & 15);
+        float f[4], r4[4];
+        workitem_floats(kp, gid, emitter, wi0 + (rl >> 2), f);
+        rotate_rands(f, (int)(rl & 3), r4);
+        const Ray64 r64 = widen(emit_fast(&E, r4));
+        const rf_primitive p = ldc(P + orig);
+        const float t = (float)hit_d_quadric(p.type, p.invm, p.util1, r64);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (st == i && (t < tb[i] || (t == tb[i] && pos < ib[i]))) { tb[i] = t; ib[i] = pos; }
+    }
+    nref += (uint32_t)pd.count;
+    pd.count = 0;
+}
+
+// The wave's 64 rays (base + lane) as the MFMA B operands: lane l supplies
+// component l / 16 of ray (set, l % 16) for origins ([s; 1]) and directions
+// ([c; 0]).
+__device__ __forceinline__ void mfma_rays(const KParams &kp, uint32_t gid, int emitter, int64_t wi0, int64_t rl,
+                                          const rf_primitive &E, int lane, float (&Bo)[4], float (&Bd)[4])
+{
+    float f[4], r4[4];
+    workitem_floats(kp, gid, emitter, wi0 + (rl >> 2), f);
+    rotate_rands(f, (int)(rl & 3), r4);
+    const Ray32 my = emit_fast(&E, r4);
+    const int j = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+        const int src = st * 16 + j;
+        const float vx = __shfl(my.sx, src), vy = __shfl(my.sy, src), vz = __shfl(my.sz, src);
+        const float wx = __shfl(my.cx, src), wy = __shfl(my.cy, src), wz = __shfl(my.cz, src);
+        Bo[st] = g == 0 ? vx : (g == 1 ? vy : (g == 2 ? vz : 1.0f));
+        Bd[st] = g == 0 ? wx : (g == 1 ? wy : (g == 2 ? wz : 0.0f));
+    }
+}
+
+template <bool REFINE, bool TRACE>
+__device__ __forceinline__ void run_unit_mfma(const KParams &kp, CPTR(GeomDesc) G, uint32_t gid, int emitter,
+                                              int64_t wi0, int64_t n_wi, uint32_t *bins, uint32_t *pend,
+                                              uint32_t &nref, uint32_t &nover)
+{
+    const rf_primitive E = ldc(cst(G->prims) + emitter);
+    const int64_t ray0 = (wi0 - kp.trace_wi0) * 4;
+    const float *Q = G->quads;
+    const int32_t *p2o = G->pos2orig;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nwaves = blockDim.x >> 6;
+    const int j = lane & 15, g = lane >> 4;
+    const int64_t n_rays = n_wi * 4;
+    for (int64_t base = (int64_t)wave * 64; base < n_rays; base += (int64_t)nwaves * 64) {
+        const int64_t rl = base + lane;
+        float Bo[4], Bd[4], tb[4];
+        int ib[4];
+        mfma_rays(kp, gid, emitter, wi0, rl, E, lane, Bo, Bd);
+#pragma unroll
+        for (int st = 0; st < 4; ++st) { tb[st] = finf(); ib[st] = -1; }
+        Pending pd{pend, 0, NO_KEY, 0, kp.tau};
+        quad_loop<SEG_CYL, REFINE, false>(Q, G->qseg_begin[SEG_CYL], G->qseg_end[SEG_CYL], Bo, Bd, tb, ib, pd, lane);
+        quad_loop<SEG_FRUSTUM, REFINE, false>(Q, G->qseg_begin[SEG_FRUSTUM], G->qseg_end[SEG_FRUSTUM], Bo, Bd, tb, ib,
+                                              pd, lane);
+        quad_loop<SEG_SPHERE, REFINE, false>(Q, G->qseg_begin[SEG_SPHERE], G->qseg_end[SEG_SPHERE], Bo, Bd, tb, ib, pd,
+                                             lane);
+        quad_loop<SEG_RING, REFINE, false>(Q, G->qseg_begin[SEG_RING], G->qseg_end[SEG_RING], Bo, Bd, tb, ib, pd, lane);
+        quad_loop<SEG_RECT, REFINE, false>(Q, G->qseg_begin[SEG_RECT], G->qseg_end[SEG_RECT], Bo, Bd, tb, ib, pd, lane);
+        if constexpr (REFINE) {
+            for (;;) {
+                if (__builtin_amdgcn_ballot_w64(pd.count > 0))
+                    resolve_pending_mfma(kp, G, E, gid, emitter, wi0, base, tb, ib, pd, nref);
+                if (!__builtin_amdgcn_ballot_w64(pd.resume != NO_KEY)) break;
+                // a lane's list filled up: rebuild the operands, re-scan the
+                // quadric quads from the first dropped pair
+                nover += pd.resume != NO_KEY;
+                pd.from = pd.resume;
+                pd.resume = NO_KEY;
+                float Bo2[4], Bd2[4];
+                mfma_rays(kp, gid, emitter, wi0, rl + opaque_zero(), E, lane, Bo2, Bd2);
+                const int lo = wave_min(pd.from) >> 2;
+                const auto beg = [&](int s) { return G->qseg_begin[s] > lo ? G->qseg_begin[s] : lo; };
+                quad_loop<SEG_CYL, true, true>(Q, beg(SEG_CYL), G->qseg_end[SEG_CYL], Bo2, Bd2, tb, ib, pd, lane);
+                quad_loop<SEG_FRUSTUM, true, true>(Q, beg(SEG_FRUSTUM), G->qseg_end[SEG_FRUSTUM], Bo2, Bd2, tb, ib,
+                                                   pd, lane);
+                quad_loop<SEG_SPHERE, true, true>(Q, beg(SEG_SPHERE), G->qseg_end[SEG_SPHERE], Bo2, Bd2, tb, ib, pd,
+                                                  lane);
+            }
+        }
+        // nearest over the 4 primitive slots of each ray (lanes j, j+16, j+32, j+48)
+        int mine = -1;
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            float t = tb[st];
+            int p = ib[st];
+#pragma unroll
+            for (int off = 16; off <= 32; off <<= 1) {
+                const float t2 = __shfl_xor(t, off);
+                const int p2 = __shfl_xor(p, off);
+                const bool take = (t2 < t) || (t2 == t && (unsigned)p2 < (unsigned)p);
+                t = take ? t2 : t;
+                p = take ? p2 : p;
+            }
+            if (g == st) mine = p;
+        }
+        if (rl < n_rays) {
+            const int idx = mine >= 0 ? p2o[mine] : -1;
+            tally<TRACE, false>(kp, G, emitter, ray0 + rl, idx, bins);
+        }
+    }
+}
+
+template <bool REFINE, bool TRACE>
+__global__ void __launch_bounds__(256) k_queue_mfma(KParams kp)
+{
+    extern __shared__ uint32_t smem[];
+    uint32_t *bins = smem;
+    uint32_t *pend = smem + (TRACE ? 0 : kp.bins);
+    __shared__ int s_unit;
+    uint32_t nref = 0, nover = 0;
+    const bool lds_tally = !TRACE && kp.bins > 0;
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = (int)atomicAdd(kp.queue_head, 1u);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        if (u >= kp.n_units) break;
+        CPTR(GeomDesc) G;
+        int emitter;
+        int64_t wi0, n_wi;
+        uint32_t gid;
+        if (kp.units) {
+            const WorkUnit U = ldc(cst(kp.units) + u);
+            G = cst(kp.geoms) + U.geom;
+            emitter = U.emitter; wi0 = U.wi0; n_wi = U.n_wi;
+            gid = G->geometry_id;
+        } else {
+            G = cst(kp.geoms);
+            emitter = kp.imp_emitter;
+            wi0 = kp.imp_wi0 + (int64_t)u * kp.imp_unit;
+            const int64_t left = kp.imp_wi0 + kp.imp_n_wi - wi0;
+            n_wi = left < kp.imp_unit ? left : kp.imp_unit;
+            gid = kp.imp_geom_id;
+        }
+        const int n = G->n;
+        if (lds_tally) {
+            for (int i = threadIdx.x; i < n; i += blockDim.x) bins[i] = 0u;
+            __syncthreads();
+        }
+        run_unit_mfma<REFINE, TRACE>(kp, G, gid, emitter, wi0, n_wi, bins, pend, nref, nover);
+        if (lds_tally) {
+            __syncthreads();
+            unsigned long long *row = G->hits + (int64_t)emitter * n;
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                const uint32_t c = bins[i];
+                if (c) atomicAdd(&row[i], (unsigned long long)c);
+            }
+        }
+        __syncthreads();
+    }
+    flush_stats(kp, 0, nref, nover);
 }
 
 // Persistent queue kernel: every workgroup pulls units until the queue is
 // empty (all waves reach the same exit).
+// Four rays per thread with the FP64 stage fits 96 VGPRs (five waves per
+// SIMD) when asked to; left alone it takes a few more and drops to four.
 template <int MATH, bool BV, bool TRACE, int NR, bool REFINE>
-__global__ void __launch_bounds__(256) k_queue(KParams kp)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 4 ? 5 : 1))) k_queue(KParams kp)
 {
     // dynamic LDS: [bins: kp.bins x u32][pending re-solves: PEND_K x blockDim x u32]
     extern __shared__ uint32_t smem[];
@@ -560,7 +964,7 @@ __global__ void __launch_bounds__(256) k_queue(KParams kp)
     uint32_t *pend = smem + (TRACE ? 0 : kp.bins);
     __shared__ int s_unit;
     uint64_t tests = 0;
-    uint32_t nref = 0;
+    uint32_t nref = 0, nover = 0;
     const bool lds_tally = !TRACE && kp.bins > 0;
     for (;;) {
         if (threadIdx.x == 0) s_unit = (int)atomicAdd(kp.queue_head, 1u);
@@ -590,7 +994,7 @@ __global__ void __launch_bounds__(256) k_queue(KParams kp)
             __syncthreads();
         }
         run_unit<MATH, BV, TRACE, NR, REFINE, false>(kp, G, gid, emitter, wi0, n_wi, threadIdx.x,
-                                                     blockDim.x, bins, pend, tests, nref);
+                                                     blockDim.x, bins, pend, tests, nref, nover);
         if (lds_tally) {
             __syncthreads();
             unsigned long long *row = G->hits + (int64_t)emitter * n;
@@ -601,7 +1005,7 @@ __global__ void __launch_bounds__(256) k_queue(KParams kp)
         }
         __syncthreads();
     }
-    flush_stats(kp, tests, nref);
+    flush_stats(kp, tests, nref, nover);
 }
 
 // Reference dispatch shape (RayFactorGPU.cl:481 launched as main.cpp:172-176):
@@ -613,11 +1017,11 @@ __global__ void __launch_bounds__(256) k_direct(KParams kp)
     extern __shared__ uint32_t smem[];   // pending re-solves: PEND_K x blockDim x u32
     CPTR(GeomDesc) G = cst(kp.geoms);
     uint64_t tests = 0;
-    uint32_t nref = 0;
+    uint32_t nref = 0, nover = 0;
     const int64_t wl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     run_unit<MATH, BV, false, 1, true, true>(kp, G, kp.imp_geom_id, kp.imp_emitter, kp.imp_wi0, kp.imp_n_wi,
-                                             wl, kp.imp_n_wi + 1, nullptr, smem, tests, nref);
-    flush_stats(kp, tests, nref);
+                                             wl, kp.imp_n_wi + 1, nullptr, smem, tests, nref, nover);
+    flush_stats(kp, tests, nref, nover);
 }
 
 __global__ void k_fill_u32(unsigned int *p, unsigned int v, size_t n)
@@ -677,6 +1081,18 @@ static hipError_t go_queue_math(const LaunchCfg &c, const KParams &kp, hipStream
 template <int MATH>
 static hipError_t go_math(const LaunchCfg &c, const KParams &kp, hipStream_t s)
 {
+    if constexpr (MATH == KM_FAST) {
+        if (c.mfma && !c.use_bv && !c.direct) {
+            if (c.trace) {
+                if (c.refine) hipLaunchKernelGGL((k_queue_mfma<true, true>), dim3(c.grid), dim3(c.block), c.lds_bytes, s, kp);
+                else hipLaunchKernelGGL((k_queue_mfma<false, true>), dim3(c.grid), dim3(c.block), c.lds_bytes, s, kp);
+            } else {
+                if (c.refine) hipLaunchKernelGGL((k_queue_mfma<true, false>), dim3(c.grid), dim3(c.block), c.lds_bytes, s, kp);
+                else hipLaunchKernelGGL((k_queue_mfma<false, false>), dim3(c.grid), dim3(c.block), c.lds_bytes, s, kp);
+            }
+            return hipGetLastError();
+        }
+    }
     if (c.direct) {
         if (c.use_bv) hipLaunchKernelGGL((k_direct<MATH, true>), dim3(c.grid), dim3(c.block), c.lds_bytes, s, kp);
         else hipLaunchKernelGGL((k_direct<MATH, false>), dim3(c.grid), dim3(c.block), c.lds_bytes, s, kp);

@@ -31,7 +31,7 @@ struct rf_ctx {
     char name[256] = {0};
     int n_cu = 0;
     int clock_mhz = 0;
-    int tuned_rpt = 0, tuned_block = 0;
+    int tuned_rpt = 0, tuned_block = 0, tuned_transform = 0;
     // scratch device buffers
     GeomDesc *d_geoms = nullptr; size_t geoms_cap = 0;
     WorkUnit *d_units = nullptr; size_t units_cap = 0;
@@ -56,6 +56,8 @@ struct rf_scene {
     rf_bounding_volume *d_bvs = nullptr;
     BvMeta *d_bv_meta = nullptr;
     unsigned long long *d_hits = nullptr;
+    float *d_quads = nullptr;            // MFMA form: primitive quads
+    int32_t *d_pos2orig = nullptr, *d_orig2pos = nullptr;
     GeomDesc *d_desc = nullptr;          // single-geometry descriptor
     GeomDesc desc{};                     // host copy
     int64_t flops_per_ray = 0;           // sum over primitives of F_type (plain kernels)
@@ -210,16 +212,52 @@ static int seg_of(int32_t type)
     }
 }
 
+// Output rows (4 x [x, y, z, 1] coefficients) of one primitive in the MFMA
+// form; see the kernel's quad layout.  pad: a slot that never hits.
+static void quad_rows(const rf_primitive *p, float rows[4][4])
+{
+    std::memset(rows, 0, sizeof(float) * 16);
+    if (!p) {
+        rows[0][3] = 1e3f;    // x = 1000, zero direction: quadrics see A = 0 (no root)
+        rows[2][3] = 1e3f;    // z = 1000, zero direction: planes see t = -inf
+        return;
+    }
+    const float *iv = p->invm;
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 4; ++k) rows[r][k] = iv[4 * r + k];
+    switch (p->type) {
+    case RF_CYLINDER:
+        rows[2][3] = iv[11] - 0.5f;
+        break;
+    case RF_FRUSTUM: {
+        const float k = p->util1 - 1.0f;
+        for (int c = 0; c < 4; ++c) { rows[3][c] = iv[8 + c]; rows[2][c] = k * iv[8 + c]; }
+        rows[2][3] += 1.0f;                         // w = 1 + (u1-1) z
+        rows[3][3] = iv[11] - 0.5f;                 // z - 1/2
+        break;
+    }
+    case RF_ANNULUS: {
+        const float rs = 1.0f / std::sqrt(p->util2);  // outer radius -> 1
+        for (int c = 0; c < 4; ++c) { rows[0][c] *= rs; rows[1][c] *= rs; }
+        rows[3][3] = p->util1 / p->util2;           // inner radius^2, normalised
+        break;
+    }
+    default:
+        break;                                      // disc (r^2 in [0,1]), sphere, rectangle
+    }
+}
+
 extern "C" int64_t rf_scene_footprint(int32_t n, int32_t nbv)
 {
     if (n < 0 || nbv < 0) return -1;
-    return (int64_t)n * (sizeof(rf_primitive) + 2 * sizeof(FastPrim)) +
+    return (int64_t)n * (sizeof(rf_primitive) + 2 * sizeof(FastPrim) + 64 + 8) + 5 * 256 +
            (int64_t)nbv * (sizeof(rf_bounding_volume) + sizeof(FastPrim) + sizeof(BvMeta)) +
            (int64_t)n * n * 8 + (int64_t)sizeof(GeomDesc);
 }
 
 static void free_scene_mem(rf_scene *s)
 {
+    (void)hipFree(s->d_quads); (void)hipFree(s->d_pos2orig); (void)hipFree(s->d_orig2pos);
     (void)hipFree(s->d_prims); (void)hipFree(s->d_fast); (void)hipFree(s->d_fast_orig);
     (void)hipFree(s->d_fast_bvs); (void)hipFree(s->d_bvs); (void)hipFree(s->d_bv_meta);
     (void)hipFree(s->d_hits); (void)hipFree(s->d_desc);
@@ -279,6 +317,31 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
         meta[i] = BvMeta{bvs[i].settings, bvs[i].objStartIndex, bvs[i].objEndIndex, bvs[i].type};
         fbv += flops_per_test(bvs[i].type);
     }
+    // MFMA form: quads of 4 primitives per type segment, padded
+    std::vector<float> quads;
+    std::vector<int32_t> pos2orig, orig2pos(n, -1);
+    int32_t qseg_begin[N_SEG], qseg_end[N_SEG];
+    for (int k = 0; k < N_SEG; ++k) {
+        qseg_begin[k] = (int32_t)(pos2orig.size() / 4);
+        std::vector<int32_t> members;
+        for (int32_t i = 0; i < n; ++i)
+            if (seg_of(prims[i].type) == k) members.push_back(i);
+        while (members.size() % 4) members.push_back(-1);
+        for (size_t q = 0; q < members.size(); q += 4) {
+            float a[64];
+            for (int slot = 0; slot < 4; ++slot) {
+                const int32_t i = members[q + slot];
+                float rows[4][4];
+                quad_rows(i >= 0 ? &prims[i] : nullptr, rows);
+                for (int r = 0; r < 4; ++r)
+                    for (int c = 0; c < 4; ++c) a[16 * c + 4 * slot + r] = rows[r][c];
+                if (i >= 0) orig2pos[i] = (int32_t)pos2orig.size();
+                pos2orig.push_back(i);
+            }
+            quads.insert(quads.end(), a, a + 64);
+        }
+        qseg_end[k] = (int32_t)(pos2orig.size() / 4);
+    }
     s->flops_per_ray = fsum;
     s->flops_per_test_avg = (double)(fsum + fbv) / (double)(n + nbv);
 
@@ -296,6 +359,9 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
         up((void **)&s->d_fast_bvs, fast_bvs.data(), sizeof(FastPrim) * nbv);
         up((void **)&s->d_bv_meta, meta.data(), sizeof(BvMeta) * nbv);
     }
+    up((void **)&s->d_quads, quads.data(), quads.size() * sizeof(float));
+    up((void **)&s->d_pos2orig, pos2orig.data(), pos2orig.size() * sizeof(int32_t));
+    up((void **)&s->d_orig2pos, orig2pos.data(), orig2pos.size() * sizeof(int32_t));
     if (e == hipSuccess) e = hipMalloc((void **)&s->d_hits, (size_t)n * n * sizeof(unsigned long long));
     if (e == hipSuccess) e = launch_fill_u64(s->d_hits, (size_t)n * n, ctx->stream);
     GeomDesc &d = s->desc;
@@ -303,6 +369,8 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     d.bvs = s->d_bvs; d.fast_bvs = s->d_fast_bvs; d.bv_meta = s->d_bv_meta;
     d.hits = s->d_hits; d.n = n; d.n_bvs = nbv; d.geometry_id = 0;
     for (int k = 0; k < N_SEG; ++k) { d.seg_begin[k] = seg_begin[k]; d.seg_end[k] = seg_end[k]; }
+    d.quads = s->d_quads; d.pos2orig = s->d_pos2orig; d.orig2pos = s->d_orig2pos;
+    for (int k = 0; k < N_SEG; ++k) { d.qseg_begin[k] = qseg_begin[k]; d.qseg_end[k] = qseg_end[k]; }
     up((void **)&s->d_desc, &d, sizeof d);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
@@ -362,6 +430,10 @@ static int check_opts(rf_ctx *ctx, const rf_solve_opts *o)
     if (o->rays_per_thread && o->rays_per_thread != 1 && o->rays_per_thread != 2 && o->rays_per_thread != 4 &&
         o->rays_per_thread != 8)
         return set_err(ctx, RF_ERR_INVALID_ARG, "rays_per_thread must be 0, 1, 2, 4 or 8");
+    if (o->transform < RF_TRANSFORM_AUTO || o->transform > RF_TRANSFORM_MFMA)
+        return set_err(ctx, RF_ERR_INVALID_ARG, "bad transform");
+    if (!(o->refine_margin >= 0.0f && o->refine_margin <= 0.25f))
+        return set_err(ctx, RF_ERR_INVALID_ARG, "refine_margin must lie in [0, 0.25]");
     if (o->block_size && (o->block_size < 64 || o->block_size > 256 || o->block_size % 64))
         return set_err(ctx, RF_ERR_INVALID_ARG, "block_size must be 0 or a multiple of 64 in [64, 256]");
     return RF_OK;
@@ -395,13 +467,17 @@ static int emitter_budget(rf_ctx *ctx, const rf_scene *s, const rf_solve_opts *o
 static constexpr int kPendK = 4;            // deferred FP64 re-solves per lane (PEND_K in the kernels)
 static constexpr int kMaxLdsBins = 15000;   // 32-bit bins per workgroup (< 64 KiB with the re-solve list)
 
-static void pick_shape(const rf_ctx *ctx, const rf_solve_opts *o, int &nr, int &block)
+static void pick_shape(const rf_ctx *ctx, const rf_solve_opts *o, int &nr, int &block, int &mfma)
 {
     nr = 1;
     block = 256;
+    mfma = 0;
     if (o->math == RF_MATH_FAST && !o->use_bvs) {
+        const int tr = o->transform ? o->transform : (ctx->tuned_transform ? ctx->tuned_transform : RF_TRANSFORM_VALU);
+        mfma = tr == RF_TRANSFORM_MFMA;
         nr = o->rays_per_thread ? o->rays_per_thread : (ctx->tuned_rpt ? ctx->tuned_rpt : 4);
         block = o->block_size ? o->block_size : (ctx->tuned_block ? ctx->tuned_block : 256);
+        if (mfma) nr = 4;   // 64 rays per wave and pass: one work item per lane-quarter
     } else if (o->block_size) {
         block = o->block_size;
     }
@@ -443,8 +519,8 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
                         int64_t *launches)
 {
     if (n_wi <= 0) return RF_OK;
-    int nr, block;
-    pick_shape(ctx, o, nr, block);
+    int nr, block, mfma;
+    pick_shape(ctx, o, nr, block, mfma);
     KParams kp{};
     kp.geoms = s->d_desc;
     kp.units = nullptr;
@@ -454,6 +530,7 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
     kp.rng = o->rng;
     kp.seed = o->seed;
     kp.refine = o->refine;
+    kp.tau = o->refine_margin > 0.0f ? o->refine_margin : rfd::TAU_DEFAULT;
     kp.bins = s->n <= kMaxLdsBins ? s->n : 0;
     kp.imp_emitter = emitter;
     kp.imp_geom_id = geom_id;
@@ -465,6 +542,7 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
     c.trace = trace_out != nullptr;
     c.nr = nr;
     c.refine = o->refine;
+    c.mfma = mfma;
     if (o->form == RF_FORM_REFERENCE && !trace_out) {
         // main.cpp:172-176: global = nthreads, local = 128
         c.direct = 1;
@@ -551,9 +629,9 @@ static double now_ms()
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-static int read_stats(rf_ctx *ctx, unsigned long long st[2])
+static int read_stats(rf_ctx *ctx, unsigned long long st[3])
 {
-    HIPCHK(ctx, hipMemcpy(st, ctx->d_stats, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(st, ctx->d_stats, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     return RF_OK;
 }
 
@@ -575,8 +653,8 @@ static int solve_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, c
                             const std::vector<std::vector<int64_t>> &budgets, const std::vector<uint32_t> &ids,
                             int64_t *launches)
 {
-    int nr, block;
-    pick_shape(ctx, o, nr, block);
+    int nr, block, mfma;
+    pick_shape(ctx, o, nr, block, mfma);
     const bool use_bv = o->use_bvs != 0;
     // geometry descriptors
     std::vector<GeomDesc> descs(n_geoms);
@@ -627,6 +705,7 @@ static int solve_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, c
     kp.rng = o->rng;
     kp.seed = o->seed;
     kp.refine = o->refine;
+    kp.tau = o->refine_margin > 0.0f ? o->refine_margin : rfd::TAU_DEFAULT;
     kp.bins = max_n <= kMaxLdsBins ? max_n : 0;
     int rc = next_counter(ctx, &kp.queue_head);
     if (rc) return rc;
@@ -635,6 +714,7 @@ static int solve_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, c
     c.use_bv = use_bv;
     c.nr = nr;
     c.refine = o->refine;
+    c.mfma = mfma;
     c.block = block;
     c.grid = (int)std::min<int64_t>((int64_t)units.size(), groups);
     c.lds_bytes = (kp.bins ? (size_t)kp.bins * sizeof(uint32_t) : 0) + (size_t)kPendK * block * sizeof(uint32_t);
@@ -664,7 +744,7 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
     // hit matrices and statistics start at zero
     for (int g = 0; g < n_geoms; ++g)
         HIPCHK(ctx, launch_fill_u64(scenes[g]->d_hits, (size_t)scenes[g]->n * scenes[g]->n, ctx->stream));
-    HIPCHK(ctx, launch_fill_u64(ctx->d_stats, 2, ctx->stream));
+    HIPCHK(ctx, launch_fill_u64(ctx->d_stats, 3, ctx->stream));
     rc = ensure_queue(ctx, 4096);
     if (rc) return rc;
     HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
@@ -681,13 +761,14 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
     HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
     float ms = 0.f;
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    unsigned long long st[2];
+    unsigned long long st[3];
     rc = read_stats(ctx, st);
     if (rc) return rc;
     rf_timing t{};
     t.kernel_ms = ms;
     t.launches = launches;
     t.refined = (int64_t)st[1];
+    t.unresolved = (int64_t)st[2];
     for (int g = 0; g < n_geoms; ++g) {
         account(scenes[g], o, budgets[g], t);
         if (o->use_bvs && scenes[g]->n_bvs > 0) {
@@ -808,22 +889,27 @@ extern "C" int32_t rf_autotune(rf_ctx *ctx, rf_scene *const *scenes, int32_t n_g
     rays = std::max<int64_t>(rays, 4096);
     if (o->budget == RF_BUDGET_EXPLICIT && o->rays_per_emitter > 0) rays = std::min(rays, o->rays_per_emitter);
     t.rays_per_emitter = rays;
-    const int rpts[] = {2, 4, 8};
+    // candidates: VALU transform with 2/4/8 rays per thread, MFMA transform
+    const int cands[][2] = {{RF_TRANSFORM_VALU, 2}, {RF_TRANSFORM_VALU, 4}, {RF_TRANSFORM_VALU, 8},
+                            {RF_TRANSFORM_MFMA, 4}};
     const int blocks[] = {128, 256};
     double best = 1e300;
-    int br = 4, bb = 256;
-    for (int rp : rpts)
+    int br = 4, bb = 256, bt = RF_TRANSFORM_VALU;
+    for (const auto &cd : cands)
         for (int bl : blocks) {
-            t.rays_per_thread = rp;
+            t.transform = cd[0];
+            t.rays_per_thread = cd[1];
             t.block_size = bl;
             rf_timing tm{};
             rc = solve_impl(ctx, scenes, n_geoms, &t, nullptr, nullptr, &tm);   // warm
             if (rc) return rc;
             rc = solve_impl(ctx, scenes, n_geoms, &t, nullptr, nullptr, &tm);
             if (rc) return rc;
-            if (tm.kernel_ms < best) { best = tm.kernel_ms; br = rp; bb = bl; }
+            if (tm.kernel_ms < best) { best = tm.kernel_ms; br = cd[1]; bb = bl; bt = cd[0]; }
         }
-    ctx->tuned_rpt = br;
+    ctx->tuned_transform = bt;
+    if (bt == RF_TRANSFORM_MFMA) br = 0;
+    ctx->tuned_rpt = br ? br : 4;
     ctx->tuned_block = bb;
     if (best_rpt) *best_rpt = br;
     if (best_block) *best_block = bb;

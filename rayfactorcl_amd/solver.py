@@ -17,13 +17,14 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib as L
-from ._lib import BV_DTYPE, FORM, MATH, PRIM_DTYPE, RNG, SolveOpts, Timing, check, lib, ptr
+from ._lib import BV_DTYPE, FORM, MATH, PRIM_DTYPE, RNG, TRANSFORM, SolveOpts, Timing, check, lib, ptr
 
 
 def make_opts(math: str = "compat", rng: str = "reference", use_bvs: bool = False,
               n_samples: int | None = None, rays_per_emitter: int | None = None,
               form: str = "per_emitter", refine: bool = True, seed: int = 0, geometry: int = 0,
-              rays_per_thread: int = 0, block_size: int = 0) -> SolveOpts:
+              rays_per_thread: int = 0, block_size: int = 0, transform: str = "auto",
+              refine_margin: float = 0.0) -> SolveOpts:
     """Solve options.  The ray budget is the reference's launch geometry
     (``n_samples``, main.cpp:172-173) unless ``rays_per_emitter`` is given."""
     o = SolveOpts()
@@ -37,6 +38,8 @@ def make_opts(math: str = "compat", rng: str = "reference", use_bvs: bool = Fals
     o.geometry = geometry & 0xFFFFFFFF
     o.rays_per_thread = rays_per_thread
     o.block_size = block_size
+    o.transform = TRANSFORM[transform]
+    o.refine_margin = float(refine_margin)
     if rays_per_emitter is not None:
         o.budget = 1
         o.rays_per_emitter = int(rays_per_emitter)
@@ -182,7 +185,8 @@ def trace(ctx: Context, scene: DeviceScene, emitter: int, wi0: int, n_wi: int, o
 
 
 def autotune(ctx: Context, scenes: list[DeviceScene], opts: SolveOpts | None = None, **kw) -> tuple[int, int]:
-    """Fastest (rays per thread, block size) for the fast kernel on these scenes."""
+    """Fastest (rays per thread, block size) for the fast kernel on these scenes;
+    rays per thread 0 means the MFMA-transform variant won."""
     o = opts if opts is not None else make_opts(**kw)
     sc = (c_void_p * len(scenes))(*[s.handle.value for s in scenes])
     r, b = c_int32(), c_int32()

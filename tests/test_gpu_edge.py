@@ -51,16 +51,26 @@ def test_bad_options_are_rejected(ctx, golden_scene):
     with pytest.raises(rf.RayFactorError) as e:
         rf.solve(ctx, dev, math="compat", n_samples=2 ** 30)                    # rayDensity*n_samples > int32
     assert e.value.code == _lib.RF_ERR_OVERFLOW
+    for bad in (dict(refine_margin=0.5), dict(refine_margin=-1e-3), dict(refine_margin=float("nan"))):
+        with pytest.raises(rf.RayFactorError) as e:
+            rf.solve(ctx, dev, math="fast", rays_per_emitter=4096, **bad)
+        assert e.value.code == _lib.RF_ERR_INVALID_ARG
+    o = rf.make_opts(math="fast", rays_per_emitter=4096)
+    o.transform = 7
+    with pytest.raises(rf.RayFactorError) as e:
+        rf.solve(ctx, dev, o)
+    assert e.value.code == _lib.RF_ERR_INVALID_ARG
 
 
-def test_large_scene_global_atomic_tally(ctx):
+@pytest.mark.parametrize("transform", ["valu", "mfma"])
+def test_large_scene_global_atomic_tally(ctx, transform):
     # more primitives than LDS bins: per-ray global atomics; rows agree with traces
     sc = scenes.parse_xml(scenes.furnace_xml(wall_bands=16500, preform_bands=10))
     assert sc.n > 16000
     dev = rf.DeviceScene(ctx, sc.prims, None)
-    r = rf.solve(ctx, dev, math="fast", rays_per_emitter=256, rng="seeded", form="persistent")
+    r = rf.solve(ctx, dev, math="fast", rays_per_emitter=256, rng="seeded", form="persistent", transform=transform)
     for e in [0, 8000, sc.n - 1]:
-        tr = rf.trace(ctx, dev, e, 0, 64, math="fast", rng="seeded")
+        tr = rf.trace(ctx, dev, e, 0, 64, math="fast", rng="seeded", transform=transform)
         row = np.bincount(tr[tr >= 0], minlength=sc.n)
         assert np.array_equal(row, r.hits[e])
 

@@ -87,3 +87,105 @@ def test_fp64_bv_matches_oracle_furnace(ctx, oracle):
     c = rf.solve(ctx, dev, math="compat", use_bvs=True, rays_per_emitter=4 * n_wi)
     d, _ = oracle.solve(sc.prims, sc.bvs, n_wi=n_wi, mode="compat", use_bvs=True)
     assert np.array_equal(c.hits, d)
+
+
+# ---------------------------------------------------------------------------
+# Ray transform on the matrix cores (v_mfma_f32_16x16x4f32)
+
+@pytest.mark.parametrize("refine", [True, False])
+def test_mfma_vs_gpu_fp64_furnace(ctx, refine):
+    sc = scenes.parse_xml(scenes.furnace_xml())
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    rays = 1 << 18
+    a = rf.solve(ctx, dev, math="fast", refine=refine, rays_per_emitter=rays, rng="seeded", form="persistent",
+                 transform="mfma")
+    b = rf.solve(ctx, dev, math="fp64", rays_per_emitter=rays, rng="seeded", form="persistent")
+    dvf = np.abs(a.hits - b.hits).max() / rays
+    print(f"[mfma] furnace refine={refine} max |dVF| = {dvf:.2e}, refined pairs {a.timing['refined']}", flush=True)
+    assert dvf <= VF_TOL
+    assert np.array_equal(a.rays, b.rays)
+    if refine:
+        assert a.timing["refined"] > 0
+
+
+def test_mfma_soup_and_config1_precision(ctx, golden_scene):
+    for sc in (scenes.parse_xml(scenes.soup_xml(300, seed=9)), golden_scene("config1")):
+        dev = rf.DeviceScene(ctx, sc.prims, None)
+        rays = 1 << 16
+        a = rf.solve(ctx, dev, math="fast", rays_per_emitter=rays, rng="seeded", form="persistent",
+                     transform="mfma")
+        b = rf.solve(ctx, dev, math="fp64", rays_per_emitter=rays, rng="seeded", form="persistent")
+        dvf = np.abs(a.hits - b.hits).max() / rays
+        print(f"[mfma] {sc.n} primitives max |dVF| = {dvf:.2e}", flush=True)
+        assert dvf <= VF_TOL
+
+
+@pytest.mark.parametrize("block", [64, 128, 256])
+def test_mfma_forms_identical(ctx, block):
+    """Persistent queue and per-emitter launches of the MFMA kernel hand the
+    same rays to the same arithmetic: identical tallies for any group size."""
+    sc = scenes.parse_xml(scenes.furnace_xml(wall_bands=40, preform_bands=12))
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    kw = dict(math="fast", rays_per_emitter=20000, rng="seeded", transform="mfma")
+    base = rf.solve(ctx, dev, form="persistent", block_size=256, **kw)
+    got = rf.solve(ctx, dev, form="per_emitter", block_size=block, **kw)
+    assert np.array_equal(got.hits, base.hits)
+
+
+def test_mfma_trace_matches_tally(ctx):
+    sc = scenes.parse_xml(scenes.furnace_xml(wall_bands=20, preform_bands=6))
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    n_wi = 3000                      # not a multiple of the 16-ray MFMA set: ragged last pass
+    res = rf.solve(ctx, dev, math="fast", rays_per_emitter=4 * n_wi, rng="seeded", form="per_emitter",
+                   transform="mfma")
+    for e in (0, sc.n // 2, sc.n - 1):
+        idx = rf.trace(ctx, dev, e, 0, n_wi, math="fast", rng="seeded", transform="mfma")
+        assert idx.shape == (4 * n_wi,)
+        assert ((idx >= -1) & (idx < sc.n)).all()
+        assert np.array_equal(np.bincount(idx[idx >= 0], minlength=sc.n), res.hits[e])
+
+
+# ---------------------------------------------------------------------------
+# Re-solve lists that fill up: a wide flag margin flags most rim crossings, so
+# lanes drop pairs and re-scan; results must not depend on the launch shape.
+
+WIDE = 0.25
+
+
+@pytest.mark.parametrize("form", ["per_emitter", "persistent", "reference"])
+@pytest.mark.parametrize("rpt", [1, 2, 4, 8])
+def test_wide_margin_forms_and_shapes_identical(ctx, form, rpt):
+    sc = scenes.parse_xml(scenes.furnace_xml(wall_bands=40, preform_bands=12))
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    kw = dict(math="fast", rays_per_emitter=20000, rng="seeded", refine_margin=WIDE, transform="valu")
+    base = rf.solve(ctx, dev, form="persistent", rays_per_thread=4, **kw)
+    got = rf.solve(ctx, dev, form=form, rays_per_thread=rpt, **kw)
+    assert np.array_equal(got.hits, base.hits)
+    # every flagged pair is re-solved exactly once, whatever the shape
+    assert got.timing["refined"] == base.timing["refined"]
+    if rpt == 4 and form == "persistent":
+        # lists overflowed and the re-scan path ran
+        assert base.timing["unresolved"] > 0
+        print(f"[wide] refined {base.timing['refined']}, re-scans {base.timing['unresolved']}", flush=True)
+
+
+@pytest.mark.parametrize("transform", ["valu", "mfma"])
+def test_wide_margin_precision(ctx, transform):
+    """Re-solving far more pairs than needed only moves the result closer to
+    FP64; the MFMA kernel's re-scan is checked against its per-emitter form."""
+    sc = scenes.parse_xml(scenes.furnace_xml())
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    rays = 1 << 16
+    kw = dict(math="fast", rays_per_emitter=rays, rng="seeded", transform=transform)
+    a = rf.solve(ctx, dev, form="persistent", refine_margin=WIDE, **kw)
+    d = rf.solve(ctx, dev, form="persistent", **kw)
+    b = rf.solve(ctx, dev, math="fp64", rays_per_emitter=rays, rng="seeded", form="persistent")
+    dvf = np.abs(a.hits - b.hits).max() / rays
+    print(f"[wide] {transform} max |dVF| = {dvf:.2e} (default margin "
+          f"{np.abs(d.hits - b.hits).max() / rays:.2e}), refined {a.timing['refined']} vs {d.timing['refined']}, "
+          f"re-scans {a.timing['unresolved']}", flush=True)
+    assert dvf <= VF_TOL
+    assert a.timing["refined"] > 10 * d.timing["refined"]
+    assert a.timing["unresolved"] > 0
+    c = rf.solve(ctx, dev, form="per_emitter", refine_margin=WIDE, **kw)
+    assert np.array_equal(a.hits, c.hits)

@@ -44,11 +44,18 @@ def test_stream_admission(ctx):
 
 
 def test_autotune_keeps_results(ctx):
+    """Tuning picks a launch shape and a transform; the VALU kernel's tallies
+    do not depend on the shape, and AUTO afterwards equals the chosen
+    transform pinned."""
     geoms = scenes.sweep(3, seed=4)
     dev = [rf.DeviceScene(ctx, g.prims, None) for g in geoms]
-    before, _ = rf.batch_solve(ctx, dev, math="fast", rng="seeded", rays_per_emitter=2000, form="persistent")
+    kw = dict(math="fast", rng="seeded", rays_per_emitter=2000, form="persistent")
+    before = {tr: rf.batch_solve(ctx, dev, transform=tr, **kw)[0] for tr in ("valu", "mfma")}
     rpt, blk = rf.autotune(ctx, dev, math="fast", rng="seeded", rays_per_emitter=2000)
-    assert rpt in (2, 4, 8) and blk in (128, 256)
-    after, _ = rf.batch_solve(ctx, dev, math="fast", rng="seeded", rays_per_emitter=2000, form="persistent")
-    for a, b in zip(before, after):
+    assert rpt in (0, 2, 4, 8) and blk in (128, 256)
+    after_valu, _ = rf.batch_solve(ctx, dev, transform="valu", **kw)
+    after_auto, _ = rf.batch_solve(ctx, dev, **kw)
+    want = before["mfma" if rpt == 0 else "valu"]
+    for a, b, c, w in zip(before["valu"], after_valu, after_auto, want):
         assert np.array_equal(a.hits, b.hits)
+        assert np.array_equal(c.hits, w.hits)
