@@ -138,34 +138,60 @@ __device__ __forceinline__ int find_bv_cd(CPTR(rf_primitive) P, CPTR(rf_bounding
 // fast: per-type tests over FastPrim records.  Each returns t (+INF on a
 // miss) and sets flag when the FP32 roots are ill-conditioned.
 // ---------------------------------------------------------------------------
+// A VALU instruction with an SGPR data operand does not dual-issue on gfx950
+// (measured: 15 vs 28 lane-FMAs per clock per SIMD).  Wave-uniform operands of
+// the hot loops (primitive rows, thresholds) are therefore copied into VGPRs
+// once per primitive / per kernel and the per-ray arithmetic reads them there.
+__device__ __forceinline__ float vcopy(float x)
+{
+    float r;
+    asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+}
+
+__device__ __forceinline__ FastPrim vcopy(const FastPrim &q)
+{
+    FastPrim v;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) v.r[i] = vcopy(q.r[i]);
+    v.k0 = vcopy(q.k0);
+    v.k1 = vcopy(q.k1);
+    v.orig = q.orig;
+    v.type = q.type;
+    return v;
+}
+
 #define ROW_S(P, i, R) __builtin_fmaf(P.r[4 * i], R.sx, __builtin_fmaf(P.r[4 * i + 1], R.sy, __builtin_fmaf(P.r[4 * i + 2], R.sz, P.r[4 * i + 3])))
 #define ROW_C(P, i, R) __builtin_fmaf(P.r[4 * i], R.cx, __builtin_fmaf(P.r[4 * i + 1], R.cy, P.r[4 * i + 2] * R.cz))
 
 template <int KIND>
-__device__ __forceinline__ float fast_test(const FastPrim &P, const Ray32 &R, bool &flag, float tau)
+__device__ __forceinline__ float fast_test(const FastPrim &P, const Ray32 &R, bool &flag, float tau, float eps)
 {
     if constexpr (KIND == SEG_CYL) {
         const float sx = ROW_S(P, 0, R), sy = ROW_S(P, 1, R), sz = ROW_S(P, 2, R);
         const float cx = ROW_C(P, 0, R), cy = ROW_C(P, 1, R), cz = ROW_C(P, 2, R);
+        // b^2 - A C = A - (s x c)_z^2 (Lagrange): no C term, and the
+        // rounding error scales with A rather than b^2
         const float A = __builtin_fmaf(cx, cx, cy * cy);
         const float b = __builtin_fmaf(sx, cx, sy * cy);
-        const float C = __builtin_fmaf(sx, sx, __builtin_fmaf(sy, sy, -1.0f));
-        const float D = __builtin_fmaf(b, b, -(A * C));
+        const float w = __builtin_fmaf(sx, cy, -(sy * cx));
+        const float D = __builtin_fmaf(-w, w, A);
         const float sq = sqrt_fast(D);
         const float rA = rcp_fast(A);
-        const float t1 = (-b - sq) * rA;
-        const float t2 = (sq - b) * rA;
+        const float tm = -b * rA;                       // tangent point
+        const float t1 = __builtin_fmaf(-sq, rA, tm);
+        const float t2 = __builtin_fmaf(sq, rA, tm);
         const float z1 = __builtin_fmaf(cz, t1, sz);
         const float z2 = __builtin_fmaf(cz, t2, sz);
-        const bool v1 = (t1 > EPS_F) & (__builtin_fabsf(z1) <= 0.5f);
-        const bool v2 = (t2 > EPS_F) & (__builtin_fabsf(z2) <= 0.5f);
-        // near-tangent only matters where the tangent point lies ahead of the
-        // ray and on this band; a rim only for a root ahead of the ray
-        const float tm = -b * rA;
+        const bool e1 = t1 > eps, e2 = t2 > eps;
+        const bool v1 = e1 & (__builtin_fabsf(z1) <= 0.5f);
+        const bool v2 = e2 & (__builtin_fabsf(z2) <= 0.5f);
+        // near-tangent only where the tangent point lies on this band; a rim
+        // only for a root that could be accepted
         const float zm = __builtin_fmaf(cz, tm, sz);
-        flag = ((__builtin_fabsf(D) <= tau * (b * b)) & (tm > 0.0f) & (__builtin_fabsf(zm) <= 0.5f + tau)) |
-               ((t1 > 0.0f) & (__builtin_fabsf(__builtin_fabsf(z1) - 0.5f) <= tau)) |
-               ((t2 > 0.0f) & (__builtin_fabsf(__builtin_fabsf(z2) - 0.5f) <= tau));
+        flag = ((__builtin_fabsf(D) <= tau * A) & (__builtin_fabsf(zm) <= 0.5f + tau)) |
+               (e1 & (__builtin_fabsf(__builtin_fabsf(z1) - 0.5f) <= tau)) |
+               (e2 & (__builtin_fabsf(__builtin_fabsf(z2) - 0.5f) <= tau));
         return v1 ? t1 : (v2 ? t2 : finf());
     } else if constexpr (KIND == SEG_FRUSTUM) {
         const float sx = ROW_S(P, 0, R), sy = ROW_S(P, 1, R), sw = ROW_S(P, 2, R);
@@ -177,21 +203,22 @@ __device__ __forceinline__ float fast_test(const FastPrim &P, const Ray32 &R, bo
         const float D = __builtin_fmaf(b, b, -(A * C));
         const float sq = sqrt_fast(D);
         const float rA = rcp_fast(A);
-        const float ta = (-b - sq) * rA;
-        const float tb = (sq - b) * rA;
+        const float tm = -b * rA;
+        const float ta = __builtin_fmaf(-sq, rA, tm);
+        const float tb = __builtin_fmaf(sq, rA, tm);
         const float t1 = __builtin_fminf(ta, tb);
         const float t2 = __builtin_fmaxf(ta, tb);
         const float w1 = __builtin_fmaf(cw, t1, sw) - P.k0;
         const float w2 = __builtin_fmaf(cw, t2, sw) - P.k0;
-        const bool v1 = (t1 > EPS_F) & (__builtin_fabsf(w1) <= P.k1);
-        const bool v2 = (t2 > EPS_F) & (__builtin_fabsf(w2) <= P.k1);
+        const bool e1 = t1 > eps, e2 = t2 > eps;
+        const bool v1 = e1 & (__builtin_fabsf(w1) <= P.k1);
+        const bool v2 = e2 & (__builtin_fabsf(w2) <= P.k1);
         const float rim = (2.0f * tau) * P.k1;
-        const float tm = -b * rA;
         const float wm = __builtin_fmaf(cw, tm, sw) - P.k0;
-        flag = ((__builtin_fabsf(D) <= tau * (b * b)) & (tm > 0.0f) & (__builtin_fabsf(wm) <= P.k1 + rim)) |
+        flag = ((__builtin_fabsf(D) <= tau * (b * b)) & (__builtin_fabsf(wm) <= P.k1 + rim)) |
                (__builtin_fabsf(A) < tau * Axy) |
-               ((t1 > 0.0f) & (__builtin_fabsf(__builtin_fabsf(w1) - P.k1) <= rim)) |
-               ((t2 > 0.0f) & (__builtin_fabsf(__builtin_fabsf(w2) - P.k1) <= rim));
+               (e1 & (__builtin_fabsf(__builtin_fabsf(w1) - P.k1) <= rim)) |
+               (e2 & (__builtin_fabsf(__builtin_fabsf(w2) - P.k1) <= rim));
         return v1 ? t1 : (v2 ? t2 : finf());
     } else if constexpr (KIND == SEG_SPHERE) {
         const float sx = ROW_S(P, 0, R), sy = ROW_S(P, 1, R), sz = ROW_S(P, 2, R);
@@ -205,7 +232,7 @@ __device__ __forceinline__ float fast_test(const FastPrim &P, const Ray32 &R, bo
         const float t1 = (-b - sq) * rA;
         const float t2 = (sq - b) * rA;
         flag = (__builtin_fabsf(D) <= tau * (b * b)) & (b < 0.0f);
-        return (t1 > EPS_F) ? t1 : ((t2 > EPS_F) ? t2 : finf());
+        return (t1 > eps) ? t1 : ((t2 > eps) ? t2 : finf());
     } else {
         // planes: object z = 0; hit point found in world space, then mapped
         const float sz = ROW_S(P, 2, R), cz = ROW_C(P, 2, R);
@@ -219,22 +246,22 @@ __device__ __forceinline__ float fast_test(const FastPrim &P, const Ray32 &R, bo
         bool v;
         if constexpr (KIND == SEG_RING) {
             const float rr = __builtin_fmaf(hx, hx, hy * hy);
-            v = (t > EPS_F) & (rr >= P.k0) & (rr <= P.k1);
+            v = (t > eps) & (rr >= P.k0) & (rr <= P.k1);
         } else {
-            v = (t > EPS_F) & (__builtin_fabsf(hx) <= 1.0f) & (__builtin_fabsf(hy) <= 1.0f);
+            v = (t > eps) & (__builtin_fabsf(hx) <= 1.0f) & (__builtin_fabsf(hy) <= 1.0f);
         }
         return v ? t : finf();
     }
 }
 
-__device__ __forceinline__ float fast_test_any(const FastPrim &P, const Ray32 &R, bool &flag, float tau)
+__device__ __forceinline__ float fast_test_any(const FastPrim &P, const Ray32 &R, bool &flag, float tau, float eps)
 {
     switch (P.type) {
-    case RF_CYLINDER: return fast_test<SEG_CYL>(P, R, flag, tau);
-    case RF_FRUSTUM: return fast_test<SEG_FRUSTUM>(P, R, flag, tau);
-    case RF_SPHERE: return fast_test<SEG_SPHERE>(P, R, flag, tau);
-    case RF_RECTANGLE: return fast_test<SEG_RECT>(P, R, flag, tau);
-    default: return fast_test<SEG_RING>(P, R, flag, tau);
+    case RF_CYLINDER: return fast_test<SEG_CYL>(P, R, flag, tau, eps);
+    case RF_FRUSTUM: return fast_test<SEG_FRUSTUM>(P, R, flag, tau, eps);
+    case RF_SPHERE: return fast_test<SEG_SPHERE>(P, R, flag, tau, eps);
+    case RF_RECTANGLE: return fast_test<SEG_RECT>(P, R, flag, tau, eps);
+    default: return fast_test<SEG_RING>(P, R, flag, tau, eps);
     }
 }
 
@@ -280,7 +307,8 @@ struct Pending {
     int count;
     int resume;        // first dropped pair key, NO_KEY if none
     int from;          // re-scan: pairs with key < from are skipped
-    float tau;         // relative flag margin
+    float tau;         // relative flag margin (VGPR copy)
+    float eps;         // EPS_F (VGPR copy)
 };
 
 // Wave-wide minimum of a lane value (re-scan start).
@@ -311,13 +339,13 @@ __device__ __forceinline__ void seg_loop(CPTR(FastPrim) F, int begin, int end, c
                                          float (&tb)[NR], int (&ib)[NR], Pending &pd, int slot = 0)
 {
     for (int w = begin; w < end; ++w) {
-        const FastPrim q = ldc(F + w);
+        const FastPrim q = vcopy(ldc(F + w));
         float t[NR];
         bool f[NR];
         bool any = false;
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
-            t[i] = fast_test<KIND>(q, R[i], f[i], pd.tau);
+            t[i] = fast_test<KIND>(q, R[i], f[i], pd.tau, pd.eps);
             if constexpr (RESCAN) {
                 const bool skip = w < pd.from;
                 f[i] = f[i] & !skip;
@@ -462,7 +490,7 @@ __device__ __forceinline__ int find_fast_range(CPTR(FastPrim) F, CPTR(rf_primiti
     for (int w = start; w < end; ++w) {
         const FastPrim q = ldc(F + w);
         bool f;
-        float t = fast_test_any(q, R, f, tau);
+        float t = fast_test_any(q, R, f, tau, EPS_F);
         if constexpr (REFINE) {
             if (__builtin_amdgcn_ballot_w64(f)) {
                 if (f) { t = refine_prim(P, w, R); ++nref; }
@@ -489,7 +517,7 @@ __device__ __forceinline__ int find_bv_fast(CPTR(GeomDesc) G, const Ray32 &R, ui
         const FastPrim q = ldc(FB + w);
         const BvMeta meta = ldc(M + w);
         bool f;
-        float t = fast_test_any(q, R, f, tau);
+        float t = fast_test_any(q, R, f, tau, EPS_F);
         if constexpr (REFINE) {
             if (__builtin_amdgcn_ballot_w64(f)) {
                 if (f) { t = refine_bv(B, w, R); ++nref; }
@@ -562,7 +590,7 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
             emit_group<NR>(kp, gid, emitter, wi0, base, E, R);
 #pragma unroll
             for (int i = 0; i < NR; ++i) { tb[i] = finf(); ib[i] = -1; }
-            Pending pd{pend, 0, NO_KEY, 0, kp.tau};
+            Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
             scan_fast<NR, REFINE>(G, R, tb, ib, pd);
             if constexpr (REFINE) {
                 if (__builtin_amdgcn_ballot_w64(pd.count > 0))
@@ -600,7 +628,7 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                         Ray32 R1[1] = {ray};
                         float tb1[1] = {finf()};
                         int ib1[1] = {-1};
-                        Pending pd{pend, 0, NO_KEY, 0, kp.tau};
+                        Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
                         scan_fast<1, REFINE>(G, R1, tb1, ib1, pd);
                         if constexpr (REFINE) {
                             if (__builtin_amdgcn_ballot_w64(pd.count > 0))
@@ -650,27 +678,28 @@ __device__ __forceinline__ void flush_stats(const KParams &kp, uint64_t tests, u
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 template <int KIND>
-__device__ __forceinline__ float quad_test(const f32x4 &o, const f32x4 &d, bool &flag, float tau)
+__device__ __forceinline__ float quad_test(const f32x4 &o, const f32x4 &d, bool &flag, float tau, float eps)
 {
     if constexpr (KIND == SEG_CYL) {
         const float sx = o.x, sy = o.y, sz = o.z, cx = d.x, cy = d.y, cz = d.z;
         const float A = __builtin_fmaf(cx, cx, cy * cy);
         const float b = __builtin_fmaf(sx, cx, sy * cy);
-        const float C = __builtin_fmaf(sx, sx, __builtin_fmaf(sy, sy, -1.0f));
-        const float D = __builtin_fmaf(b, b, -(A * C));
+        const float w = __builtin_fmaf(sx, cy, -(sy * cx));
+        const float D = __builtin_fmaf(-w, w, A);
         const float sq = sqrt_fast(D);
         const float rA = rcp_fast(A);
-        const float t1 = (-b - sq) * rA;
-        const float t2 = (sq - b) * rA;
+        const float tm = -b * rA;
+        const float t1 = __builtin_fmaf(-sq, rA, tm);
+        const float t2 = __builtin_fmaf(sq, rA, tm);
         const float z1 = __builtin_fmaf(cz, t1, sz);
         const float z2 = __builtin_fmaf(cz, t2, sz);
-        const bool v1 = (t1 > EPS_F) & (__builtin_fabsf(z1) <= 0.5f);
-        const bool v2 = (t2 > EPS_F) & (__builtin_fabsf(z2) <= 0.5f);
-        const float tm = -b * rA;
+        const bool e1 = t1 > eps, e2 = t2 > eps;
+        const bool v1 = e1 & (__builtin_fabsf(z1) <= 0.5f);
+        const bool v2 = e2 & (__builtin_fabsf(z2) <= 0.5f);
         const float zm = __builtin_fmaf(cz, tm, sz);
-        flag = ((__builtin_fabsf(D) <= tau * (b * b)) & (tm > 0.0f) & (__builtin_fabsf(zm) <= 0.5f + tau)) |
-               ((t1 > 0.0f) & (__builtin_fabsf(__builtin_fabsf(z1) - 0.5f) <= tau)) |
-               ((t2 > 0.0f) & (__builtin_fabsf(__builtin_fabsf(z2) - 0.5f) <= tau));
+        flag = ((__builtin_fabsf(D) <= tau * A) & (__builtin_fabsf(zm) <= 0.5f + tau)) |
+               (e1 & (__builtin_fabsf(__builtin_fabsf(z1) - 0.5f) <= tau)) |
+               (e2 & (__builtin_fabsf(__builtin_fabsf(z2) - 0.5f) <= tau));
         return v1 ? t1 : (v2 ? t2 : finf());
     } else if constexpr (KIND == SEG_FRUSTUM) {
         const float sx = o.x, sy = o.y, sw = o.z, sz = o.w, cx = d.x, cy = d.y, cw = d.z, cz = d.w;
@@ -681,20 +710,21 @@ __device__ __forceinline__ float quad_test(const f32x4 &o, const f32x4 &d, bool 
         const float D = __builtin_fmaf(b, b, -(A * C));
         const float sq = sqrt_fast(D);
         const float rA = rcp_fast(A);
-        const float ta = (-b - sq) * rA;
-        const float tb = (sq - b) * rA;
+        const float tm = -b * rA;
+        const float ta = __builtin_fmaf(-sq, rA, tm);
+        const float tb = __builtin_fmaf(sq, rA, tm);
         const float t1 = __builtin_fminf(ta, tb);
         const float t2 = __builtin_fmaxf(ta, tb);
         const float z1 = __builtin_fmaf(cz, t1, sz);
         const float z2 = __builtin_fmaf(cz, t2, sz);
-        const bool v1 = (t1 > EPS_F) & (__builtin_fabsf(z1) <= 0.5f);
-        const bool v2 = (t2 > EPS_F) & (__builtin_fabsf(z2) <= 0.5f);
-        const float tm = -b * rA;
+        const bool e1 = t1 > eps, e2 = t2 > eps;
+        const bool v1 = e1 & (__builtin_fabsf(z1) <= 0.5f);
+        const bool v2 = e2 & (__builtin_fabsf(z2) <= 0.5f);
         const float zm = __builtin_fmaf(cz, tm, sz);
-        flag = ((__builtin_fabsf(D) <= tau * (b * b)) & (tm > 0.0f) & (__builtin_fabsf(zm) <= 0.5f + tau)) |
+        flag = ((__builtin_fabsf(D) <= tau * (b * b)) & (__builtin_fabsf(zm) <= 0.5f + tau)) |
                (__builtin_fabsf(A) < tau * Axy) |
-               ((t1 > 0.0f) & (__builtin_fabsf(__builtin_fabsf(z1) - 0.5f) <= tau)) |
-               ((t2 > 0.0f) & (__builtin_fabsf(__builtin_fabsf(z2) - 0.5f) <= tau));
+               (e1 & (__builtin_fabsf(__builtin_fabsf(z1) - 0.5f) <= tau)) |
+               (e2 & (__builtin_fabsf(__builtin_fabsf(z2) - 0.5f) <= tau));
         return v1 ? t1 : (v2 ? t2 : finf());
     } else if constexpr (KIND == SEG_SPHERE) {
         const float sx = o.x, sy = o.y, sz = o.z, cx = d.x, cy = d.y, cz = d.z;
@@ -707,7 +737,7 @@ __device__ __forceinline__ float quad_test(const f32x4 &o, const f32x4 &d, bool 
         const float t1 = (-b - sq) * rA;
         const float t2 = (sq - b) * rA;
         flag = (__builtin_fabsf(D) <= tau * (b * b)) & (b < 0.0f);
-        return (t1 > EPS_F) ? t1 : ((t2 > EPS_F) ? t2 : finf());
+        return (t1 > eps) ? t1 : ((t2 > eps) ? t2 : finf());
     } else {
         const float t = -o.z * rcp_fast(d.z);
         const float hx = __builtin_fmaf(d.x, t, o.x);
@@ -716,9 +746,9 @@ __device__ __forceinline__ float quad_test(const f32x4 &o, const f32x4 &d, bool 
         bool v;
         if constexpr (KIND == SEG_RING) {
             const float rr = __builtin_fmaf(hx, hx, hy * hy);
-            v = (t > EPS_F) & (rr <= 1.0f) & (rr >= o.w);
+            v = (t > eps) & (rr <= 1.0f) & (rr >= o.w);
         } else {
-            v = (t > EPS_F) & (__builtin_fabsf(hx) <= 1.0f) & (__builtin_fabsf(hy) <= 1.0f);
+            v = (t > eps) & (__builtin_fabsf(hx) <= 1.0f) & (__builtin_fabsf(hy) <= 1.0f);
         }
         return v ? t : finf();
     }
@@ -747,7 +777,7 @@ __device__ __forceinline__ void quad_loop(const float *__restrict__ Q, int qb, i
         bool any = false;
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
-            t[st] = quad_test<KIND>(Do[st], Dd[st], f[st], pd.tau);
+            t[st] = quad_test<KIND>(Do[st], Dd[st], f[st], pd.tau, pd.eps);
             if constexpr (RESCAN) {
                 const bool skip = q * 4 + st < pd.from;
                 f[st] = f[st] & !skip;
@@ -849,7 +879,7 @@ __device__ __forceinline__ void run_unit_mfma(const KParams &kp, CPTR(GeomDesc) 
         mfma_rays(kp, gid, emitter, wi0, rl, E, lane, Bo, Bd);
 #pragma unroll
         for (int st = 0; st < 4; ++st) { tb[st] = finf(); ib[st] = -1; }
-        Pending pd{pend, 0, NO_KEY, 0, kp.tau};
+        Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
         quad_loop<SEG_CYL, REFINE, false>(Q, G->qseg_begin[SEG_CYL], G->qseg_end[SEG_CYL], Bo, Bd, tb, ib, pd, lane);
         quad_loop<SEG_FRUSTUM, REFINE, false>(Q, G->qseg_begin[SEG_FRUSTUM], G->qseg_end[SEG_FRUSTUM], Bo, Bd, tb, ib,
                                               pd, lane);
