@@ -25,7 +25,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector (packed) / dense FP32 peak
+PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector peak: 256 CUs x 4 SIMDs x 32 FMA lanes/clk x 2.4 GHz x 2
 PEAK_HBM_GBS = 8000.0
 
 
@@ -49,22 +49,32 @@ def build_workload(args):
     return geoms, desc
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(geoms, seconds: float, threads: int):
     """Oracle (compat FP32 restatement of the reference kernel) on host cores,
     a bounded sample: every emitter of the median-size geometry, ray budget
-    scaled so the run takes about `seconds`."""
+    scaled (two calibration passes) so the timed run takes about `seconds`."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     sizes = [g.n for g in geoms]
     g = geoms[int(np.argsort(sizes)[len(sizes) // 2])]
     n = g.n
-    # calibrate on one emitter
-    wi = 256
-    t0 = time.perf_counter()
-    O.solve(g.prims, None, n_wi=wi, mode="compat", rng="seeded", threads=threads, emitters=(0, min(n, 4)))
-    dt = time.perf_counter() - t0
-    tests_per_s = 4 * wi * n * min(n, 4) / max(dt, 1e-6)
-    n_wi = max(16, int(seconds * tests_per_s / (4.0 * n * n)))
+    n_wi = 16
+    for budget in (0.2, 2.0):                   # calibration passes (seconds)
+        t0 = time.perf_counter()
+        O.solve(g.prims, None, n_wi=n_wi, mode="compat", rng="seeded", threads=threads)
+        dt = max(time.perf_counter() - t0, 1e-6)
+        n_wi = max(16, int(n_wi * budget / dt))
+    n_wi = max(16, int(n_wi * seconds / 2.0))
     t0 = time.perf_counter()
     hits, rays = O.solve(g.prims, None, n_wi=n_wi, mode="compat", rng="seeded", threads=threads)
     dt = time.perf_counter() - t0
@@ -73,7 +83,8 @@ def cpu_baseline(geoms, seconds: float, threads: int):
     return {
         "value": tests / dt, "unit": "tests/s", "rays_per_s": total_rays / dt, "cores": threads, "kind": "port",
         "sample": f"compat FP32 oracle, all {n} emitters of one {n}-primitive sweep geometry, "
-                  f"{4 * n_wi} rays/emitter ({tests:.3e} tests, {dt:.1f} s)",
+                  f"{4 * n_wi} rays/emitter ({tests:.3e} tests, {dt:.1f} s) on {threads} threads of "
+                  f"{os.cpu_count()} x {cpu_model()}",
     }
 
 
@@ -106,6 +117,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-refine", action="store_true")
+    ap.add_argument("--traffic-file", default=None,
+                    help="measured HBM bytes per launch of the solve kernel (JSON written from rocprofv3 "
+                         "FETCH_SIZE/WRITE_SIZE passes); default profiles/hbm_traffic_<workload>.json")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (scales the HBM byte counters)")
     args = ap.parse_args()
@@ -123,11 +137,15 @@ def main() -> int:
     dev = [rf.DeviceScene(ctx, g.prims, None) for g in geoms]
     rays = int(args.rays)
     refine = not args.no_refine
+    rpt, blk = 4, 256          # library defaults (VALU transform)
     if not args.no_tune:
         t0 = time.perf_counter()
         rpt, blk = rf.autotune(ctx, dev, rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays,
                                                       refine=refine))
-        log(f"autotune: {rpt} rays/thread, {blk} threads/group ({time.perf_counter() - t0:.1f} s)")
+        log(f"autotune: {'MFMA transform' if rpt == 0 else f'{rpt} rays/thread'}, {blk} threads/group "
+            f"({time.perf_counter() - t0:.1f} s)")
+    kernel = (f"rf::k_queue_mfma<{str(refine).lower()}, false>" if rpt == 0 else
+              f"rf::k_queue<2, false, false, {rpt}, {str(refine).lower()}>")
     opts = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", refine=refine,
                         seed=args.seed)
     for i in range(args.warmup):
@@ -181,6 +199,16 @@ def main() -> int:
             "batched_over_reference_form": res["batched"]["tests_per_s"] / res["reference_form"]["tests_per_s"],
         }
 
+    traffic = None
+    tfile = Path(args.traffic_file) if args.traffic_file else ROOT / "profiles" / f"hbm_traffic_{args.workload}{args.geoms if args.workload == 'sweep' else ''}.json"
+    if tfile.exists():
+        rec = json.loads(tfile.read_text())
+        if rec.get("kernel") == kernel and int(rec.get("rays_per_emitter", -1)) == rays:
+            traffic = rec["hbm_bytes_per_launch"]
+            log(f"traffic: {traffic:.3e} B per launch from {tfile.name}")
+        else:
+            log(f"traffic: {tfile.name} is for another kernel or budget; not used")
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -208,8 +236,8 @@ def main() -> int:
         "rays_per_s": nrays / wall,
         "refined_pairs": refined,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "rf::k_queue<fast,plain>", "avg_launch_ms": kernel_ms / max(launches, 1),
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "kernel": kernel, "avg_launch_ms": kernel_ms / max(launches, 1),
                      "flops_per_launch": flops / max(launches, 1)},
         "plain_comparison": plain,
         "cpu_baseline": cpu,

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <numeric>
 #include <set>
 #include <string>
 #include <vector>
@@ -659,25 +660,32 @@ static int solve_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, c
     // geometry descriptors
     std::vector<GeomDesc> descs(n_geoms);
     int max_n = 0;
-    int64_t total_wi = 0;
     for (int g = 0; g < n_geoms; ++g) {
         descs[g] = scenes[g]->desc;
         descs[g].geometry_id = ids[g];
         max_n = std::max(max_n, scenes[g]->n);
-        for (int64_t w : budgets[g]) total_wi += w;
     }
     const int groups = max_groups(ctx, block);
     const int64_t pass_wi = std::max<int64_t>(1, (int64_t)block * (nr >= 4 ? nr / 4 : 1));
-    // ~8 units per resident group for balance, each at most 16384 work items
-    int64_t target = total_wi / ((int64_t)groups * 8);
-    target = std::max<int64_t>(pass_wi, std::min<int64_t>(target, 16384));
-    target = ((target + pass_wi - 1) / pass_wi) * pass_wi;
+    // Units by cost (work items x primitives): about 8 per resident group for
+    // balance.  An emitter is split only when it costs more than that, so most
+    // emitter rows are flushed from LDS once (one 64-bit atomic per bin).
+    double total_cost = 0.0;
+    for (int g = 0; g < n_geoms; ++g) total_cost += (double)scenes[g]->n * (double)std::accumulate(
+                                                        budgets[g].begin(), budgets[g].end(), (int64_t)0);
+    const double unit_cost = std::max(1.0, total_cost / ((double)groups * 8.0));
+    const int64_t max_unit_wi = (int64_t)1 << 28;   // 32-bit LDS bins: < 2^30 rays per unit
     std::vector<WorkUnit> units;
     for (int g = 0; g < n_geoms; ++g)
         for (int e = 0; e < scenes[g]->n; ++e) {
             const int64_t nw = budgets[g][e];
-            for (int64_t w0 = 0; w0 < nw; w0 += target)
-                units.push_back(WorkUnit{g, e, w0, std::min(target, nw - w0), 0});
+            if (nw <= 0) continue;
+            const int64_t pieces = std::max<int64_t>(
+                (int64_t)std::ceil((double)nw * scenes[g]->n / unit_cost), (nw + max_unit_wi - 1) / max_unit_wi);
+            int64_t size = (nw + pieces - 1) / pieces;
+            size = ((size + pass_wi - 1) / pass_wi) * pass_wi;
+            for (int64_t w0 = 0; w0 < nw; w0 += size)
+                units.push_back(WorkUnit{g, e, w0, std::min(size, nw - w0), 0});
         }
     if (units.empty()) return RF_OK;
     if (units.size() > (size_t)INT32_MAX) return set_err(ctx, RF_ERR_CAPACITY, "too many work units");
