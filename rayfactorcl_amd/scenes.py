@@ -285,9 +285,11 @@ def sweep(n_geoms: int = 256, seed: int = 0x52465331) -> list[SceneData]:
     return [parse_xml(sweep_xml(p)) for p in sweep_params(n_geoms, seed)]
 
 
-def streaming_requests(n: int, seed: int = 0x52465331, rate_hz: float = 2.0, max_prims: int = 2000):
+def streaming_requests(n: int, seed: int = 0x52465331, rate_hz: float = 2.0, max_prims: int = 2000,
+                       min_rays: float = 1e5, max_rays: float = 1e8):
     """Poisson-arrival solve requests: (arrival_s, scene, rays_per_emitter).
-    Scene sizes U{10..max_prims} drawn as a furnace or a soup; budgets log-U[1e5, 1e8]."""
+    Scene sizes U{10..max_prims} drawn as a furnace or a soup; budgets
+    log-U[min_rays, max_rays] (default [1e5, 1e8])."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -300,6 +302,6 @@ def streaming_requests(n: int, seed: int = 0x52465331, rate_hz: float = 2.0, max
             sc = parse_xml(furnace_xml(wall_bands=wb, preform_bands=pb))
         else:
             sc = parse_xml(soup_xml(size, seed=seed + i + 1))
-        rays = int(round(float(np.exp(rng.uniform(math.log(1e5), math.log(1e8))))))
+        rays = int(round(float(np.exp(rng.uniform(math.log(min_rays), math.log(max_rays))))))
         out.append((t, sc, rays))
     return out

@@ -59,3 +59,14 @@ def test_autotune_keeps_results(ctx):
     for a, b, c, w in zip(before["valu"], after_valu, after_auto, want):
         assert np.array_equal(a.hits, b.hits)
         assert np.array_equal(c.hits, w.hits)
+
+
+def test_configs_report_quick(ctx):
+    """The per-configuration report runs end to end (reduced budgets)."""
+    from rayfactorcl_amd import configs
+    r1 = configs.config1(ctx, quick=True)
+    assert all(c["within_4sigma_plus_edge"] for c in r1["fast"]["closed_forms"])
+    assert all(c["within_4sigma_plus_edge"] for c in r1["compat"]["closed_forms"])
+    assert r1["fast_vs_fp64_max_dvf"] <= 1e-4 * 10     # 1e6 rays: binomial noise of differing rays
+    r5 = configs.config5(ctx, quick=True, n_req=6)
+    assert r5["requests"] == 6 and r5["latency_p99_s"] >= r5["latency_p50_s"] > 0
