@@ -86,7 +86,7 @@ def config2(ctx, quick: bool) -> dict:
     sc = scenes.parse_xml(scenes.furnace_xml(with_bv=True))
     dev = DeviceScene(ctx, sc.prims, sc.bvs)
     rays = 10_000_000 if quick else 100_000_000
-    out = {"config": 2, "scene": "furnace: 180 wall bands + 60 preform bands + neck + annuli + end disc, 1 BV",
+    out = {"config": 2, "scene": "furnace: 180 wall bands + 60 preform bands + neck + annuli + end disc, one bounding surface per group",
            "primitives": sc.n, "bounding_volumes": len(sc.bvs), "rays_per_emitter": rays}
     for bv in (False, True):
         key = "with_bv" if bv else "without_bv"
