@@ -43,6 +43,13 @@ struct GeomDesc {
     const int32_t *orig2pos;
     int32_t qseg_begin[N_SEG];
     int32_t qseg_end[N_SEG];
+    // Children of each bounding surface in type-segmented order (binned BV
+    // kernel): segment s of BV b spans [bvc_seg[b*(N_SEG+1)+s], bvc_seg[b*(N_SEG+1)+s+1])
+    // of bvc_fast, whose records carry the child's index in the caller's order.
+    const rfd::FastPrim *bvc_fast;
+    const int32_t *bvc_seg;
+    int32_t bv_binned;                  // 1: no BV asks for the intermediate check (settings != 1)
+    int32_t pad2;
 };
 
 // One work unit: work items [wi0, wi0+n_wi) of one emitter of one geometry.
@@ -75,6 +82,7 @@ struct KParams {
     int64_t imp_n_wi;
     int64_t imp_unit;
     float tau;                          // fast form: relative margin of the FP64 re-solve flags
+    int32_t bv_slots;                   // binned BV kernel: LDS buckets (>= n_bvs of every geometry)
 };
 
 enum KMath { KM_COMPAT = 0, KM_FP64 = 1, KM_FAST = 2 };
@@ -89,6 +97,7 @@ struct LaunchCfg {
     int grid;        // workgroups
     int direct;      // 1: reference form (one work item per thread, global atomics, no queue)
     int mfma;        // 1: fast plain form with the transform on the matrix cores
+    int bvbin;       // 1: fast BV form with rays binned by bounding surface
     size_t lds_bytes;
 };
 

@@ -333,10 +333,13 @@ __device__ __forceinline__ int opaque_zero()
 }
 
 // RESCAN: one ray (NR = 1) of lane slot `slot`, positions below pd.from
-// skipped, `resume` records a position instead of a key.
+// skipped, `resume` records a position instead of a key.  act = false: the
+// lane only rides along (its flags are dropped; its minimum is the caller's
+// concern).
 template <int KIND, int NR, bool REFINE, bool RESCAN>
 __device__ __forceinline__ void seg_loop(CPTR(FastPrim) F, int begin, int end, const Ray32 (&R)[NR],
-                                         float (&tb)[NR], int (&ib)[NR], Pending &pd, int slot = 0)
+                                         float (&tb)[NR], int (&ib)[NR], Pending &pd, int slot = 0,
+                                         bool act = true)
 {
     for (int w = begin; w < end; ++w) {
         const FastPrim q = vcopy(ldc(F + w));
@@ -351,6 +354,7 @@ __device__ __forceinline__ void seg_loop(CPTR(FastPrim) F, int begin, int end, c
                 f[i] = f[i] & !skip;
                 t[i] = skip ? finf() : t[i];
             }
+            f[i] = f[i] & act;
             any |= f[i];
         }
         if constexpr (REFINE && KIND <= SEG_SPHERE) {
@@ -380,9 +384,10 @@ __device__ __forceinline__ void seg_loop(CPTR(FastPrim) F, int begin, int end, c
 // regenerated from its index (emission is a pure function of the random
 // block), so the scan's ray registers are dead here.
 // ONE: every entry belongs to one ray, merged into tb[0] / ib[0].
-template <int NR, bool ONE = false>
+// ray_of(k): unit-local index of the ray in slot k.
+template <int NR, bool ONE = false, class RayOf>
 __device__ __forceinline__ void resolve_pending(const KParams &kp, CPTR(GeomDesc) G, const rf_primitive &E,
-                                                uint32_t gid, int emitter, int64_t wi0, int64_t base,
+                                                uint32_t gid, int emitter, int64_t wi0, RayOf ray_of,
                                                 float (&tb)[NR], int (&ib)[NR], Pending &pd, uint32_t &nref)
 {
     CPTR(rf_primitive) P = cst(G->prims);
@@ -390,7 +395,7 @@ __device__ __forceinline__ void resolve_pending(const KParams &kp, CPTR(GeomDesc
         const uint32_t ent = pd.buf[e * blockDim.x + threadIdx.x];
         const int prim = (int)(ent >> 3);
         const int k = (int)(ent & 7u);
-        const int64_t rl = base + k;
+        const int64_t rl = ray_of(k);
         float f[4], r4[4];
         workitem_floats(kp, gid, emitter, wi0 + (rl >> 2), f);
         rotate_rands(f, (int)(rl & 3), r4);
@@ -408,14 +413,15 @@ __device__ __forceinline__ void resolve_pending(const KParams &kp, CPTR(GeomDesc
 // Pairs dropped from a full list (first dropped key pd.resume): re-scan the
 // quadrics for one ray slot at a time, from the first position whose pair of
 // that slot may have been dropped, re-solving each time the list fills.
-template <int NR>
+// Segment s of the scanned record array F spans [sb[s], se[s]).
+template <int NR, class RayOf>
 __device__ __forceinline__ void finish_dropped(const KParams &kp, CPTR(GeomDesc) G, const rf_primitive &E,
-                                               uint32_t gid, int emitter, int64_t wi0, int64_t base,
+                                               uint32_t gid, int emitter, int64_t wi0, RayOf ray_of,
+                                               CPTR(FastPrim) F, CPTR(int32_t) sb, CPTR(int32_t) se,
                                                float (&tb)[NR], int (&ib)[NR], Pending &pd, uint32_t &nref,
                                                uint32_t &nover)
 {
     const int K = pd.resume;
-    CPTR(FastPrim) F = cst(G->fast);
 #pragma unroll 1
     for (int i = 0; i < NR; ++i) {
         int from = K == NO_KEY ? NO_KEY : (K >> 3) + (i < (K & 7) ? 1 : 0);
@@ -426,7 +432,7 @@ __device__ __forceinline__ void finish_dropped(const KParams &kp, CPTR(GeomDesc)
             Ray32 R1[1];
             {
                 float f[4], r4[4];
-                const int64_t rl = base + i + opaque_zero();
+                const int64_t rl = ray_of(i) + opaque_zero();
                 workitem_floats(kp, gid, emitter, wi0 + (rl >> 2), f);
                 rotate_rands(f, (int)(rl & 3), r4);
                 R1[0] = emit_fast(&E, r4);
@@ -435,12 +441,12 @@ __device__ __forceinline__ void finish_dropped(const KParams &kp, CPTR(GeomDesc)
             pd.resume = NO_KEY;
             pd.from = from;
             const int lo = wave_min(from);
-            const auto beg = [&](int sg) { return G->seg_begin[sg] > lo ? G->seg_begin[sg] : lo; };
-            seg_loop<SEG_CYL, 1, true, true>(F, beg(SEG_CYL), G->seg_end[SEG_CYL], R1, tr, ir, pd, i);
-            seg_loop<SEG_FRUSTUM, 1, true, true>(F, beg(SEG_FRUSTUM), G->seg_end[SEG_FRUSTUM], R1, tr, ir, pd, i);
-            seg_loop<SEG_SPHERE, 1, true, true>(F, beg(SEG_SPHERE), G->seg_end[SEG_SPHERE], R1, tr, ir, pd, i);
+            const auto beg = [&](int sg) { return sb[sg] > lo ? sb[sg] : lo; };
+            seg_loop<SEG_CYL, 1, true, true>(F, beg(SEG_CYL), se[SEG_CYL], R1, tr, ir, pd, i);
+            seg_loop<SEG_FRUSTUM, 1, true, true>(F, beg(SEG_FRUSTUM), se[SEG_FRUSTUM], R1, tr, ir, pd, i);
+            seg_loop<SEG_SPHERE, 1, true, true>(F, beg(SEG_SPHERE), se[SEG_SPHERE], R1, tr, ir, pd, i);
             if (__builtin_amdgcn_ballot_w64(pd.count > 0))
-                resolve_pending<1, true>(kp, G, E, gid, emitter, wi0, base, tr, ir, pd, nref);
+                resolve_pending<1, true>(kp, G, E, gid, emitter, wi0, ray_of, tr, ir, pd, nref);
             from = pd.resume;
         }
 #pragma unroll
@@ -593,10 +599,12 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
             Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
             scan_fast<NR, REFINE>(G, R, tb, ib, pd);
             if constexpr (REFINE) {
+                const auto ray_of = [base](int k) { return base + k; };
                 if (__builtin_amdgcn_ballot_w64(pd.count > 0))
-                    resolve_pending<NR>(kp, G, E, gid, emitter, wi0, base, tb, ib, pd, nref);
+                    resolve_pending<NR>(kp, G, E, gid, emitter, wi0, ray_of, tb, ib, pd, nref);
                 if (__builtin_amdgcn_ballot_w64(pd.resume != NO_KEY))
-                    finish_dropped<NR>(kp, G, E, gid, emitter, wi0, base, tb, ib, pd, nref, nover);
+                    finish_dropped<NR>(kp, G, E, gid, emitter, wi0, ray_of, cst(G->fast), G->seg_begin, G->seg_end,
+                                       tb, ib, pd, nref, nover);
             }
 #pragma unroll
             for (int i = 0; i < NR; ++i)
@@ -631,10 +639,13 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                         Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
                         scan_fast<1, REFINE>(G, R1, tb1, ib1, pd);
                         if constexpr (REFINE) {
+                            const int64_t r0 = wl * 4 + v;
+                            const auto ray_of = [r0](int k) { return r0 + k; };
                             if (__builtin_amdgcn_ballot_w64(pd.count > 0))
-                                resolve_pending<1>(kp, G, E, gid, emitter, wi0, wl * 4 + v, tb1, ib1, pd, nref);
+                                resolve_pending<1>(kp, G, E, gid, emitter, wi0, ray_of, tb1, ib1, pd, nref);
                             if (__builtin_amdgcn_ballot_w64(pd.resume != NO_KEY))
-                                finish_dropped<1>(kp, G, E, gid, emitter, wi0, wl * 4 + v, tb1, ib1, pd, nref, nover);
+                                finish_dropped<1>(kp, G, E, gid, emitter, wi0, ray_of, cst(G->fast), G->seg_begin,
+                                                  G->seg_end, tb1, ib1, pd, nref, nover);
                         }
                         idx = ib1[0];
                     }
@@ -981,6 +992,245 @@ __global__ void __launch_bounds__(256) k_queue_mfma(KParams kp)
     flush_stats(kp, 0, nref, nover);
 }
 
+// ---------------------------------------------------------------------------
+// Fast BV form with rays binned by bounding surface: runBVGPU semantics
+// (RayFactorGPU.cl:530-626) for scenes whose bounding surfaces do not ask for
+// the intermediate check.  Per pass of blockDim x BNR x BSUB rays:
+//   A. each thread emits BSUB x BNR rays and finds each ray's nearest
+//      bounding surface (strict <, so the lowest BV index wins ties; flagged
+//      BV roots are re-solved in FP64);
+//   B. a counting sort in LDS buckets the rays by that surface;
+//   C. every wave takes 64 x BNR rays of one bucket (buckets start on whole
+//      waves) and scans that surface's children with the type-segmented fast
+//      loop, BNR rays per lane.
+// A ray that hits no bounding surface hits nothing, as in the reference.
+// One ray per thread with divergent child ranges would scan the union of
+// the wave's ranges; binned, a wave scans exactly one range.
+constexpr int BNR = 4;
+constexpr int BSUB = 4;          // sub-passes bucketed together: less padding per bucket
+constexpr int WAVE_RAYS = 64 * BNR;
+
+template <bool REFINE, bool TRACE>
+__device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc) G, uint32_t gid, int emitter,
+                                               int64_t wi0, int64_t n_wi, uint32_t *bins, uint32_t *pend,
+                                               uint32_t *ids, uint32_t *cnt, uint32_t *off, uint32_t *cur,
+                                               uint32_t *pad, int *s_total, uint64_t &tests, uint32_t &nref,
+                                               uint32_t &nover)
+{
+    const rf_primitive E = ldc(cst(G->prims) + emitter);
+    const int64_t ray0 = (wi0 - kp.trace_wi0) * 4;
+    const int64_t n_rays = n_wi * 4;
+    const int nbv = G->n_bvs;
+    CPTR(FastPrim) FB = cst(G->fast_bvs);
+    CPTR(rf_bounding_volume) B = cst(G->bvs);
+    CPTR(BvMeta) M = cst(G->bv_meta);
+    CPTR(FastPrim) FC = cst(G->bvc_fast);
+    const float tau = vcopy(kp.tau), eps = vcopy(EPS_F);
+    const int SP = (int)blockDim.x * BNR;            // rays per sub-pass
+    const int P = SP * BSUB;                          // rays per pass (bucketed together)
+    const int lid0 = (int)threadIdx.x * BNR;
+    for (int64_t pass0 = 0; pass0 < n_rays; pass0 += P) {
+        for (int b = threadIdx.x; b < nbv; b += blockDim.x) cnt[b] = 0u;
+        __syncthreads();
+        // A: nearest bounding surface of this thread's rays, BSUB sub-passes
+        // (-2: beyond the unit)
+        int bvi[BSUB * BNR];
+#pragma unroll
+        for (int sp = 0; sp < BSUB; ++sp) {
+            const int lsp = sp * SP + lid0;
+            Ray32 R[BNR];
+            emit_group<BNR>(kp, gid, emitter, wi0, pass0 + lsp, E, R);
+            float tbv[BNR];
+            int bv[BNR];
+#pragma unroll
+            for (int i = 0; i < BNR; ++i) { tbv[i] = finf(); bv[i] = -1; }
+            for (int w = 0; w < nbv; ++w) {
+                const FastPrim q = vcopy(ldc(FB + w));
+#pragma unroll
+                for (int i = 0; i < BNR; ++i) {
+                    bool f;
+                    float t = fast_test_any(q, R[i], f, tau, eps);
+                    if constexpr (REFINE) {
+                        if (__builtin_amdgcn_ballot_w64(f)) {
+                            if (f) {
+                                // only quadrics flag
+                                const rf_bounding_volume bvr = ldc(B + w);
+                                t = (float)hit_d_quadric(bvr.type, bvr.invm, bvr.util1, widen(R[i]));
+                                ++nref;
+                            }
+                        }
+                    }
+                    if (t < tbv[i]) { tbv[i] = t; bv[i] = w; }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < BNR; ++i) {
+                if (pass0 + lsp + i >= n_rays) {
+                    bv[i] = -2;
+                } else {
+                    tests += (uint64_t)nbv;
+                    if (bv[i] >= 0) atomicAdd(&cnt[bv[i]], 1u);
+                    else tally<TRACE, false>(kp, G, emitter, ray0 + pass0 + lsp + i, -1, bins);
+                }
+                bvi[sp * BNR + i] = bv[i];
+            }
+        }
+        __syncthreads();
+        // B: counting sort by bounding surface.  Rays are stored compactly by
+        // bucket (off[b] = first slot of bucket b); each bucket also gets a
+        // range of whole-wave slots (pad[b], a multiple of 64 x BNR), so that
+        // every wave of phase C reads rays of one bucket only.
+        if (threadIdx.x == 0) {
+            uint32_t sum = 0, psum = 0;
+            for (int b = 0; b < nbv; ++b) {
+                const uint32_t c = cnt[b];
+                off[b] = sum;
+                cur[b] = sum;
+                pad[b] = psum;
+                sum += c;
+                psum += (c + WAVE_RAYS - 1) / WAVE_RAYS * WAVE_RAYS;
+            }
+            *s_total = (int)psum;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < BSUB * BNR; ++k) {
+            if (bvi[k] >= 0) {
+                const uint32_t at = atomicAdd(&cur[bvi[k]], 1u);
+                ids[at] = (uint32_t)((k / BNR) * SP + lid0 + (k % BNR));
+            }
+        }
+        __syncthreads();
+        // C: each wave takes WAVE_RAYS consecutive padded slots (one bucket),
+        // each lane BNR of them
+        const int total = *s_total;
+        const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+        for (int chunk0 = 0; chunk0 < total; chunk0 += SP) {
+            const int s0 = chunk0 + wave * WAVE_RAYS;       // wave-uniform
+            if (s0 >= total) continue;
+            // bucket of this wave: last b with pad[b] <= s0 and a non-empty range
+            int lo = 0, hi = nbv - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if ((int)pad[mid] <= s0) lo = mid; else hi = mid - 1;
+            }
+            while (lo > 0 && cnt[lo] == 0u) --lo;            // empty buckets share their pad offset
+            const int b = __builtin_amdgcn_readfirstlane(lo);
+            const int j0 = s0 - (int)pad[b] + lane * BNR;    // index within bucket b
+            const int nb = (int)cnt[b];
+            uint32_t lid[BNR];
+            bool valid[BNR];
+#pragma unroll
+            for (int i = 0; i < BNR; ++i) {
+                valid[i] = j0 + i < nb;
+                lid[i] = valid[i] ? ids[off[b] + j0 + i] : 0u;
+            }
+            const uint32_t l0 = lid[0], l1 = lid[1], l2 = lid[2], l3 = lid[3];
+            const auto ray_of = [=](int k) -> int64_t {
+                const uint32_t l = k == 0 ? l0 : (k == 1 ? l1 : (k == 2 ? l2 : l3));
+                return pass0 + (int64_t)l;
+            };
+            Ray32 R[BNR];
+            float tb[BNR];
+            int ib[BNR];
+#pragma unroll
+            for (int i = 0; i < BNR; ++i) {
+                float f[4], r4[4];
+                const int64_t rl = pass0 + lid[i];
+                workitem_floats(kp, gid, emitter, wi0 + (rl >> 2), f);
+                rotate_rands(f, (int)(rl & 3), r4);
+                R[i] = emit_fast(&E, r4);
+                tb[i] = finf();
+                ib[i] = -1;
+            }
+            CPTR(int32_t) sg = cst(G->bvc_seg) + b * (N_SEG + 1);
+            Pending pd{pend, 0, NO_KEY, 0, tau, eps};
+            seg_loop<SEG_CYL, BNR, REFINE, false>(FC, sg[SEG_CYL], sg[SEG_CYL + 1], R, tb, ib, pd);
+            seg_loop<SEG_FRUSTUM, BNR, REFINE, false>(FC, sg[SEG_FRUSTUM], sg[SEG_FRUSTUM + 1], R, tb, ib, pd);
+            seg_loop<SEG_SPHERE, BNR, REFINE, false>(FC, sg[SEG_SPHERE], sg[SEG_SPHERE + 1], R, tb, ib, pd);
+            seg_loop<SEG_RING, BNR, REFINE, false>(FC, sg[SEG_RING], sg[SEG_RING + 1], R, tb, ib, pd);
+            seg_loop<SEG_RECT, BNR, REFINE, false>(FC, sg[SEG_RECT], sg[SEG_RECT + 1], R, tb, ib, pd);
+            if constexpr (REFINE) {
+                if (__builtin_amdgcn_ballot_w64(pd.count > 0))
+                    resolve_pending<BNR>(kp, G, E, gid, emitter, wi0, ray_of, tb, ib, pd, nref);
+                if (__builtin_amdgcn_ballot_w64(pd.resume != NO_KEY))
+                    finish_dropped<BNR>(kp, G, E, gid, emitter, wi0, ray_of, FC, sg, sg + 1, tb, ib, pd, nref, nover);
+            }
+            const BvMeta meta = ldc(M + b);
+#pragma unroll
+            for (int i = 0; i < BNR; ++i) {
+                if (valid[i]) {
+                    tests += (uint64_t)(meta.end - meta.start);
+                    tally<TRACE, false>(kp, G, emitter, ray0 + pass0 + lid[i], ib[i], bins);
+                }
+            }
+        }
+        __syncthreads();   // buckets are rebuilt by the next pass
+    }
+}
+
+// Registers: 94 without the FP64 stage (5 waves per SIMD); with it the
+// allocator wants ~170, capped at 128 (4 waves) with spills only outside the
+// child-scan loops.
+template <bool REFINE, bool TRACE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REFINE ? 4 : 5))) k_queue_bvbin(KParams kp)
+{
+    // dynamic LDS: [bins][re-solve lists: PEND_K x blockDim][rays by bucket: blockDim x BNR x BSUB]
+    //              [bucket counts, offsets, cursors, wave-padded offsets: 4 x bv_slots]
+    extern __shared__ uint32_t smem[];
+    uint32_t *bins = smem;
+    uint32_t *pend = smem + (TRACE ? 0 : kp.bins);
+    uint32_t *ids = pend + PEND_K * blockDim.x;
+    uint32_t *cnt = ids + blockDim.x * BNR * BSUB;
+    uint32_t *off = cnt + kp.bv_slots;
+    uint32_t *cur = off + kp.bv_slots;
+    uint32_t *pad = cur + kp.bv_slots;
+    __shared__ int s_unit, s_total;
+    uint64_t tests = 0;
+    uint32_t nref = 0, nover = 0;
+    const bool lds_tally = !TRACE && kp.bins > 0;
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = (int)atomicAdd(kp.queue_head, 1u);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        if (u >= kp.n_units) break;
+        CPTR(GeomDesc) G;
+        int emitter;
+        int64_t wi0, n_wi;
+        uint32_t gid;
+        if (kp.units) {
+            const WorkUnit U = ldc(cst(kp.units) + u);
+            G = cst(kp.geoms) + U.geom;
+            emitter = U.emitter; wi0 = U.wi0; n_wi = U.n_wi;
+            gid = G->geometry_id;
+        } else {
+            G = cst(kp.geoms);
+            emitter = kp.imp_emitter;
+            wi0 = kp.imp_wi0 + (int64_t)u * kp.imp_unit;
+            const int64_t left = kp.imp_wi0 + kp.imp_n_wi - wi0;
+            n_wi = left < kp.imp_unit ? left : kp.imp_unit;
+            gid = kp.imp_geom_id;
+        }
+        const int n = G->n;
+        if (lds_tally) {
+            for (int i = threadIdx.x; i < n; i += blockDim.x) bins[i] = 0u;
+            __syncthreads();
+        }
+        run_unit_bvbin<REFINE, TRACE>(kp, G, gid, emitter, wi0, n_wi, bins, pend, ids, cnt, off, cur, pad, &s_total,
+                                      tests, nref, nover);
+        if (lds_tally) {
+            __syncthreads();
+            unsigned long long *row = G->hits + (int64_t)emitter * n;
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                const uint32_t c = bins[i];
+                if (c) atomicAdd(&row[i], (unsigned long long)c);
+            }
+        }
+        __syncthreads();
+    }
+    flush_stats(kp, tests, nref, nover);
+}
+
 // Persistent queue kernel: every workgroup pulls units until the queue is
 // empty (all waves reach the same exit).
 // Four rays per thread with the FP64 stage fits 96 VGPRs (five waves per
@@ -1119,6 +1369,18 @@ static hipError_t go_math(const LaunchCfg &c, const KParams &kp, hipStream_t s)
             } else {
                 if (c.refine) hipLaunchKernelGGL((k_queue_mfma<true, false>), dim3(c.grid), dim3(c.block), c.lds_bytes, s, kp);
                 else hipLaunchKernelGGL((k_queue_mfma<false, false>), dim3(c.grid), dim3(c.block), c.lds_bytes, s, kp);
+            }
+            return hipGetLastError();
+        }
+    }
+    if constexpr (MATH == KM_FAST) {
+        if (c.bvbin && c.use_bv && !c.direct) {
+            if (c.trace) {
+                if (c.refine) hipLaunchKernelGGL((k_queue_bvbin<true, true>), dim3(c.grid), dim3(c.block), c.lds_bytes, s, kp);
+                else hipLaunchKernelGGL((k_queue_bvbin<false, true>), dim3(c.grid), dim3(c.block), c.lds_bytes, s, kp);
+            } else {
+                if (c.refine) hipLaunchKernelGGL((k_queue_bvbin<true, false>), dim3(c.grid), dim3(c.block), c.lds_bytes, s, kp);
+                else hipLaunchKernelGGL((k_queue_bvbin<false, false>), dim3(c.grid), dim3(c.block), c.lds_bytes, s, kp);
             }
             return hipGetLastError();
         }

@@ -59,11 +59,17 @@ struct rf_scene {
     unsigned long long *d_hits = nullptr;
     float *d_quads = nullptr;            // MFMA form: primitive quads
     int32_t *d_pos2orig = nullptr, *d_orig2pos = nullptr;
+    FastPrim *d_bvc_fast = nullptr;      // binned BV form: children per BV, type-segmented
+    int32_t *d_bvc_seg = nullptr;
     GeomDesc *d_desc = nullptr;          // single-geometry descriptor
     GeomDesc desc{};                     // host copy
     int64_t flops_per_ray = 0;           // sum over primitives of F_type (plain kernels)
     double flops_per_test_avg = 0.0;     // BV accounting
 };
+
+// Binned BV form: at most this many bounding surfaces per scene (LDS buckets;
+// the kernel packs the BV index into 16 bits).
+static constexpr int kMaxBvBins = 512;
 
 static thread_local std::string g_err;
 
@@ -252,13 +258,15 @@ extern "C" int64_t rf_scene_footprint(int32_t n, int32_t nbv)
 {
     if (n < 0 || nbv < 0) return -1;
     return (int64_t)n * (sizeof(rf_primitive) + 2 * sizeof(FastPrim) + 64 + 8) + 5 * 256 +
-           (int64_t)nbv * (sizeof(rf_bounding_volume) + sizeof(FastPrim) + sizeof(BvMeta)) +
+           (int64_t)nbv * (sizeof(rf_bounding_volume) + sizeof(FastPrim) + sizeof(BvMeta) + (N_SEG + 1) * 4) +
+           (nbv ? (int64_t)n * sizeof(FastPrim) : 0) +
            (int64_t)n * n * 8 + (int64_t)sizeof(GeomDesc);
 }
 
 static void free_scene_mem(rf_scene *s)
 {
     (void)hipFree(s->d_quads); (void)hipFree(s->d_pos2orig); (void)hipFree(s->d_orig2pos);
+    (void)hipFree(s->d_bvc_fast); (void)hipFree(s->d_bvc_seg);
     (void)hipFree(s->d_prims); (void)hipFree(s->d_fast); (void)hipFree(s->d_fast_orig);
     (void)hipFree(s->d_fast_bvs); (void)hipFree(s->d_bvs); (void)hipFree(s->d_bv_meta);
     (void)hipFree(s->d_hits); (void)hipFree(s->d_desc);
@@ -343,6 +351,25 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
         }
         qseg_end[k] = (int32_t)(pos2orig.size() / 4);
     }
+    // binned BV form: the children of each BV in type-segmented order; usable
+    // when no BV asks for the intermediate check (runBVGPU's settings == 1)
+    std::vector<FastPrim> bvc;
+    std::vector<int32_t> bvc_seg;
+    bool binned = nbv >= 1 && nbv <= kMaxBvBins;
+    for (int32_t b = 0; b < nbv && binned; ++b)
+        if (bvs[b].settings == 1) binned = false;
+    if (binned) {
+        bvc_seg.resize((size_t)nbv * (N_SEG + 1));
+        for (int32_t b = 0; b < nbv; ++b) {
+            for (int k = 0; k < N_SEG; ++k) {
+                bvc_seg[(size_t)b * (N_SEG + 1) + k] = (int32_t)bvc.size();
+                for (int32_t i = bvs[b].objStartIndex; i < bvs[b].objEndIndex; ++i)
+                    if (seg_of(prims[i].type) == k)
+                        bvc.push_back(make_fast(prims[i].invm, prims[i].type, prims[i].util1, prims[i].util2, i));
+            }
+            bvc_seg[(size_t)b * (N_SEG + 1) + N_SEG] = (int32_t)bvc.size();
+        }
+    }
     s->flops_per_ray = fsum;
     s->flops_per_test_avg = (double)(fsum + fbv) / (double)(n + nbv);
 
@@ -363,6 +390,10 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     up((void **)&s->d_quads, quads.data(), quads.size() * sizeof(float));
     up((void **)&s->d_pos2orig, pos2orig.data(), pos2orig.size() * sizeof(int32_t));
     up((void **)&s->d_orig2pos, orig2pos.data(), orig2pos.size() * sizeof(int32_t));
+    if (binned) {
+        up((void **)&s->d_bvc_fast, bvc.data(), bvc.size() * sizeof(FastPrim));
+        up((void **)&s->d_bvc_seg, bvc_seg.data(), bvc_seg.size() * sizeof(int32_t));
+    }
     if (e == hipSuccess) e = hipMalloc((void **)&s->d_hits, (size_t)n * n * sizeof(unsigned long long));
     if (e == hipSuccess) e = launch_fill_u64(s->d_hits, (size_t)n * n, ctx->stream);
     GeomDesc &d = s->desc;
@@ -372,6 +403,7 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     for (int k = 0; k < N_SEG; ++k) { d.seg_begin[k] = seg_begin[k]; d.seg_end[k] = seg_end[k]; }
     d.quads = s->d_quads; d.pos2orig = s->d_pos2orig; d.orig2pos = s->d_orig2pos;
     for (int k = 0; k < N_SEG; ++k) { d.qseg_begin[k] = qseg_begin[k]; d.qseg_end[k] = qseg_end[k]; }
+    d.bvc_fast = s->d_bvc_fast; d.bvc_seg = s->d_bvc_seg; d.bv_binned = binned ? 1 : 0;
     up((void **)&s->d_desc, &d, sizeof d);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
@@ -467,6 +499,16 @@ static int emitter_budget(rf_ctx *ctx, const rf_scene *s, const rf_solve_opts *o
 // ---------------------------------------------------------------------------
 static constexpr int kPendK = 4;            // deferred FP64 re-solves per lane (PEND_K in the kernels)
 static constexpr int kMaxLdsBins = 15000;   // 32-bit bins per workgroup (< 64 KiB with the re-solve list)
+static constexpr int kBnr = 4;              // binned BV form: rays per thread (BNR in the kernels)
+static constexpr int kBsub = 4;             // binned BV form: sub-passes per bucketing (BSUB)
+static constexpr size_t kMaxLds = 64 * 1024;
+
+// Dynamic LDS of the binned BV kernel without / with tally bins.
+static size_t bvbin_lds(int block, int bv_slots, int bins)
+{
+    return (size_t)kPendK * block * 4 + (size_t)block * kBnr * kBsub * 4 + (size_t)4 * bv_slots * 4 +
+           (size_t)bins * 4;
+}
 
 static void pick_shape(const rf_ctx *ctx, const rf_solve_opts *o, int &nr, int &block, int &mfma)
 {
@@ -573,6 +615,12 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
     c.grid = (int)std::min<int64_t>(n_units, groups);
     c.lds_bytes = ((c.trace || kp.bins == 0) ? 0 : (size_t)kp.bins * sizeof(uint32_t)) +
                   (size_t)kPendK * block * sizeof(uint32_t);
+    if (c.math == KM_FAST && c.use_bv && s->desc.bv_binned) {
+        c.bvbin = 1;
+        kp.bv_slots = s->n_bvs;
+        if (bvbin_lds(block, kp.bv_slots, kp.bins) > kMaxLds) kp.bins = 0;
+        c.lds_bytes = bvbin_lds(block, kp.bv_slots, c.trace ? 0 : kp.bins);
+    }
     HIPCHK(ctx, launch_solve(c, kp, ctx->stream));
     ++*launches;
     return RF_OK;
@@ -726,6 +774,20 @@ static int solve_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, c
     c.block = block;
     c.grid = (int)std::min<int64_t>((int64_t)units.size(), groups);
     c.lds_bytes = (kp.bins ? (size_t)kp.bins * sizeof(uint32_t) : 0) + (size_t)kPendK * block * sizeof(uint32_t);
+    if (c.math == KM_FAST && use_bv) {
+        bool binned = true;
+        int slots = 0;
+        for (int g = 0; g < n_geoms; ++g) {
+            binned = binned && scenes[g]->desc.bv_binned;
+            slots = std::max(slots, scenes[g]->n_bvs);
+        }
+        if (binned) {
+            c.bvbin = 1;
+            kp.bv_slots = slots;
+            if (bvbin_lds(block, slots, kp.bins) > kMaxLds) kp.bins = 0;
+            c.lds_bytes = bvbin_lds(block, slots, kp.bins);
+        }
+    }
     HIPCHK(ctx, launch_solve(c, kp, ctx->stream));
     ++*launches;
     return RF_OK;
