@@ -131,3 +131,37 @@ def test_no_device_is_a_status(monkeypatch):
 def test_scene_footprint():
     assert rf.scene_footprint(100, 0) > 100 * 100 * 8
     assert rf.scene_footprint(-1, 0) == -1
+
+
+def test_ctypes_mirrors_header_structs(tmp_path):
+    """The ctypes structures match the C compiler's layout of include/rayfactor.h."""
+    import ctypes
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("no C compiler")
+    structs = {"rf_solve_opts": _lib.SolveOpts, "rf_timing": _lib.Timing, "rf_host_scene": _lib.HostScene}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "rayfactor.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_solve_opts_defaults():
+    import ctypes
+    o = _lib.SolveOpts()
+    rf.lib().rf_solve_opts_default(ctypes.byref(o))
+    assert (o.math, o.rng, o.use_bvs, o.budget, o.n_samples, o.form, o.refine) == (0, 0, 0, 0, 1000000, 0, 1)
+    assert (o.rays_per_thread, o.block_size, o.transform, o.refine_margin) == (0, 0, 0, 0.0)
