@@ -144,7 +144,9 @@ typedef struct rf_timing {
     int64_t flops;            /* algorithmic FP32 flops: sum over tests of F_type */
     int64_t refined;          /* (ray, primitive) pairs re-solved in FP64 */
     int64_t launches;         /* kernel launches issued */
-    int64_t unresolved;       /* flagged pairs kept at their FP32 root (re-solve list full) */
+    int64_t rescans;          /* lane re-scans after a full FP64 re-solve list (no flagged pair is left unsolved) */
+    int64_t graph_launches;   /* launches issued as a HIP graph (persistent solves repeated with the
+                                 same scenes and options: captured on the second call, replayed after) */
 } rf_timing;
 
 /* ------------------------------------------------------------------ */
@@ -226,7 +228,7 @@ int32_t rf_run_cl_gpurng(rf_ctx *ctx, const rf_primitive *prims, int32_t n_prims
 
 /* Solves n_geoms independent scenes.  RF_FORM_PERSISTENT runs all their
  * (geometry, emitter, ray-chunk) work units in one persistent launch fed by
- * a device work queue (captured in a HIP graph); RF_FORM_PER_EMITTER /
+ * a device work queue (a HIP graph from the second identical call on); RF_FORM_PER_EMITTER /
  * RF_FORM_REFERENCE issue one launch per emitter of each geometry.
  * hits[g] is a caller-owned n_g*n_g int64 matrix (may be NULL to skip the
  * readback); rays[g] (may be NULL) receives n_g int64.  With RF_RNG_SEEDED,

@@ -57,7 +57,8 @@ class SolveOpts(ctypes.Structure):
 class Timing(ctypes.Structure):
     _fields_ = [
         ("kernel_ms", c_double), ("wall_ms", c_double), ("rays", c_int64), ("tests", c_int64),
-        ("flops", c_int64), ("refined", c_int64), ("launches", c_int64), ("unresolved", c_int64),
+        ("flops", c_int64), ("refined", c_int64), ("launches", c_int64), ("rescans", c_int64),
+        ("graph_launches", c_int64),
     ]
 
     def as_dict(self) -> dict:

@@ -657,7 +657,7 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
 }
 
 // stats[0]: tests counted on the device (BV forms); stats[1]: pairs re-solved
-// in FP64; stats[2]: flagged pairs left at their FP32 root (full list).
+// in FP64; stats[2]: lane re-scans after a full re-solve list.
 __device__ __forceinline__ void flush_stats(const KParams &kp, uint64_t tests, uint32_t nref, uint32_t nover)
 {
     __shared__ unsigned long long s_stats[3];

@@ -38,10 +38,15 @@ struct rf_ctx {
     WorkUnit *d_units = nullptr; size_t units_cap = 0;
     unsigned *d_queue = nullptr; size_t queue_cap = 0; size_t queue_next = 0;
     unsigned long long *d_stats = nullptr;
-    // cached HIP graph of the last persistent batched solve
+    // Persistent solves repeated with the same scenes and options replay a
+    // HIP graph of [zero fills, queue kernel]: the first call runs eagerly, the
+    // second captures, later ones replay.  d_geoms / d_units hold that
+    // solve's descriptors and units while graph_key is set.
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    std::vector<uintptr_t> graph_key;
+    std::vector<uint64_t> graph_key;     // solve captured in graph_exec
+    std::vector<uint64_t> last_key;      // last persistent solve run eagerly
+    int64_t graph_replays = 0;
     // live objects owned by the context (released by rf_ctx_destroy)
     std::set<rf_scene *> scenes;
     std::set<rf_stream *> streams;
@@ -49,6 +54,7 @@ struct rf_ctx {
 
 struct rf_scene {
     rf_ctx *ctx = nullptr;
+    uint64_t serial = 0;                 // unique per scene object (graph keys)
     int n = 0, n_bvs = 0;
     std::vector<rf_primitive> prims;
     std::vector<rf_bounding_volume> bvs;
@@ -70,6 +76,8 @@ struct rf_scene {
 // Binned BV form: at most this many bounding surfaces per scene (LDS buckets;
 // the kernel packs the BV index into 16 bits).
 static constexpr int kMaxBvBins = 512;
+
+static void drop_graph(rf_ctx *ctx);
 
 static thread_local std::string g_err;
 
@@ -157,8 +165,7 @@ extern "C" int32_t rf_ctx_destroy(rf_ctx *c)
     // streams own scenes of their own; release them first, then remaining scenes
     while (!c->streams.empty()) rf_stream_destroy(*c->streams.begin());
     while (!c->scenes.empty()) rf_scene_destroy(*c->scenes.begin());
-    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
+    drop_graph(c);
     (void)hipFree(c->d_geoms);
     (void)hipFree(c->d_units);
     (void)hipFree(c->d_queue);
@@ -299,7 +306,9 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     int rc = validate_scene(prims, n, bvs, nbv, why);
     if (rc) return set_err(ctx, rc, why);
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    static uint64_t next_serial = 1;
     rf_scene *s = new rf_scene;
+    s->serial = next_serial++;
     s->ctx = ctx; s->n = n; s->n_bvs = nbv;
     s->prims.assign(prims, prims + n);
     if (nbv) s->bvs.assign(bvs, bvs + nbv);
@@ -538,13 +547,21 @@ static int ensure_queue(rf_ctx *ctx, size_t need)
     return RF_OK;
 }
 
-// One queue counter, zeroed on the stream right before the launch that uses it.
-static int next_counter(rf_ctx *ctx, unsigned **slot)
+// One queue counter of the ring (not yet zeroed).
+static int take_counter(rf_ctx *ctx, unsigned **slot)
 {
     int rc = ensure_queue(ctx, 4096);
     if (rc) return rc;
     if (ctx->queue_next >= ctx->queue_cap) ctx->queue_next = 0;
     *slot = ctx->d_queue + ctx->queue_next++;
+    return RF_OK;
+}
+
+// One queue counter, zeroed on the stream right before the launch that uses it.
+static int next_counter(rf_ctx *ctx, unsigned **slot)
+{
+    int rc = take_counter(ctx, slot);
+    if (rc) return rc;
     HIPCHK(ctx, launch_fill_u32(*slot, 0u, 1, ctx->stream));
     return RF_OK;
 }
@@ -698,9 +715,12 @@ static void account(const rf_scene *s, const rf_solve_opts *o, const std::vector
 
 // Persistent batched solve over several scenes (and the single-scene
 // persistent form).
-static int solve_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const rf_solve_opts *o,
-                            const std::vector<std::vector<int64_t>> &budgets, const std::vector<uint32_t> &ids,
-                            int64_t *launches)
+// Units, descriptors and launch configuration of a persistent solve; the
+// descriptors and units are uploaded to d_geoms / d_units (stream order), the
+// queue counter is chosen but not zeroed (enqueue_persistent does that).
+static int prepare_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const rf_solve_opts *o,
+                              const std::vector<std::vector<int64_t>> &budgets, const std::vector<uint32_t> &ids,
+                              LaunchCfg &c, KParams &kp, bool &empty)
 {
     int nr, block, mfma;
     pick_shape(ctx, o, nr, block, mfma);
@@ -735,7 +755,8 @@ static int solve_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, c
             for (int64_t w0 = 0; w0 < nw; w0 += size)
                 units.push_back(WorkUnit{g, e, w0, std::min(size, nw - w0), 0});
         }
-    if (units.empty()) return RF_OK;
+    empty = units.empty();
+    if (empty) return RF_OK;
     if (units.size() > (size_t)INT32_MAX) return set_err(ctx, RF_ERR_CAPACITY, "too many work units");
     // largest units first: cost ~ work items x primitives
     std::stable_sort(units.begin(), units.end(), [&](const WorkUnit &a, const WorkUnit &b) {
@@ -751,9 +772,11 @@ static int solve_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, c
         HIPCHK(ctx, hipMalloc((void **)&ctx->d_units, units.size() * sizeof(WorkUnit)));
         ctx->units_cap = units.size();
     }
+    // the host vectors die with this call: copy synchronously (before any launch)
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_geoms, descs.data(), n_geoms * sizeof(GeomDesc), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_units, units.data(), units.size() * sizeof(WorkUnit), hipMemcpyHostToDevice, ctx->stream));
-    KParams kp{};
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    kp = KParams{};
     kp.geoms = ctx->d_geoms;
     kp.units = ctx->d_units;
     kp.stats = ctx->d_stats;
@@ -763,9 +786,9 @@ static int solve_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, c
     kp.refine = o->refine;
     kp.tau = o->refine_margin > 0.0f ? o->refine_margin : rfd::TAU_DEFAULT;
     kp.bins = max_n <= kMaxLdsBins ? max_n : 0;
-    int rc = next_counter(ctx, &kp.queue_head);
+    int rc = take_counter(ctx, &kp.queue_head);
     if (rc) return rc;
-    LaunchCfg c{};
+    c = LaunchCfg{};
     c.math = o->math == RF_MATH_COMPAT ? KM_COMPAT : (o->math == RF_MATH_FP64 ? KM_FP64 : KM_FAST);
     c.use_bv = use_bv;
     c.nr = nr;
@@ -788,8 +811,109 @@ static int solve_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, c
             c.lds_bytes = bvbin_lds(block, slots, kp.bins);
         }
     }
-    HIPCHK(ctx, launch_solve(c, kp, ctx->stream));
-    ++*launches;
+    return RF_OK;
+}
+
+// Zero fills of the tallies, statistics and queue counter, then the launch.
+// All kernels (no hipMemsetAsync), so the sequence can be captured in a graph.
+static hipError_t enqueue_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const LaunchCfg &c,
+                                     const KParams &kp)
+{
+    hipError_t e = hipSuccess;
+    for (int g = 0; g < n_geoms && e == hipSuccess; ++g)
+        e = launch_fill_u64(scenes[g]->d_hits, (size_t)scenes[g]->n * scenes[g]->n, ctx->stream);
+    if (e == hipSuccess) e = launch_fill_u64(ctx->d_stats, 3, ctx->stream);
+    if (e == hipSuccess && kp.queue_head) e = launch_fill_u32(kp.queue_head, 0u, 1, ctx->stream);
+    if (e == hipSuccess && kp.n_units > 0) e = launch_solve(c, kp, ctx->stream);
+    return e;
+}
+
+// Identity of a persistent solve for the graph cache: scene objects, options,
+// launch shape choices, geometry ids and ray budgets.
+static std::vector<uint64_t> persistent_key(const rf_ctx *ctx, rf_scene *const *scenes, int n_geoms,
+                                            const rf_solve_opts *o, const std::vector<uint32_t> &ids,
+                                            const std::vector<std::vector<int64_t>> &budgets)
+{
+    std::vector<uint64_t> k;
+    k.push_back((uint64_t)n_geoms);
+    unsigned char ob[sizeof(rf_solve_opts)];
+    std::memcpy(ob, o, sizeof ob);
+    for (size_t i = 0; i < sizeof ob; i += 8) {
+        uint64_t w = 0;
+        std::memcpy(&w, ob + i, std::min<size_t>(8, sizeof ob - i));
+        k.push_back(w);
+    }
+    k.push_back(((uint64_t)ctx->tuned_rpt << 40) ^ ((uint64_t)ctx->tuned_block << 20) ^ (uint64_t)ctx->tuned_transform);
+    for (int g = 0; g < n_geoms; ++g) {
+        k.push_back(scenes[g]->serial);
+        k.push_back(ids[g]);
+        uint64_t h = 1469598103934665603ull;   // FNV-1a over the per-emitter budgets
+        for (int64_t w : budgets[g]) { h ^= (uint64_t)w; h *= 1099511628211ull; }
+        k.push_back(h);
+        k.push_back(budgets[g].size());
+    }
+    return k;
+}
+
+static void drop_graph(rf_ctx *ctx)
+{
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+    ctx->graph_key.clear();
+}
+
+// Persistent solve between the timing events: replay the cached graph when
+// the same solve ran before; otherwise prepare, and run eagerly (first time)
+// or capture and launch a graph (second time).
+static int solve_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const rf_solve_opts *o,
+                            const std::vector<std::vector<int64_t>> &budgets, const std::vector<uint32_t> &ids,
+                            int64_t *launches, int64_t *graphed)
+{
+    const std::vector<uint64_t> key = persistent_key(ctx, scenes, n_geoms, o, ids, budgets);
+    if (ctx->graph_exec && key == ctx->graph_key) {
+        HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        HIPCHK(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+        ++ctx->graph_replays;
+        ++*launches;
+        ++*graphed;
+        return RF_OK;
+    }
+    // d_geoms / d_units are about to change: the cached graph no longer applies
+    drop_graph(ctx);
+    LaunchCfg c{};
+    KParams kp{};
+    bool empty = false;
+    int rc = prepare_persistent(ctx, scenes, n_geoms, o, budgets, ids, c, kp, empty);
+    if (rc) return rc;
+    if (!empty && key == ctx->last_key) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t e = enqueue_persistent(ctx, scenes, n_geoms, c, kp);
+        const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return set_err(ctx, RF_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        const hipError_t e3 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e3 != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return set_err(ctx, RF_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e3));
+        }
+        ctx->graph = g;
+        ctx->graph_exec = ge;
+        ctx->graph_key = key;
+        HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        HIPCHK(ctx, hipGraphLaunch(ge, ctx->stream));
+        ++*graphed;
+    } else {
+        ctx->last_key = key;
+        HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        HIPCHK(ctx, enqueue_persistent(ctx, scenes, n_geoms, c, kp));
+    }
+    if (!empty) ++*launches;
     return RF_OK;
 }
 
@@ -811,17 +935,18 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
         rc = emitter_budget(ctx, scenes[g], o, budgets[g], geom_rays ? geom_rays[g] : -1);
         if (rc) return rc;
     }
-    // hit matrices and statistics start at zero
-    for (int g = 0; g < n_geoms; ++g)
-        HIPCHK(ctx, launch_fill_u64(scenes[g]->d_hits, (size_t)scenes[g]->n * scenes[g]->n, ctx->stream));
-    HIPCHK(ctx, launch_fill_u64(ctx->d_stats, 3, ctx->stream));
     rc = ensure_queue(ctx, 4096);
     if (rc) return rc;
-    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    int64_t launches = 0;
+    int64_t launches = 0, graphed = 0;
     if (o->form == RF_FORM_PERSISTENT) {
-        rc = solve_persistent(ctx, scenes, n_geoms, o, budgets, ids, &launches);
+        // zero fills are part of the (graph-captured) persistent sequence
+        rc = solve_persistent(ctx, scenes, n_geoms, o, budgets, ids, &launches, &graphed);
     } else {
+        // hit matrices and statistics start at zero
+        for (int g = 0; g < n_geoms; ++g)
+            HIPCHK(ctx, launch_fill_u64(scenes[g]->d_hits, (size_t)scenes[g]->n * scenes[g]->n, ctx->stream));
+        HIPCHK(ctx, launch_fill_u64(ctx->d_stats, 3, ctx->stream));
+        HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         for (int g = 0; g < n_geoms && !rc; ++g)
             for (int e = 0; e < scenes[g]->n && !rc; ++e)
                 rc = launch_range(ctx, scenes[g], e, 0, budgets[g][e], o, ids[g], nullptr, 0, &launches);
@@ -837,8 +962,9 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
     rf_timing t{};
     t.kernel_ms = ms;
     t.launches = launches;
+    t.graph_launches = graphed;
     t.refined = (int64_t)st[1];
-    t.unresolved = (int64_t)st[2];
+    t.rescans = (int64_t)st[2];
     for (int g = 0; g < n_geoms; ++g) {
         account(scenes[g], o, budgets[g], t);
         if (o->use_bvs && scenes[g]->n_bvs > 0) {

@@ -56,3 +56,31 @@ def test_closure_of_enclosure_rows(ctx, sweep8):
     for r in res:
         frac = r.hits.sum(axis=1) / r.rays
         assert np.all(frac <= 1.0) and np.median(frac) > 0.8
+
+
+def test_graph_replays_match_eager(ctx, sweep8):
+    """A persistent solve repeated with the same scenes and options is captured
+    in a HIP graph on the second call and replayed afterwards.  Every replay,
+    with other eager work and synchronisations in between, matches the eager
+    result; a different solve in between drops the graph."""
+    geoms, dev = sweep8
+    sub = dev[:4]
+    o = rf.make_opts(math="fast", rng="seeded", seed=11, rays_per_emitter=3000, form="persistent")
+    first, t0 = rf.batch_solve(ctx, sub, o)
+    assert t0["graph_launches"] == 0
+    for k in range(4):
+        res, t = rf.batch_solve(ctx, sub, o)
+        assert t["graph_launches"] == 1 and t["launches"] == 1
+        for a, b in zip(first, res):
+            assert np.array_equal(a.hits, b.hits), k
+        # eager work between replays
+        rf.trace(ctx, sub[0], 0, 0, 256, math="fast", rng="seeded")
+        rf.solve(ctx, sub[1], math="compat", rays_per_emitter=512, form="per_emitter")
+        ctx.sync()
+    other, t2 = rf.batch_solve(ctx, sub, rf.make_opts(math="fast", rng="seeded", seed=12, rays_per_emitter=3000,
+                                                      form="persistent"))
+    assert t2["graph_launches"] == 0
+    again, t3 = rf.batch_solve(ctx, sub, o)
+    assert t3["graph_launches"] == 0
+    for a, b in zip(first, again):
+        assert np.array_equal(a.hits, b.hits)

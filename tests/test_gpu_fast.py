@@ -165,8 +165,8 @@ def test_wide_margin_forms_and_shapes_identical(ctx, form, rpt):
     assert got.timing["refined"] == base.timing["refined"]
     if rpt == 4 and form == "persistent":
         # lists overflowed and the re-scan path ran
-        assert base.timing["unresolved"] > 0
-        print(f"[wide] refined {base.timing['refined']}, re-scans {base.timing['unresolved']}", flush=True)
+        assert base.timing["rescans"] > 0
+        print(f"[wide] refined {base.timing['refined']}, re-scans {base.timing['rescans']}", flush=True)
 
 
 @pytest.mark.parametrize("transform", ["valu", "mfma"])
@@ -183,10 +183,10 @@ def test_wide_margin_precision(ctx, transform):
     dvf = np.abs(a.hits - b.hits).max() / rays
     print(f"[wide] {transform} max |dVF| = {dvf:.2e} (default margin "
           f"{np.abs(d.hits - b.hits).max() / rays:.2e}), refined {a.timing['refined']} vs {d.timing['refined']}, "
-          f"re-scans {a.timing['unresolved']}", flush=True)
+          f"re-scans {a.timing['rescans']}", flush=True)
     assert dvf <= VF_TOL
     assert a.timing["refined"] > 10 * d.timing["refined"]
-    assert a.timing["unresolved"] > 0
+    assert a.timing["rescans"] > 0
     c = rf.solve(ctx, dev, form="per_emitter", refine_margin=WIDE, **kw)
     assert np.array_equal(a.hits, c.hits)
 
