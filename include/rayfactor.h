@@ -132,7 +132,8 @@ typedef enum rf_transform {
 } rf_transform;
 
 /* Fills *opts with defaults: COMPAT math, reference RNG and budget,
- * n_samples = 1000000, per-emitter form, refine = 1. */
+ * n_samples = 1000000, persistent form (same tallies as one launch per
+ * emitter, one launch for all emitters), refine = 1. */
 void rf_solve_opts_default(rf_solve_opts *opts);
 
 /* Measurements of the last solve on a context. */

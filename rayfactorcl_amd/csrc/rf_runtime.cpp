@@ -122,7 +122,7 @@ extern "C" void rf_solve_opts_default(rf_solve_opts *o)
     o->use_bvs = 0;
     o->budget = RF_BUDGET_REFERENCE;
     o->n_samples = 1000000;
-    o->form = RF_FORM_PER_EMITTER;
+    o->form = RF_FORM_PERSISTENT;
     o->refine = 1;
 }
 

@@ -22,7 +22,7 @@ from ._lib import BV_DTYPE, FORM, MATH, PRIM_DTYPE, RNG, TRANSFORM, SolveOpts, T
 
 def make_opts(math: str = "compat", rng: str = "reference", use_bvs: bool = False,
               n_samples: int | None = None, rays_per_emitter: int | None = None,
-              form: str = "per_emitter", refine: bool = True, seed: int = 0, geometry: int = 0,
+              form: str = "persistent", refine: bool = True, seed: int = 0, geometry: int = 0,
               rays_per_thread: int = 0, block_size: int = 0, transform: str = "auto",
               refine_margin: float = 0.0) -> SolveOpts:
     """Solve options.  The ray budget is the reference's launch geometry

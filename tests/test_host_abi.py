@@ -163,5 +163,5 @@ def test_solve_opts_defaults():
     import ctypes
     o = _lib.SolveOpts()
     rf.lib().rf_solve_opts_default(ctypes.byref(o))
-    assert (o.math, o.rng, o.use_bvs, o.budget, o.n_samples, o.form, o.refine) == (0, 0, 0, 0, 1000000, 0, 1)
+    assert (o.math, o.rng, o.use_bvs, o.budget, o.n_samples, o.form, o.refine) == (0, 0, 0, 0, 1000000, 1, 1)
     assert (o.rays_per_thread, o.block_size, o.transform, o.refine_margin) == (0, 0, 0, 0.0)
