@@ -160,49 +160,45 @@ __device__ __forceinline__ float roots_select_c(float A, float B, float C, const
 }
 
 // One test of findObject's switch (RayFactorGPU.cl:452-461), hit tests :126-291.
-__device__ __forceinline__ float hit_c(int type, const float *__restrict__ m, float u1, float u2,
-                                       const Ray32 &w)
+// One type of the compat hit test (operation for operation as the cited
+// reference lines); hit_c dispatches on the type.
+template <int TYPE>
+__device__ __forceinline__ float hit_c_t(const float *__restrict__ m, float u1, float u2, const Ray32 &w)
 {
     const Ray32 o = xfrm_c(m, w);
-    switch (type) {
-    case RF_RECTANGLE: {                                               // :126-145
+    if constexpr (TYPE == RF_RECTANGLE) {                              // :126-145
         const float th = -o.sz * recip_c(o.cz);
         const float hx = __builtin_fabsf(o.sx + o.cx * th);
         const float hy = __builtin_fabsf(o.sy + o.cy * th);
         if (hx > 1.f || hy > 1.f || th < EPS_F) return finf();
         return th;
-    }
-    case RF_DISC: {                                                    // :147-166
+    } else if constexpr (TYPE == RF_DISC) {                            // :147-166
         const float th = -o.sz * recip_c(o.cz);
         const float hx = o.sx + o.cx * th;
         const float hy = o.sy + o.cy * th;
         const float e = hx * hx + hy * hy;
         if (e <= 1.f && th > EPS_F) return th;
         return finf();
-    }
-    case RF_ANNULUS: {                                                 // :168-187
+    } else if constexpr (TYPE == RF_ANNULUS) {                         // :168-187
         const float th = -o.sz * recip_c(o.cz);
         const float hx = o.sx + o.cx * th;
         const float hy = o.sy + o.cy * th;
         const float e = __builtin_fmaf(hx, hx, hy * hy);
         if (e >= u1 && e <= u2 && th > EPS_F) return th;
         return finf();
-    }
-    case RF_CYLINDER: {                                                // :189-222
+    } else if constexpr (TYPE == RF_CYLINDER) {                        // :189-222
         const float A = __builtin_fmaf(o.cx, o.cx, o.cy * o.cy);
         const float B = -__builtin_fmaf(o.sx, o.cx, o.sy * o.cy);
         const float C = __builtin_fmaf(o.sx, o.sx, __builtin_fmaf(o.sy, o.sy, -1.f));
         return roots_select_c(A, B, C, o);
-    }
-    case RF_FRUSTUM: {                                                 // :224-260
+    } else if constexpr (TYPE == RF_FRUSTUM) {                         // :224-260
         const float fDir = (u1 - 1.0f) * o.cz;
         const float fStart = (u1 - 1.0f) * o.sz + 1.0f;
         const float A = __builtin_fmaf(o.cx, o.cx, __builtin_fmaf(o.cy, o.cy, (-fDir) * fDir));
         const float B = -__builtin_fmaf(o.sx, o.cx, __builtin_fmaf(o.sy, o.cy, (-fStart) * fDir));
         const float C = __builtin_fmaf(o.sx, o.sx, __builtin_fmaf(o.sy, o.sy, (-fStart) * fStart));
         return roots_select_c(A, B, C, o);
-    }
-    case RF_SPHERE: {                                                  // :262-291 (no sqrt, as the reference)
+    } else {                                                           // :262-291 (no sqrt, as the reference)
         const float A = o.cx * o.cx + o.cy * o.cy + o.cz * o.cz;
         const float B = o.cx * o.sx + o.cy * o.sy + o.cz * o.sz;
         const float C = o.sx * o.sx + o.sy * o.sy + o.sz * o.sz - 1.0f;
@@ -216,8 +212,19 @@ __device__ __forceinline__ float hit_c(int type, const float *__restrict__ m, fl
         }
         return finf();
     }
-    default:
-        return finf();
+}
+
+__device__ __forceinline__ float hit_c(int type, const float *__restrict__ m, float u1, float u2,
+                                       const Ray32 &w)
+{
+    switch (type) {
+    case RF_RECTANGLE: return hit_c_t<RF_RECTANGLE>(m, u1, u2, w);
+    case RF_DISC: return hit_c_t<RF_DISC>(m, u1, u2, w);
+    case RF_ANNULUS: return hit_c_t<RF_ANNULUS>(m, u1, u2, w);
+    case RF_CYLINDER: return hit_c_t<RF_CYLINDER>(m, u1, u2, w);
+    case RF_FRUSTUM: return hit_c_t<RF_FRUSTUM>(m, u1, u2, w);
+    case RF_SPHERE: return hit_c_t<RF_SPHERE>(m, u1, u2, w);
+    default: return finf();
     }
 }
 

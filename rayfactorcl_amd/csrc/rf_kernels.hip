@@ -161,6 +161,63 @@ __device__ __forceinline__ FastPrim vcopy(const FastPrim &q)
     return v;
 }
 
+// Compat scan of the NR rays of one work item: each record is read once,
+// copied to VGPRs and tested against every ray with the reference's
+// arithmetic (hit_c_t); caller order and strict < keep the reference's tie
+// rule (RayFactorGPU.cl:445-472), so each ray's result is find_c's.
+template <int NR>
+__device__ __forceinline__ void find_c_multi(CPTR(rf_primitive) P, int start, int end, const Ray32 (&ray)[NR],
+                                             int (&obj)[NR])
+{
+    float tbest[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) { tbest[r] = finf(); obj[r] = -1; }
+    for (int w = start; w < end; ++w) {
+        const rf_primitive p = ldc(P + w);
+        float m[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) m[k] = vcopy(p.invm[k]);
+        float t[NR];
+        switch (p.type) {
+        case RF_CYLINDER:
+#pragma unroll
+            for (int r = 0; r < NR; ++r) t[r] = hit_c_t<RF_CYLINDER>(m, 0.f, 0.f, ray[r]);
+            break;
+        case RF_FRUSTUM: {
+            const float u1 = vcopy(p.util1);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) t[r] = hit_c_t<RF_FRUSTUM>(m, u1, 0.f, ray[r]);
+            break;
+        }
+        case RF_SPHERE:
+#pragma unroll
+            for (int r = 0; r < NR; ++r) t[r] = hit_c_t<RF_SPHERE>(m, 0.f, 0.f, ray[r]);
+            break;
+        case RF_ANNULUS: {
+            const float u1 = vcopy(p.util1), u2 = vcopy(p.util2);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) t[r] = hit_c_t<RF_ANNULUS>(m, u1, u2, ray[r]);
+            break;
+        }
+        case RF_RECTANGLE:
+#pragma unroll
+            for (int r = 0; r < NR; ++r) t[r] = hit_c_t<RF_RECTANGLE>(m, 0.f, 0.f, ray[r]);
+            break;
+        case RF_DISC:
+#pragma unroll
+            for (int r = 0; r < NR; ++r) t[r] = hit_c_t<RF_DISC>(m, 0.f, 0.f, ray[r]);
+            break;
+        default:
+#pragma unroll
+            for (int r = 0; r < NR; ++r) t[r] = finf();
+            break;
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            if (t[r] < tbest[r]) { tbest[r] = t[r]; obj[r] = w; }
+    }
+}
+
 #define ROW_S(P, i, R) __builtin_fmaf(P.r[4 * i], R.sx, __builtin_fmaf(P.r[4 * i + 1], R.sy, __builtin_fmaf(P.r[4 * i + 2], R.sz, P.r[4 * i + 3])))
 #define ROW_C(P, i, R) __builtin_fmaf(P.r[4 * i], R.cx, __builtin_fmaf(P.r[4 * i + 1], R.cy, P.r[4 * i + 2] * R.cz))
 
@@ -611,10 +668,25 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                 if (base + i < n_rays) tally<TRACE, DIRECT>(kp, G, emitter, ray0 + base + i, ib[i], bins);
         }
     } else {
-        // one work item (4 rays, one at a time) per thread and step
+        // one work item (4 rays) per thread and step
         for (int64_t wl = first; wl < n_wi; wl += stride) {
             float f[4];
             workitem_floats(kp, gid, emitter, wi0 + wl, f);
+            if constexpr (MATH == KM_COMPAT && !BV) {
+                // the work item's 4 rays share each primitive visit
+                Ray32 R4[4];
+                int idx4[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    float r4[4];
+                    rotate_rands(f, v, r4);
+                    R4[v] = emit_c(&E, r4);
+                }
+                find_c_multi<4>(cst(G->prims), 0, G->n, R4, idx4);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) tally<TRACE, DIRECT>(kp, G, emitter, ray0 + wl * 4 + v, idx4[v], bins);
+                continue;
+            }
             for (int v = 0; v < 4; ++v) {
                 float r4[4];
                 rotate_rands(f, v, r4);

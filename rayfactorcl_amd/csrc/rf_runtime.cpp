@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -306,9 +307,9 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     int rc = validate_scene(prims, n, bvs, nbv, why);
     if (rc) return set_err(ctx, rc, why);
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    static uint64_t next_serial = 1;
+    static std::atomic<uint64_t> next_serial{1};
     rf_scene *s = new rf_scene;
-    s->serial = next_serial++;
+    s->serial = next_serial.fetch_add(1);
     s->ctx = ctx; s->n = n; s->n_bvs = nbv;
     s->prims.assign(prims, prims + n);
     if (nbv) s->bvs.assign(bvs, bvs + nbv);
@@ -620,9 +621,11 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
     const int64_t pass_wi = std::max<int64_t>(1, (int64_t)block * (nr >= 4 ? nr / 4 : 1));
     const int groups = max_groups(ctx, block);
     int64_t unit = (n_wi + groups - 1) / groups;
+    unit = std::min<int64_t>(unit, (int64_t)1 << 28);   // 32-bit LDS bins: < 2^30 rays per unit
     unit = ((unit + pass_wi - 1) / pass_wi) * pass_wi;
     unit = std::max(unit, pass_wi);
     const int64_t n_units = (n_wi + unit - 1) / unit;
+    if (n_units > INT32_MAX) return set_err(ctx, RF_ERR_CAPACITY, "ray budget too large for one launch");
     kp.imp_unit = unit;
     kp.n_units = (int32_t)n_units;
     int rc = next_counter(ctx, &kp.queue_head);
