@@ -8,12 +8,17 @@
 // atomic each.  Scene records are read through the constant address space
 // at wave-uniform indices, so they arrive in SGPRs (s_load) and every lane
 // of a wave walks the same primitive at the same time: the type dispatch
-// never diverges.
+// never diverges.  The hot loops copy each record to VGPRs once (an SGPR
+// operand would keep the arithmetic from dual-issuing).
 //
-//   compat  FP32 reference semantics, caller's primitive order
+//   compat  FP32 reference semantics, caller's primitive order, the 4 rays of
+//           a work item per primitive visit
 //   fp64    FP64 physics semantics, caller's primitive order
 //   fast    FP32 with hardware rcp/sqrt over type-segmented primitive
-//           records, NR rays per thread, FP64 re-solve of flagged roots
+//           records, NR rays per thread, FP64 re-solve of flagged roots;
+//           variants with the ray transform on the matrix cores
+//           (k_queue_mfma) and with rays binned by bounding surface
+//           (k_queue_bvbin)
 //
 // "direct" launches keep the reference's dispatch shape instead: one work
 // item per thread, 128-wide groups, one global atomic per hitting ray.
