@@ -25,6 +25,9 @@ using rfd::FastPrim;
 // ---------------------------------------------------------------------------
 // Context and scene objects
 // ---------------------------------------------------------------------------
+static constexpr int kFan = 4;          // lanes of concurrent per-emitter launches
+static constexpr size_t kRing = 4096;   // queue counters per lane
+
 struct rf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -37,7 +40,18 @@ struct rf_ctx {
     // scratch device buffers
     GeomDesc *d_geoms = nullptr; size_t geoms_cap = 0;
     WorkUnit *d_units = nullptr; size_t units_cap = 0;
-    unsigned *d_queue = nullptr; size_t queue_cap = 0; size_t queue_next = 0;
+    // queue counters: one ring of kRing per fan lane (a lane reuses its own
+    // slots only, after its earlier launches in stream order)
+    unsigned *d_queue = nullptr;
+    size_t queue_next[kFan] = {};
+    // Fan-out of independent per-emitter launches over kFan lanes: lane 0 is
+    // `stream`, the others extra streams (default priority).  They fork from
+    // `stream` and join back into it before anything reads results.
+    hipStream_t fan[kFan] = {};
+    hipEvent_t fan_ev[kFan] = {};
+    hipEvent_t fork_ev = nullptr;
+    bool fan_open = false;
+    int fan_next = 0;
     unsigned long long *d_stats = nullptr;
     // Persistent solves repeated with the same scenes and options replay a
     // HIP graph of [zero fills, queue kernel]: the first call runs eagerly, the
@@ -79,6 +93,7 @@ struct rf_scene {
 static constexpr int kMaxBvBins = 512;
 
 static void drop_graph(rf_ctx *ctx);
+static int fan_join(rf_ctx *ctx);
 
 static thread_local std::string g_err;
 
@@ -143,6 +158,11 @@ extern "C" int32_t rf_ctx_create(int32_t dev, rf_ctx **out)
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming);
+    for (int i = 1; i < kFan && e == hipSuccess; ++i) {
+        e = hipStreamCreateWithFlags(&c->fan[i], hipStreamNonBlocking);   // default priority
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fan_ev[i], hipEventDisableTiming);
+    }
     hipDeviceProp_t prop;
     if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
     if (e == hipSuccess) e = hipMalloc((void **)&c->d_stats, 4 * sizeof(unsigned long long));
@@ -162,7 +182,10 @@ extern "C" int32_t rf_ctx_destroy(rf_ctx *c)
 {
     if (!c) return RF_ERR_INVALID_ARG;
     (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream) {
+        (void)fan_join(c);
+        (void)hipStreamSynchronize(c->stream);
+    }
     // streams own scenes of their own; release them first, then remaining scenes
     while (!c->streams.empty()) rf_stream_destroy(*c->streams.begin());
     while (!c->scenes.empty()) rf_scene_destroy(*c->scenes.begin());
@@ -173,6 +196,11 @@ extern "C" int32_t rf_ctx_destroy(rf_ctx *c)
     (void)hipFree(c->d_stats);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+    for (int i = 1; i < kFan; ++i) {
+        if (c->fan_ev[i]) (void)hipEventDestroy(c->fan_ev[i]);
+        if (c->fan[i]) (void)hipStreamDestroy(c->fan[i]);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return RF_OK;
@@ -432,6 +460,7 @@ extern "C" int32_t rf_scene_destroy(rf_scene *s)
 {
     if (!s) return RF_ERR_INVALID_ARG;
     (void)hipSetDevice(s->ctx->device);
+    (void)fan_join(s->ctx);                      // launches on side lanes may still read the scene
     (void)hipStreamSynchronize(s->ctx->stream);
     s->ctx->scenes.erase(s);
     free_scene_mem(s);
@@ -536,35 +565,61 @@ static void pick_shape(const rf_ctx *ctx, const rf_solve_opts *o, int &nr, int &
     }
 }
 
-static int ensure_queue(rf_ctx *ctx, size_t need)
+static int ensure_queue(rf_ctx *ctx)
 {
-    if (ctx->queue_cap >= need) return RF_OK;
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(ctx->d_queue);
-    size_t cap = std::max<size_t>(need, 4096);
-    HIPCHK(ctx, hipMalloc((void **)&ctx->d_queue, cap * sizeof(unsigned)));
-    ctx->queue_cap = cap;
-    ctx->queue_next = 0;
+    if (ctx->d_queue) return RF_OK;
+    HIPCHK(ctx, hipMalloc((void **)&ctx->d_queue, kFan * kRing * sizeof(unsigned)));
     return RF_OK;
 }
 
-// One queue counter of the ring (not yet zeroed).
-static int take_counter(rf_ctx *ctx, unsigned **slot)
+static hipStream_t lane_stream(const rf_ctx *ctx, int lane) { return lane == 0 ? ctx->stream : ctx->fan[lane]; }
+
+// One queue counter of a lane's ring (not yet zeroed).
+static int take_counter(rf_ctx *ctx, unsigned **slot, int lane = 0)
 {
-    int rc = ensure_queue(ctx, 4096);
+    int rc = ensure_queue(ctx);
     if (rc) return rc;
-    if (ctx->queue_next >= ctx->queue_cap) ctx->queue_next = 0;
-    *slot = ctx->d_queue + ctx->queue_next++;
+    *slot = ctx->d_queue + (size_t)lane * kRing + (ctx->queue_next[lane]++ % kRing);
     return RF_OK;
 }
 
-// One queue counter, zeroed on the stream right before the launch that uses it.
-static int next_counter(rf_ctx *ctx, unsigned **slot)
+// One queue counter, zeroed on the lane's stream right before the launch that uses it.
+static int next_counter(rf_ctx *ctx, unsigned **slot, int lane)
 {
-    int rc = take_counter(ctx, slot);
+    int rc = take_counter(ctx, slot, lane);
     if (rc) return rc;
-    HIPCHK(ctx, launch_fill_u32(*slot, 0u, 1, ctx->stream));
+    HIPCHK(ctx, launch_fill_u32(*slot, 0u, 1, lane_stream(ctx, lane)));
     return RF_OK;
+}
+
+// Side lanes start after everything queued on `stream` so far.
+static int fan_fork(rf_ctx *ctx)
+{
+    if (ctx->fan_open) return RF_OK;
+    HIPCHK(ctx, hipEventRecord(ctx->fork_ev, ctx->stream));
+    for (int i = 1; i < kFan; ++i) HIPCHK(ctx, hipStreamWaitEvent(ctx->fan[i], ctx->fork_ev, 0));
+    ctx->fan_open = true;
+    ctx->fan_next = 0;
+    return RF_OK;
+}
+
+// `stream` continues after everything queued on the side lanes.
+static int fan_join(rf_ctx *ctx)
+{
+    if (!ctx->fan_open) return RF_OK;
+    for (int i = 1; i < kFan; ++i) {
+        HIPCHK(ctx, hipEventRecord(ctx->fan_ev[i], ctx->fan[i]));
+        HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->fan_ev[i], 0));
+    }
+    ctx->fan_open = false;
+    return RF_OK;
+}
+
+static int fan_lane(rf_ctx *ctx)
+{
+    const int l = ctx->fan_next;
+    ctx->fan_next = (l + 1) % kFan;
+    return l;
 }
 
 static int max_groups(const rf_ctx *ctx, int block)
@@ -575,10 +630,12 @@ static int max_groups(const rf_ctx *ctx, int block)
 }
 
 // Implicit-unit launch over work items [wi0, wi0+n_wi) of one emitter.
+// lane: fan lane whose stream takes the launch (0 = the context stream).
 static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int64_t n_wi,
                         const rf_solve_opts *o, uint32_t geom_id, int32_t *trace_out, int64_t trace_wi0,
-                        int64_t *launches)
+                        int64_t *launches, int lane = 0)
 {
+    const hipStream_t st = lane_stream(ctx, lane);
     if (n_wi <= 0) return RF_OK;
     int nr, block, mfma;
     pick_shape(ctx, o, nr, block, mfma);
@@ -612,7 +669,7 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
         if (grid > INT32_MAX) return set_err(ctx, RF_ERR_CAPACITY, "grid too large");
         c.grid = (int)grid;
         c.lds_bytes = (size_t)kPendK * c.block * sizeof(uint32_t);
-        HIPCHK(ctx, launch_solve(c, kp, ctx->stream));
+        HIPCHK(ctx, launch_solve(c, kp, st));
         ++*launches;
         return RF_OK;
     }
@@ -628,7 +685,7 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
     if (n_units > INT32_MAX) return set_err(ctx, RF_ERR_CAPACITY, "ray budget too large for one launch");
     kp.imp_unit = unit;
     kp.n_units = (int32_t)n_units;
-    int rc = next_counter(ctx, &kp.queue_head);
+    int rc = next_counter(ctx, &kp.queue_head, lane);
     if (rc) return rc;
     c.direct = 0;
     c.block = block;
@@ -641,7 +698,7 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
         if (bvbin_lds(block, kp.bv_slots, kp.bins) > kMaxLds) kp.bins = 0;
         c.lds_bytes = bvbin_lds(block, kp.bv_slots, c.trace ? 0 : kp.bins);
     }
-    HIPCHK(ctx, launch_solve(c, kp, ctx->stream));
+    HIPCHK(ctx, launch_solve(c, kp, st));
     ++*launches;
     return RF_OK;
 }
@@ -658,19 +715,31 @@ extern "C" int32_t rf_launch_emitter(rf_ctx *ctx, rf_scene *s, int32_t emitter, 
         return set_err(ctx, RF_ERR_OVERFLOW, "reference RNG keys work items with 32 bits");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     int64_t launches = 0;
-    return launch_range(ctx, s, emitter, wi0, n_wi, o, o->geometry, nullptr, 0, &launches);
+    // the reference shape keeps one in-order queue; queue-kernel launches fan out
+    if (o->form == RF_FORM_REFERENCE) {
+        rc = fan_join(ctx);
+        if (rc) return rc;
+        return launch_range(ctx, s, emitter, wi0, n_wi, o, o->geometry, nullptr, 0, &launches, 0);
+    }
+    rc = fan_fork(ctx);
+    if (rc) return rc;
+    return launch_range(ctx, s, emitter, wi0, n_wi, o, o->geometry, nullptr, 0, &launches, fan_lane(ctx));
 }
 
 extern "C" int32_t rf_sync(rf_ctx *ctx)
 {
     if (!ctx) return RF_ERR_INVALID_ARG;
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    int rc = fan_join(ctx);
+    if (rc) return rc;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return RF_OK;
 }
 
 static int read_hits(rf_ctx *ctx, rf_scene *s, int64_t *hits)
 {
+    int rc = fan_join(ctx);
+    if (rc) return rc;
     HIPCHK(ctx, hipMemcpyAsync(hits, s->d_hits, (size_t)s->n * s->n * sizeof(int64_t), hipMemcpyDeviceToHost,
                                ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -688,6 +757,8 @@ extern "C" int32_t rf_hits_reset(rf_ctx *ctx, rf_scene *s)
 {
     if (!ctx || !s) return set_err(ctx, RF_ERR_INVALID_ARG, "NULL argument");
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    int rc = fan_join(ctx);
+    if (rc) return rc;
     HIPCHK(ctx, launch_fill_u64(s->d_hits, (size_t)s->n * s->n, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return RF_OK;
@@ -938,7 +1009,9 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
         rc = emitter_budget(ctx, scenes[g], o, budgets[g], geom_rays ? geom_rays[g] : -1);
         if (rc) return rc;
     }
-    rc = ensure_queue(ctx, 4096);
+    rc = fan_join(ctx);
+    if (rc) return rc;
+    rc = ensure_queue(ctx);
     if (rc) return rc;
     int64_t launches = 0, graphed = 0;
     if (o->form == RF_FORM_PERSISTENT) {
@@ -950,9 +1023,17 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
             HIPCHK(ctx, launch_fill_u64(scenes[g]->d_hits, (size_t)scenes[g]->n * scenes[g]->n, ctx->stream));
         HIPCHK(ctx, launch_fill_u64(ctx->d_stats, 3, ctx->stream));
         HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        // one launch per emitter; queue-kernel launches of different emitters
+        // are independent and fan out over the lanes, the reference shape
+        // keeps one in-order queue
+        const bool fan = o->form != RF_FORM_REFERENCE;
+        if (fan) rc = fan_fork(ctx);
         for (int g = 0; g < n_geoms && !rc; ++g)
             for (int e = 0; e < scenes[g]->n && !rc; ++e)
-                rc = launch_range(ctx, scenes[g], e, 0, budgets[g][e], o, ids[g], nullptr, 0, &launches);
+                rc = launch_range(ctx, scenes[g], e, 0, budgets[g][e], o, ids[g], nullptr, 0, &launches,
+                                  fan ? fan_lane(ctx) : 0);
+        const int rj = fan_join(ctx);
+        if (!rc) rc = rj;
     }
     if (rc) { (void)hipStreamSynchronize(ctx->stream); return rc; }
     HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -1017,6 +1098,8 @@ extern "C" int32_t rf_trace_rays(rf_ctx *ctx, rf_scene *s, int32_t emitter, int6
     if (emitter < 0 || emitter >= s->n || wi0 < 0 || n_wi <= 0) return set_err(ctx, RF_ERR_INVALID_ARG, "bad range");
     if (n_wi > ((int64_t)1 << 26)) return set_err(ctx, RF_ERR_CAPACITY, "trace window above 2^26 work items");
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    rc = fan_join(ctx);
+    if (rc) return rc;
     int32_t *d_out = nullptr;
     HIPCHK(ctx, hipMalloc((void **)&d_out, (size_t)n_wi * 4 * sizeof(int32_t)));
     HIPCHK(ctx, launch_fill_u32((unsigned *)d_out, 0xffffffffu, (size_t)n_wi * 4, ctx->stream));

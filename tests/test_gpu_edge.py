@@ -103,3 +103,22 @@ def test_config1_closed_forms_at_scale(ctx, golden_scene):
     assert np.all(F[:3].sum(axis=1) > 0.998)
     # reciprocity A_i F_ij = A_j F_ji (wall area 4 pi, discs pi)
     assert 4 * F[0, 1] == pytest.approx(F[1, 0], rel=2e-3)
+
+
+@pytest.mark.parametrize("form", ["per_emitter", "reference"])
+def test_launch_emitter_fan_out_and_implicit_join(ctx, form):
+    """Per-emitter launches of the queue kernel run on several streams; reading
+    the tally (without rf_sync) waits for all of them.  The reference shape
+    keeps one in-order queue.  Both equal one persistent solve."""
+    sc = scenes.parse_xml(scenes.furnace_xml(wall_bands=30, preform_bands=10))
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    n_wi = 3000
+    o = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=0, form=form)
+    want = rf.solve(ctx, dev, math="fast", rng="seeded", rays_per_emitter=4 * n_wi, form="persistent").hits
+    for rep in range(2):
+        dev.reset_hits()
+        for e in range(sc.n):
+            dev.launch_emitter(e, 0, n_wi // 2, o)
+        for e in reversed(range(sc.n)):
+            dev.launch_emitter(e, n_wi // 2, n_wi - n_wi // 2, o)
+        assert np.array_equal(dev.hits(), want), rep
