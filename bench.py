@@ -190,13 +190,26 @@ def main() -> int:
             res[name] = {"tests_per_s": t["tests"] / dt, "kernel_ms": t["kernel_ms"], "wall_ms": dt * 1e3,
                          "launches": t["launches"]}
             log(f"sample {name}: {res[name]['tests_per_s']:.3e} tests/s, {t['launches']} launches")
+        # one persistent solve per geometry, one after the other
+        tests = launches_g = 0
+        s0 = time.perf_counter()
+        for i, d in enumerate(sub):
+            og = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", refine=refine,
+                              seed=args.seed, geometry=i)
+            _, t = rf.batch_solve(ctx, [d], og, readback=False)
+            tests += t["tests"]; launches_g += t["launches"]
+        dt = time.perf_counter() - s0
+        res["per_geometry"] = {"tests_per_s": tests / dt, "wall_ms": dt * 1e3, "launches": launches_g}
+        log(f"sample per_geometry: {res['per_geometry']['tests_per_s']:.3e} tests/s, {launches_g} launches")
         plain = {
             "geometries": len(sub),
             "batched_tests_per_s": res["batched"]["tests_per_s"],
             "plain_per_emitter_tests_per_s": res["per_emitter"]["tests_per_s"],
             "plain_reference_form_tests_per_s": res["reference_form"]["tests_per_s"],
+            "one_geometry_per_launch_tests_per_s": res["per_geometry"]["tests_per_s"],
             "batched_over_per_emitter": res["batched"]["tests_per_s"] / res["per_emitter"]["tests_per_s"],
             "batched_over_reference_form": res["batched"]["tests_per_s"] / res["reference_form"]["tests_per_s"],
+            "batched_over_one_geometry_per_launch": res["batched"]["tests_per_s"] / res["per_geometry"]["tests_per_s"],
         }
 
     traffic = None
