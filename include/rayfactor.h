@@ -298,6 +298,27 @@ int32_t rf_write_output(const char *path, int32_t n, const int64_t *hits, const 
 /* FP64 normalisation: vf[i][j] = hits[i][j] / rays[i] (0 where rays[i] == 0). */
 int32_t rf_normalise(int32_t n, const int64_t *hits, const int64_t *rays, double *vf);
 
+/* Surface area of each primitive: its object-space shape (unit cylinder
+ * z in [0,1], frustum radius 1+(util1-1)z, unit sphere, unit disc, annulus
+ * r^2 in [util1, util2], rectangle [-1,1]^2) under the linear part of m,
+ * integrated by quadrature (exact for the reference's scalings and
+ * rotations).  No reference counterpart: input of the reciprocity check. */
+int32_t rf_areas(const rf_primitive *prims, int32_t n, double *area);
+
+/* Closure and reciprocity of a view-factor matrix (row = emitter), the
+ * checks beside the output writer.  closure (may be NULL) receives
+ * sum_j vf[i][j].  Reciprocity compares A_i F_ij with A_j F_ji over pairs
+ * where both are at least min_product; rel is |d| / max of the two. */
+typedef struct rf_vf_report {
+    double min_closure, max_closure;  /* over emitters */
+    double max_recip_abs;             /* max |A_i F_ij - A_j F_ji| */
+    double max_recip_rel;             /* max of the same relative to the larger product */
+    int32_t worst_i, worst_j;         /* pair of max_recip_rel (-1 if none compared) */
+    int32_t pairs_compared;
+} rf_vf_report;
+int32_t rf_vf_check(int32_t n, const double *vf, const double *area, double min_product, double *closure,
+                    rf_vf_report *report);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,16 @@ class HostScene(ctypes.Structure):
     ]
 
 
+class VfReport(ctypes.Structure):
+    _fields_ = [
+        ("min_closure", c_double), ("max_closure", c_double), ("max_recip_abs", c_double),
+        ("max_recip_rel", c_double), ("worst_i", c_int32), ("worst_j", c_int32), ("pairs_compared", c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class RayFactorError(RuntimeError):
     def __init__(self, code: int, where: str, detail: str = ""):
         self.code = code
@@ -113,6 +123,8 @@ SIGNATURES = {
     "rf_host_scene_free": (None, [POINTER(HostScene)]),
     "rf_write_output": (c_int32, [c_char_p, c_int32, _P, _P]),
     "rf_normalise": (c_int32, [c_int32, _P, _P, _P]),
+    "rf_areas": (c_int32, [_P, c_int32, _P]),
+    "rf_vf_check": (c_int32, [c_int32, _P, _P, c_double, _P, POINTER(VfReport)]),
 }
 
 _lib = None

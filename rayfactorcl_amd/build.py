@@ -33,7 +33,7 @@ COMMON = [
 # kernels: MFMA results written straight to VGPRs (no accumulator-register copies)
 KERNEL_FLAGS = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
 
-SOURCES = ["rf_kernels.hip", "rf_runtime.cpp", "rf_xml.cpp"]
+SOURCES = ["rf_kernels.hip", "rf_runtime.cpp", "rf_xml.cpp", "rf_report.cpp"]
 
 
 def _run(cmds):

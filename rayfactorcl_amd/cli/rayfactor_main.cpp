@@ -3,10 +3,12 @@
 // reference), runs the view-factor solve on HIP device 0 and writes
 // "output.txt" in the reference format.
 //
-//   rayfactor [scene.xml] [--math compat|fp64|fast] [--rays N] [--out PATH]
+//   rayfactor [scene.xml] [--math compat|fp64|fast] [--rays N] [--out PATH] [--report]
 //
 // --math compat (default) reproduces the reference's launch geometry and
 // arithmetic; --rays N switches to an explicit budget of N rays per emitter.
+// --report prints the closure (row sums) and reciprocity (A_i F_ij against
+// A_j F_ji over pairs with both products >= 1e-3) of the FP64 view factors.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -26,6 +28,7 @@ int main(int argc, char **argv)
     std::string path = "input.xml", out = "output.txt";
     int math = RF_MATH_COMPAT;
     long long rays = 0;
+    bool report = false;
     for (int i = 1; i < argc; ++i) {
         if (!std::strcmp(argv[i], "--math") && i + 1 < argc) {
             const char *m = argv[++i];
@@ -34,6 +37,8 @@ int main(int argc, char **argv)
             rays = std::atoll(argv[++i]);
         } else if (!std::strcmp(argv[i], "--out") && i + 1 < argc) {
             out = argv[++i];
+        } else if (!std::strcmp(argv[i], "--report")) {
+            report = true;
         } else {
             path = argv[i];
         }
@@ -63,6 +68,20 @@ int main(int argc, char **argv)
     rc = rf_write_output(out.c_str(), (int)n, hits.data(), samples.data());
     if (rc) return fail("output", rc, ctx);
     std::printf("Number of objects : %zu\nKernel time : %.3f ms\nRays : %lld\n", n, t.kernel_ms, (long long)t.rays);
+    if (report) {
+        std::vector<double> vf(n * n), area(n), closure(n);
+        rf_vf_report r;
+        rc = rf_normalise((int)n, hits.data(), nr.data(), vf.data());
+        if (!rc) rc = rf_areas(hs.prims, (int)n, area.data());
+        if (!rc) rc = rf_vf_check((int)n, vf.data(), area.data(), 1e-3, closure.data(), &r);
+        if (rc) return fail("report", rc, ctx);
+        std::printf("Closure : min %.8f max %.8f\n", r.min_closure, r.max_closure);
+        if (r.pairs_compared > 0)
+            std::printf("Reciprocity : %d pairs, max |AiFij - AjFji| %.3e, max relative %.3e (pair %d, %d)\n",
+                        r.pairs_compared, r.max_recip_abs, r.max_recip_rel, r.worst_i, r.worst_j);
+        else
+            std::printf("Reciprocity : no pair with both products >= 1e-3\n");
+    }
     rf_scene_destroy(scene);
     rf_ctx_destroy(ctx);
     rf_host_scene_free(&hs);

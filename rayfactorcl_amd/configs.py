@@ -21,7 +21,7 @@ import time
 import numpy as np
 
 from . import scenes
-from .solver import Context, DeviceScene, Stream, make_opts, solve, trace
+from .solver import Context, DeviceScene, Stream, areas, make_opts, solve, trace, vf_report
 
 # (emitter, receiver, closed form, edge allowance, what) for the config-1 scene:
 # closed cylinder r=1, H=2 with end discs; coaxial unit discs at spacing 1.
@@ -59,6 +59,14 @@ def _ray_diff(ctx, dev, n, n_wi, bv, seeds=8):
     return diff / max(total, 1), total
 
 
+def _report(vf, area, min_product: float = 1e-3) -> dict:
+    """Closure and reciprocity summary of one view-factor matrix."""
+    r = vf_report(vf, area, min_product)
+    c = r.pop("closure")
+    r["closure"] = [round(float(x), 8) for x in c]
+    return r
+
+
 def config1(ctx, quick: bool) -> dict:
     sc = scenes.parse_xml(scenes.config1_xml())
     dev = DeviceScene(ctx, sc.prims, None)
@@ -66,6 +74,7 @@ def config1(ctx, quick: bool) -> dict:
     out = {"config": 1, "scene": "closed cylinder r=1 H=2 + end discs, coaxial unit discs at spacing 1",
            "primitives": sc.n, "rays_per_emitter": rays}
     res = {}
+    area = areas(sc.prims)
     for name, kw in [("compat", dict(math="compat", rng="reference")), ("fast", dict(math="fast", rng="seeded")),
                      ("fp64", dict(math="fp64", rng="seeded"))]:
         r = solve(ctx, dev, rays_per_emitter=rays, form="persistent", **kw)
@@ -76,7 +85,7 @@ def config1(ctx, quick: bool) -> dict:
             sig = 2.0 * math.sqrt(exact * (1 - exact) / rays)   # 4-ray work items share a block
             checks.append({"pair": [i, j], "what": what, "vf": f, "closed_form": exact, "sigma": sig,
                            "within_4sigma_plus_edge": abs(f - exact) <= 4 * sig + allow})
-        out[name] = {**_rate(r), "closed_forms": checks}
+        out[name] = {**_rate(r), "closed_forms": checks, "report": _report(r.vf, area)}
     out["fast_vs_fp64_max_dvf"] = _dvf(res["fast"], res["fp64"])
     dev.close()
     return out
@@ -88,13 +97,14 @@ def config2(ctx, quick: bool) -> dict:
     rays = 10_000_000 if quick else 100_000_000
     out = {"config": 2, "scene": "furnace: 180 wall bands + 60 preform bands + neck + annuli + end disc, one bounding surface per group",
            "primitives": sc.n, "bounding_volumes": len(sc.bvs), "rays_per_emitter": rays}
+    area = areas(sc.prims)
     for bv in (False, True):
         key = "with_bv" if bv else "without_bv"
         a = solve(ctx, dev, math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", use_bvs=bv)
         b = solve(ctx, dev, math="fp64", rng="seeded", rays_per_emitter=rays, form="persistent", use_bvs=bv)
         frac, nr = _ray_diff(ctx, dev, sc.n, 1 << 16, bv)
         out[key] = {"fast": _rate(a), "fp64": _rate(b), "fast_vs_fp64_max_dvf": _dvf(a, b),
-                    "differing_rays": frac, "rays_compared": nr}
+                    "differing_rays": frac, "rays_compared": nr, "fast_report": _report(a.vf, area)}
     dev.close()
     return out
 

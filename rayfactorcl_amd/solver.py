@@ -218,6 +218,32 @@ def normalise(hits: np.ndarray, rays: np.ndarray) -> np.ndarray:
     return vf
 
 
+def areas(prims: np.ndarray) -> np.ndarray:
+    """Surface area of each primitive (object shape under its m transform, by quadrature)."""
+    prims = np.ascontiguousarray(prims, dtype=PRIM_DTYPE)
+    out = np.zeros(len(prims), dtype=np.float64)
+    check(lib().rf_areas(ptr(prims), len(prims), ptr(out)), "rf_areas")
+    return out
+
+
+def vf_report(vf: np.ndarray, area: np.ndarray, min_product: float = 0.0) -> dict:
+    """Closure (row sums) and reciprocity (A_i F_ij vs A_j F_ji) of a view-factor matrix.
+
+    Returns the report fields plus ``closure``, the per-emitter row sums.
+    """
+    vf = np.ascontiguousarray(vf, dtype=np.float64)
+    area = np.ascontiguousarray(area, dtype=np.float64)
+    n = len(area)
+    if vf.shape != (n, n):
+        raise ValueError(f"vf has shape {vf.shape}, expected {(n, n)}")
+    closure = np.zeros(n, dtype=np.float64)
+    rep = L.VfReport()
+    check(lib().rf_vf_check(n, ptr(vf), ptr(area), float(min_product), ptr(closure), byref(rep)), "rf_vf_check")
+    out = rep.as_dict()
+    out["closure"] = closure
+    return out
+
+
 def write_output(path: str, hits: np.ndarray, samples: np.ndarray) -> None:
     """output.txt in the reference format (main.cpp:236-251)."""
     hits = np.ascontiguousarray(hits, dtype=np.int64)

@@ -70,3 +70,47 @@ def test_configs_report_quick(ctx):
     assert r1["fast_vs_fp64_max_dvf"] <= 1e-4 * 10     # 1e6 rays: binomial noise of differing rays
     r5 = configs.config5(ctx, quick=True, n_req=6)
     assert r5["requests"] == 6 and r5["latency_p99_s"] >= r5["latency_p50_s"] > 0
+
+
+def test_closure_reciprocity_report(ctx):
+    """Report of a solved closed-cylinder configuration: the enclosure rows
+    close to 1 (only rim-grazing rays escape), the open coaxial pair sees its
+    partner at the disc-to-disc view factor, and A_i F_ij matches A_j F_ji
+    within the Monte Carlo error."""
+    sc = scenes.parse_xml(scenes.config1_xml())
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    rays = 4_000_000
+    r = rf.solve(ctx, dev, math="fast", rng="seeded", rays_per_emitter=rays)
+    rep = rf.vf_report(r.vf, rf.areas(sc.prims), min_product=1e-2)
+    dev.close()
+    c = rep["closure"]
+    assert np.all(c[:3] <= 1.0) and np.all(c[:3] >= 1.0 - 1e-5)
+    f_pp = 0.5 * (3.0 - 5.0 ** 0.5)
+    sig = 2.0 * (f_pp * (1 - f_pp) / rays) ** 0.5
+    # the pair also sees the enclosure's outside at grazing angles (< 1e-2)
+    for i, j in ((3, 4), (4, 3)):
+        assert abs(r.vf[i, j] - f_pp) < 5 * sig
+        assert r.vf[i, j] <= c[i] < r.vf[i, j] + 1e-2
+    assert rep["pairs_compared"] == 4
+    assert rep["max_recip_rel"] < 8e-3
+
+
+def test_cli_output_and_report(ctx, tmp_path):
+    """The drop-in program writes the same output.txt as a library solve with
+    the same options and prints the closure / reciprocity report."""
+    import subprocess
+    from rayfactorcl_amd.build import CLI
+    xml = tmp_path / "input.xml"
+    xml.write_text(scenes.config1_xml())
+    out = tmp_path / "output.txt"
+    rays = 1_000_000
+    p = subprocess.run([str(CLI), str(xml), "--math", "fast", "--rays", str(rays), "--out", str(out), "--report"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert "Closure : min" in p.stdout and "Reciprocity : 4 pairs" in p.stdout
+    sc = scenes.parse_xml(scenes.config1_xml())
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    r = rf.solve(ctx, dev, math="fast", rays_per_emitter=rays)
+    dev.close()
+    rf.write_output(str(tmp_path / "expected.txt"), r.hits, r.rays.astype(np.float64))
+    assert out.read_text() == (tmp_path / "expected.txt").read_text()
