@@ -74,7 +74,7 @@ def test_configs_report_quick(ctx):
 
 def test_closure_reciprocity_report(ctx):
     """Report of a solved closed-cylinder configuration: the enclosure rows
-    close to 1 (only rim-grazing rays escape), the open coaxial pair sees its
+    close to 1 (less the seam losses of the EPS guard), the open coaxial pair sees its
     partner at the disc-to-disc view factor, and A_i F_ij matches A_j F_ji
     within the Monte Carlo error."""
     sc = scenes.parse_xml(scenes.config1_xml())
@@ -84,7 +84,10 @@ def test_closure_reciprocity_report(ctx):
     rep = rf.vf_report(r.vf, rf.areas(sc.prims), min_product=1e-2)
     dev.close()
     c = rep["closure"]
-    assert np.all(c[:3] <= 1.0) and np.all(c[:3] >= 1.0 - 1e-5)
+    # rays leaving within EPS of a seam meet the neighbouring surface at
+    # t <= EPS and are not counted (the reference's self-hit guard): an edge
+    # loss of a few 1e-4, bounded as the closed-form checks bound it
+    assert np.all(c[:3] <= 1.0) and np.all(c[:3] >= 1.0 - 1.5e-3)
     f_pp = 0.5 * (3.0 - 5.0 ** 0.5)
     sig = 2.0 * (f_pp * (1 - f_pp) / rays) ** 0.5
     # the pair also sees the enclosure's outside at grazing angles (< 1e-2)
