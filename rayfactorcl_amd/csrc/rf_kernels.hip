@@ -226,7 +226,7 @@ __device__ __forceinline__ void find_c_multi(CPTR(rf_primitive) P, int start, in
 #define ROW_S(P, i, R) __builtin_fmaf(P.r[4 * i], R.sx, __builtin_fmaf(P.r[4 * i + 1], R.sy, __builtin_fmaf(P.r[4 * i + 2], R.sz, P.r[4 * i + 3])))
 #define ROW_C(P, i, R) __builtin_fmaf(P.r[4 * i], R.cx, __builtin_fmaf(P.r[4 * i + 1], R.cy, P.r[4 * i + 2] * R.cz))
 
-template <int KIND>
+template <int KIND, bool REFINE = true>
 __device__ __forceinline__ float fast_test(const FastPrim &P, const Ray32 &R, bool &flag, float tau, float eps)
 {
     if constexpr (KIND == SEG_CYL) {
@@ -246,15 +246,24 @@ __device__ __forceinline__ float fast_test(const FastPrim &P, const Ray32 &R, bo
         const float z1 = __builtin_fmaf(cz, t1, sz);
         const float z2 = __builtin_fmaf(cz, t2, sz);
         const bool e1 = t1 > eps, e2 = t2 > eps;
-        const bool v1 = e1 & (__builtin_fabsf(z1) <= 0.5f);
-        const bool v2 = e2 & (__builtin_fabsf(z2) <= 0.5f);
-        // near-tangent only where the tangent point lies on this band; a rim
-        // only for a root that could be accepted
-        const float zm = __builtin_fmaf(cz, tm, sz);
-        flag = ((__builtin_fabsf(D) <= tau * A) & (__builtin_fabsf(zm) <= 0.5f + tau)) |
-               (e1 & (__builtin_fabsf(__builtin_fabsf(z1) - 0.5f) <= tau)) |
-               (e2 & (__builtin_fabsf(__builtin_fabsf(z2) - 0.5f) <= tau));
-        return v1 ? t1 : (v2 ? t2 : finf());
+        const float a1 = __builtin_fabsf(z1), a2 = __builtin_fabsf(z2);
+        if constexpr (!REFINE) {
+            flag = false;
+            return (e1 & (a1 <= 0.5f)) ? t1 : ((e2 & (a2 <= 0.5f)) ? t2 : finf());
+        } else {
+            // A root is certainly on the band for |z| < 1/2 - tau and
+            // certainly off it for |z| > 1/2 + tau; in between it is a rim
+            // flag.  A flagged pair's FP32 t is discarded, so the FP32
+            // decision of an unflagged pair equals the |z| <= 1/2 test.
+            const float lo = 0.5f - tau, hi = 0.5f + tau;
+            const bool in1 = a1 < lo, in2 = a2 < lo;
+            // near-tangent only where the tangent point lies on this band
+            const float zm = __builtin_fmaf(cz, tm, sz);
+            // in implies a <= hi (lo <= hi), so "a <= hi and not in" is an xor
+            flag = ((__builtin_fabsf(D) <= tau * A) & (__builtin_fabsf(zm) <= hi)) |
+                   (e1 & ((a1 <= hi) ^ in1)) | (e2 & ((a2 <= hi) ^ in2));
+            return (e1 & in1) ? t1 : ((e2 & in2) ? t2 : finf());
+        }
     } else if constexpr (KIND == SEG_FRUSTUM) {
         const float sx = ROW_S(P, 0, R), sy = ROW_S(P, 1, R), sw = ROW_S(P, 2, R);
         const float cx = ROW_C(P, 0, R), cy = ROW_C(P, 1, R), cw = ROW_C(P, 2, R);
@@ -273,15 +282,20 @@ __device__ __forceinline__ float fast_test(const FastPrim &P, const Ray32 &R, bo
         const float w1 = __builtin_fmaf(cw, t1, sw) - P.k0;
         const float w2 = __builtin_fmaf(cw, t2, sw) - P.k0;
         const bool e1 = t1 > eps, e2 = t2 > eps;
-        const bool v1 = e1 & (__builtin_fabsf(w1) <= P.k1);
-        const bool v2 = e2 & (__builtin_fabsf(w2) <= P.k1);
-        const float rim = (2.0f * tau) * P.k1;
-        const float wm = __builtin_fmaf(cw, tm, sw) - P.k0;
-        flag = ((__builtin_fabsf(D) <= tau * (b * b)) & (__builtin_fabsf(wm) <= P.k1 + rim)) |
-               (__builtin_fabsf(A) < tau * Axy) |
-               (e1 & (__builtin_fabsf(__builtin_fabsf(w1) - P.k1) <= rim)) |
-               (e2 & (__builtin_fabsf(__builtin_fabsf(w2) - P.k1) <= rim));
-        return v1 ? t1 : (v2 ? t2 : finf());
+        const float a1 = __builtin_fabsf(w1), a2 = __builtin_fabsf(w2);
+        if constexpr (!REFINE) {
+            flag = false;
+            return (e1 & (a1 <= P.k1)) ? t1 : ((e2 & (a2 <= P.k1)) ? t2 : finf());
+        } else {
+            // band of half-width k1 with rims of 2 tau k1 (as the cylinder)
+            const float lo = (1.0f - 2.0f * tau) * P.k1, hi = (1.0f + 2.0f * tau) * P.k1;
+            const bool in1 = a1 < lo, in2 = a2 < lo;
+            const float wm = __builtin_fmaf(cw, tm, sw) - P.k0;
+            flag = ((__builtin_fabsf(D) <= tau * (b * b)) & (__builtin_fabsf(wm) <= hi)) |
+                   (__builtin_fabsf(A) < tau * Axy) |
+                   (e1 & ((a1 <= hi) ^ in1)) | (e2 & ((a2 <= hi) ^ in2));
+            return (e1 & in1) ? t1 : ((e2 & in2) ? t2 : finf());
+        }
     } else if constexpr (KIND == SEG_SPHERE) {
         const float sx = ROW_S(P, 0, R), sy = ROW_S(P, 1, R), sz = ROW_S(P, 2, R);
         const float cx = ROW_C(P, 0, R), cy = ROW_C(P, 1, R), cz = ROW_C(P, 2, R);
@@ -316,14 +330,15 @@ __device__ __forceinline__ float fast_test(const FastPrim &P, const Ray32 &R, bo
     }
 }
 
+template <bool REFINE = true>
 __device__ __forceinline__ float fast_test_any(const FastPrim &P, const Ray32 &R, bool &flag, float tau, float eps)
 {
     switch (P.type) {
-    case RF_CYLINDER: return fast_test<SEG_CYL>(P, R, flag, tau, eps);
-    case RF_FRUSTUM: return fast_test<SEG_FRUSTUM>(P, R, flag, tau, eps);
-    case RF_SPHERE: return fast_test<SEG_SPHERE>(P, R, flag, tau, eps);
-    case RF_RECTANGLE: return fast_test<SEG_RECT>(P, R, flag, tau, eps);
-    default: return fast_test<SEG_RING>(P, R, flag, tau, eps);
+    case RF_CYLINDER: return fast_test<SEG_CYL, REFINE>(P, R, flag, tau, eps);
+    case RF_FRUSTUM: return fast_test<SEG_FRUSTUM, REFINE>(P, R, flag, tau, eps);
+    case RF_SPHERE: return fast_test<SEG_SPHERE, REFINE>(P, R, flag, tau, eps);
+    case RF_RECTANGLE: return fast_test<SEG_RECT, REFINE>(P, R, flag, tau, eps);
+    default: return fast_test<SEG_RING, REFINE>(P, R, flag, tau, eps);
     }
 }
 
@@ -403,14 +418,18 @@ __device__ __forceinline__ void seg_loop(CPTR(FastPrim) F, int begin, int end, c
                                          float (&tb)[NR], int (&ib)[NR], Pending &pd, int slot = 0,
                                          bool act = true)
 {
+    if (begin >= end) return;
+    // the next record's scalar load is issued a whole iteration ahead
+    FastPrim next = ldc(F + begin);
     for (int w = begin; w < end; ++w) {
-        const FastPrim q = vcopy(ldc(F + w));
+        const FastPrim q = vcopy(next);
+        next = ldc(F + min(w + 1, end - 1));
         float t[NR];
         bool f[NR];
         bool any = false;
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
-            t[i] = fast_test<KIND>(q, R[i], f[i], pd.tau, pd.eps);
+            t[i] = fast_test<KIND, REFINE>(q, R[i], f[i], pd.tau, pd.eps);
             if constexpr (RESCAN) {
                 const bool skip = w < pd.from;
                 f[i] = f[i] & !skip;
@@ -558,7 +577,7 @@ __device__ __forceinline__ int find_fast_range(CPTR(FastPrim) F, CPTR(rf_primiti
     for (int w = start; w < end; ++w) {
         const FastPrim q = ldc(F + w);
         bool f;
-        float t = fast_test_any(q, R, f, tau, EPS_F);
+        float t = fast_test_any<REFINE>(q, R, f, tau, EPS_F);
         if constexpr (REFINE) {
             if (__builtin_amdgcn_ballot_w64(f)) {
                 if (f) { t = refine_prim(P, w, R); ++nref; }
@@ -585,7 +604,7 @@ __device__ __forceinline__ int find_bv_fast(CPTR(GeomDesc) G, const Ray32 &R, ui
         const FastPrim q = ldc(FB + w);
         const BvMeta meta = ldc(M + w);
         bool f;
-        float t = fast_test_any(q, R, f, tau, EPS_F);
+        float t = fast_test_any<REFINE>(q, R, f, tau, EPS_F);
         if constexpr (REFINE) {
             if (__builtin_amdgcn_ballot_w64(f)) {
                 if (f) { t = refine_bv(B, w, R); ++nref; }
@@ -1126,7 +1145,7 @@ __device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc)
 #pragma unroll
                 for (int i = 0; i < BNR; ++i) {
                     bool f;
-                    float t = fast_test_any(q, R[i], f, tau, eps);
+                    float t = fast_test_any<REFINE>(q, R[i], f, tau, eps);
                     if constexpr (REFINE) {
                         if (__builtin_amdgcn_ballot_w64(f)) {
                             if (f) {
