@@ -154,13 +154,22 @@ __device__ __forceinline__ float vcopy(float x)
     return r;
 }
 
+// Two floats per v_mov_b64 (gfx950 has the 64-bit move).
+__device__ __forceinline__ void vcopy2(float a, float b, float &ra, float &rb)
+{
+    const uint64_t x = ((uint64_t)__float_as_uint(b) << 32) | __float_as_uint(a);
+    uint64_t r;
+    asm("v_mov_b64 %0, %1" : "=v"(r) : "s"(x));
+    ra = __uint_as_float((uint32_t)r);
+    rb = __uint_as_float((uint32_t)(r >> 32));
+}
+
 __device__ __forceinline__ FastPrim vcopy(const FastPrim &q)
 {
     FastPrim v;
 #pragma unroll
-    for (int i = 0; i < 12; ++i) v.r[i] = vcopy(q.r[i]);
-    v.k0 = vcopy(q.k0);
-    v.k1 = vcopy(q.k1);
+    for (int i = 0; i < 12; i += 2) vcopy2(q.r[i], q.r[i + 1], v.r[i], v.r[i + 1]);
+    vcopy2(q.k0, q.k1, v.k0, v.k1);
     v.orig = q.orig;
     v.type = q.type;
     return v;
@@ -181,7 +190,7 @@ __device__ __forceinline__ void find_c_multi(CPTR(rf_primitive) P, int start, in
         const rf_primitive p = ldc(P + w);
         float m[12];
 #pragma unroll
-        for (int k = 0; k < 12; ++k) m[k] = vcopy(p.invm[k]);
+        for (int k = 0; k < 12; k += 2) vcopy2(p.invm[k], p.invm[k + 1], m[k], m[k + 1]);
         float t[NR];
         switch (p.type) {
         case RF_CYLINDER:
