@@ -298,6 +298,19 @@ int32_t rf_write_output(const char *path, int32_t n, const int64_t *hits, const 
 /* FP64 normalisation: vf[i][j] = hits[i][j] / rays[i] (0 where rays[i] == 0). */
 int32_t rf_normalise(int32_t n, const int64_t *hits, const int64_t *rays, double *vf);
 
+/* Device self-test of the compat arithmetic (no reference counterpart).  The
+ * compat kernels pin native_recip / native_sqrt to IEEE 1/x and sqrt and reach
+ * them through short hardware sequences (a Newton step, a one-ulp correction)
+ * that report when an operand is outside their exact range; the test is then
+ * repeated with the IEEE operations.  This checks, over all 2^32 float inputs,
+ * that the sequences give the compiler's IEEE results wherever they report
+ * success: mismatches[0] (reciprocal) and mismatches[1] (square root) are 0
+ * on a conforming device.  [2] counts the Newton reciprocal's misses on all
+ * inputs and [3] the raw v_sqrt_f32's misses on positive normal inputs
+ * (information).
+ * first[k] = lowest mismatching input bits (0xffffffff if none). */
+int32_t rf_selftest_math(rf_ctx *ctx, uint64_t mismatches[4], uint32_t first[4]);
+
 /* Surface area of each primitive: its object-space shape (unit cylinder
  * z in [0,1], frustum radius 1+(util1-1)z, unit sphere, unit disc, annulus
  * r^2 in [util1, util2], rectangle [-1,1]^2) under the linear part of m,

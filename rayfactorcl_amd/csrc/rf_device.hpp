@@ -128,8 +128,68 @@ struct Ray64 { double sx, sy, sz, cx, cy, cz; };
 // ===========================================================================
 // COMPAT (FP32, reference operation order)
 // ===========================================================================
-__device__ __forceinline__ float recip_c(float x) { return 1.0f / x; }                  // native_recip
-__device__ __forceinline__ float rsqrt_c(float x) { return 1.0f / __builtin_sqrtf(x); } // native_rsqrt
+// native_recip / native_sqrt / native_rsqrt are pinned to IEEE 1/x and sqrt
+// (correctly rounded).  The compiler's IEEE sequences cost 11 (division) and
+// ~16 (square root) instructions.  Inside the primitive scan (find_c_multi)
+// the hit tests use FastOps instead, which reach the same bits from the
+// hardware approximations in 3 and 8 instructions:
+//   recip_nr: v_rcp_f32 and one FMA Newton step, valid where the result is a
+//     normal float;
+//   sqrt_adj: v_sqrt_f32 and a one-ulp correction from the FMA residuals
+//     x - s*(s -/+ ulp), valid for |x| >= 2^-96 (negative x and NaN give NaN).
+// An operand outside those ranges clears FastOps::ok, and the scan repeats
+// that ray's test with IeeeOps (a wave-uniform branch, rarely taken).
+// rf_selftest_math checks "ok implies the same bits as 1.0f / x, sqrtf(x)"
+// over all 2^32 inputs on the device.
+__device__ __forceinline__ float recip_ieee(float x) { return 1.0f / x; }
+__device__ __forceinline__ float sqrt_ieee(float x) { return __builtin_sqrtf(x); }
+
+__device__ __forceinline__ float recip_nr(float x)
+{
+    const float r0 = __builtin_amdgcn_rcpf(x);
+    return __builtin_fmaf(__builtin_fmaf(-x, r0, 1.0f), r0, r0);
+}
+
+__device__ __forceinline__ float sqrt_adj(float x)
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+    const float su = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = (__builtin_fmaf(-sd, s, x) <= 0.0f) ? sd : s;
+    r = (__builtin_fmaf(-su, s, x) > 0.0f) ? su : r;
+    return r;
+}
+
+// class mask of v_cmp_class_f32: -normal | +normal
+constexpr int CLS_NORMAL = 0x108;
+// sqrt_adj's residuals lose accuracy for tiny |x| (on gfx950 it differs from
+// sqrtf for |x| < 2^-104, subnormals included); below this bound the IEEE
+// operation runs
+constexpr float SQRT_ADJ_MIN = 0x1p-96f;
+
+struct IeeeOps {
+    __device__ __forceinline__ float rcp(float x) { return recip_ieee(x); }
+    __device__ __forceinline__ float sqrt(float x) { return sqrt_ieee(x); }
+};
+
+struct FastOps {
+    bool ok = true;
+    __device__ __forceinline__ float rcp(float x)
+    {
+        const float r = recip_nr(x);
+        ok = ok & __builtin_amdgcn_class(r, CLS_NORMAL);
+        return r;
+    }
+    __device__ __forceinline__ float sqrt(float x)
+    {
+        ok = ok & !(__builtin_fabsf(x) < SQRT_ADJ_MIN);
+        return sqrt_adj(x);
+    }
+};
+
+__device__ __forceinline__ float recip_c(float x) { return recip_ieee(x); }
+__device__ __forceinline__ float sqrt_c(float x) { return sqrt_ieee(x); }
+__device__ __forceinline__ float rsqrt_c(float x) { return 1.0f / sqrt_ieee(x); }
 
 // xfrmRay (RayFactorGPU.cl:99-109)
 __device__ __forceinline__ Ray32 xfrm_c(const float *__restrict__ m, const Ray32 &w)
@@ -145,11 +205,12 @@ __device__ __forceinline__ Ray32 xfrm_c(const float *__restrict__ m, const Ray32
 }
 
 // shared tail of cylinderHit (:199-221) and frustumHit (:237-259)
-__device__ __forceinline__ float roots_select_c(float A, float B, float C, const Ray32 &o)
+template <class Ops>
+__device__ __forceinline__ float roots_select_c(float A, float B, float C, const Ray32 &o, Ops &ops)
 {
-    const float discrim = __builtin_sqrtf(B * B - A * C);
+    const float discrim = ops.sqrt(B * B - A * C);
     float t1, t2;
-    t1 = t2 = recip_c(A);
+    t1 = t2 = ops.rcp(A);
     t1 *= (B - discrim);
     t2 *= (B + discrim);
     const float z1 = o.sz + o.cz * t1;
@@ -162,25 +223,25 @@ __device__ __forceinline__ float roots_select_c(float A, float B, float C, const
 // One test of findObject's switch (RayFactorGPU.cl:452-461), hit tests :126-291.
 // One type of the compat hit test (operation for operation as the cited
 // reference lines); hit_c dispatches on the type.
-template <int TYPE>
-__device__ __forceinline__ float hit_c_t(const float *__restrict__ m, float u1, float u2, const Ray32 &w)
+template <int TYPE, class Ops>
+__device__ __forceinline__ float hit_c_ops(const float *__restrict__ m, float u1, float u2, const Ray32 &w, Ops &ops)
 {
     const Ray32 o = xfrm_c(m, w);
     if constexpr (TYPE == RF_RECTANGLE) {                              // :126-145
-        const float th = -o.sz * recip_c(o.cz);
+        const float th = -o.sz * ops.rcp(o.cz);
         const float hx = __builtin_fabsf(o.sx + o.cx * th);
         const float hy = __builtin_fabsf(o.sy + o.cy * th);
         if (hx > 1.f || hy > 1.f || th < EPS_F) return finf();
         return th;
     } else if constexpr (TYPE == RF_DISC) {                            // :147-166
-        const float th = -o.sz * recip_c(o.cz);
+        const float th = -o.sz * ops.rcp(o.cz);
         const float hx = o.sx + o.cx * th;
         const float hy = o.sy + o.cy * th;
         const float e = hx * hx + hy * hy;
         if (e <= 1.f && th > EPS_F) return th;
         return finf();
     } else if constexpr (TYPE == RF_ANNULUS) {                         // :168-187
-        const float th = -o.sz * recip_c(o.cz);
+        const float th = -o.sz * ops.rcp(o.cz);
         const float hx = o.sx + o.cx * th;
         const float hy = o.sy + o.cy * th;
         const float e = __builtin_fmaf(hx, hx, hy * hy);
@@ -190,14 +251,14 @@ __device__ __forceinline__ float hit_c_t(const float *__restrict__ m, float u1, 
         const float A = __builtin_fmaf(o.cx, o.cx, o.cy * o.cy);
         const float B = -__builtin_fmaf(o.sx, o.cx, o.sy * o.cy);
         const float C = __builtin_fmaf(o.sx, o.sx, __builtin_fmaf(o.sy, o.sy, -1.f));
-        return roots_select_c(A, B, C, o);
+        return roots_select_c(A, B, C, o, ops);
     } else if constexpr (TYPE == RF_FRUSTUM) {                         // :224-260
         const float fDir = (u1 - 1.0f) * o.cz;
         const float fStart = (u1 - 1.0f) * o.sz + 1.0f;
         const float A = __builtin_fmaf(o.cx, o.cx, __builtin_fmaf(o.cy, o.cy, (-fDir) * fDir));
         const float B = -__builtin_fmaf(o.sx, o.cx, __builtin_fmaf(o.sy, o.cy, (-fStart) * fDir));
         const float C = __builtin_fmaf(o.sx, o.sx, __builtin_fmaf(o.sy, o.sy, (-fStart) * fStart));
-        return roots_select_c(A, B, C, o);
+        return roots_select_c(A, B, C, o, ops);
     } else {                                                           // :262-291 (no sqrt, as the reference)
         const float A = o.cx * o.cx + o.cy * o.cy + o.cz * o.cz;
         const float B = o.cx * o.sx + o.cy * o.sy + o.cz * o.sz;
@@ -205,13 +266,20 @@ __device__ __forceinline__ float hit_c_t(const float *__restrict__ m, float u1, 
         float discrim = B * B - A * C;
         discrim = (discrim < 0.0f) ? finf() : discrim;
         if (discrim > 0.f) {
-            const float t1 = (-B - discrim) * recip_c(A);
-            const float t2 = (-B + discrim) * recip_c(A);
+            const float t1 = (-B - discrim) * ops.rcp(A);
+            const float t2 = (-B + discrim) * ops.rcp(A);
             if (t2 < t1 && t2 > EPS_F) return t2;
             else if (t1 > EPS_F) return t1;
         }
         return finf();
     }
+}
+
+template <int TYPE>
+__device__ __forceinline__ float hit_c_t(const float *__restrict__ m, float u1, float u2, const Ray32 &w)
+{
+    IeeeOps ops;
+    return hit_c_ops<TYPE>(m, u1, u2, w, ops);
 }
 
 __device__ __forceinline__ float hit_c(int type, const float *__restrict__ m, float u1, float u2,
@@ -245,8 +313,8 @@ __device__ __forceinline__ void direction_c(const float r4[4], const float n[3],
     float o0, o1, o2;
     sincos_pinned(r4[2] * M_2PI_F, &o1, &o0);
     float t = r4[3];
-    o2 = __builtin_sqrtf(1.0f - t);
-    t = __builtin_sqrtf(t);
+    o2 = sqrt_c(1.0f - t);
+    t = sqrt_c(t);
     o0 = t * o0;
     o1 = t * o1;
     float d = recip_c(1.0f + n[2]);
@@ -274,7 +342,7 @@ __device__ __forceinline__ Ray32 emit_c(const rf_primitive *__restrict__ p, cons
     if (type == RF_DISC || type == RF_ANNULUS) {                       // :298-345
         sincos_pinned(r4[0] * M_2PI_F, &sn, &cs);
         vx = sn; vy = cs;
-        t = (type == RF_DISC) ? __builtin_sqrtf(r4[1]) : __builtin_sqrtf((u2 - u1) * r4[1] + u1);
+        t = (type == RF_DISC) ? sqrt_c(r4[1]) : sqrt_c((u2 - u1) * r4[1] + u1);
         vx = t * vx; vy = t * vy;
         point_c(p->m, vx, vy, 0.f, ray);
         normal_c(p->invm, 0.f, 0.f, 1.f, n);
@@ -284,7 +352,7 @@ __device__ __forceinline__ Ray32 emit_c(const rf_primitive *__restrict__ p, cons
         point_c(p->m, vx, vy, vz, ray);
         normal_c(p->invm, vx * u2, vy * u2, 0.f, n);
     } else if (type == RF_FRUSTUM) {                                   // :369-393
-        vz = (__builtin_sqrtf((1.0f - r4[0]) + (r4[0] * u1) * u1) - 1.0f) * recip_c(u1 - 1.0f);
+        vz = (sqrt_c((1.0f - r4[0]) + (r4[0] * u1) * u1) - 1.0f) * recip_c(u1 - 1.0f);
         t = 1.0f + vz * (u1 - 1.0f);
         sincos_pinned(M_2PI_F * r4[1], &sn, &cs);
         vy = sn; vx = cs;
@@ -298,7 +366,7 @@ __device__ __forceinline__ Ray32 emit_c(const rf_primitive *__restrict__ p, cons
         normal_c(p->invm, 0.f, 0.f, 1.f, n);
     } else {                                                           // sphere :419-443
         vz = 2.0f * r4[0] - 1.0f;
-        t = __builtin_sqrtf(1.0f - vz * vz);
+        t = sqrt_c(1.0f - vz * vz);
         sincos_pinned(M_2PI_F * r4[1], &sn, &cs);
         vy = sn; vx = cs;
         vx *= t; vy *= t;

@@ -104,6 +104,7 @@ struct LaunchCfg {
 hipError_t launch_solve(const LaunchCfg &cfg, const KParams &kp, hipStream_t stream);
 hipError_t launch_fill_u32(unsigned int *p, unsigned int value, size_t count, hipStream_t stream);
 hipError_t launch_fill_u64(unsigned long long *p, size_t count, hipStream_t stream);
+hipError_t launch_selftest_math(unsigned long long *counts, unsigned int *first, hipStream_t stream);
 
 // Algorithmic FP32 flops per ray-primitive test (counted from the reference
 // formulas: FMA = 2, mul/add = 1, rcp/sqrt = 1, compare/select = 0).

@@ -175,9 +175,31 @@ __device__ __forceinline__ FastPrim vcopy(const FastPrim &q)
     return v;
 }
 
+// One primitive type against the NR rays of find_c_multi: FastOps first; a
+// ray whose operands left FastOps' exact range repeats the test with IeeeOps.
+template <int TYPE, int NR>
+__device__ __forceinline__ void scan_c_type(const float (&m)[12], float u1, float u2, const Ray32 (&ray)[NR],
+                                            float (&t)[NR])
+{
+    bool redo[NR];
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        FastOps ops;
+        t[r] = hit_c_ops<TYPE>(m, u1, u2, ray[r], ops);
+        redo[r] = !ops.ok;
+        any |= redo[r];
+    }
+    if (__builtin_amdgcn_ballot_w64(any)) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            if (redo[r]) t[r] = hit_c_t<TYPE>(m, u1, u2, ray[r]);
+    }
+}
+
 // Compat scan of the NR rays of one work item: each record is read once,
 // copied to VGPRs and tested against every ray with the reference's
-// arithmetic (hit_c_t); caller order and strict < keep the reference's tie
+// arithmetic (hit_c_ops, IEEE-exact); caller order and strict < keep the reference's tie
 // rule (RayFactorGPU.cl:445-472), so each ray's result is find_c's.
 template <int NR>
 __device__ __forceinline__ void find_c_multi(CPTR(rf_primitive) P, int start, int end, const Ray32 (&ray)[NR],
@@ -193,34 +215,12 @@ __device__ __forceinline__ void find_c_multi(CPTR(rf_primitive) P, int start, in
         for (int k = 0; k < 12; k += 2) vcopy2(p.invm[k], p.invm[k + 1], m[k], m[k + 1]);
         float t[NR];
         switch (p.type) {
-        case RF_CYLINDER:
-#pragma unroll
-            for (int r = 0; r < NR; ++r) t[r] = hit_c_t<RF_CYLINDER>(m, 0.f, 0.f, ray[r]);
-            break;
-        case RF_FRUSTUM: {
-            const float u1 = vcopy(p.util1);
-#pragma unroll
-            for (int r = 0; r < NR; ++r) t[r] = hit_c_t<RF_FRUSTUM>(m, u1, 0.f, ray[r]);
-            break;
-        }
-        case RF_SPHERE:
-#pragma unroll
-            for (int r = 0; r < NR; ++r) t[r] = hit_c_t<RF_SPHERE>(m, 0.f, 0.f, ray[r]);
-            break;
-        case RF_ANNULUS: {
-            const float u1 = vcopy(p.util1), u2 = vcopy(p.util2);
-#pragma unroll
-            for (int r = 0; r < NR; ++r) t[r] = hit_c_t<RF_ANNULUS>(m, u1, u2, ray[r]);
-            break;
-        }
-        case RF_RECTANGLE:
-#pragma unroll
-            for (int r = 0; r < NR; ++r) t[r] = hit_c_t<RF_RECTANGLE>(m, 0.f, 0.f, ray[r]);
-            break;
-        case RF_DISC:
-#pragma unroll
-            for (int r = 0; r < NR; ++r) t[r] = hit_c_t<RF_DISC>(m, 0.f, 0.f, ray[r]);
-            break;
+        case RF_CYLINDER: scan_c_type<RF_CYLINDER, NR>(m, 0.f, 0.f, ray, t); break;
+        case RF_FRUSTUM: scan_c_type<RF_FRUSTUM, NR>(m, vcopy(p.util1), 0.f, ray, t); break;
+        case RF_SPHERE: scan_c_type<RF_SPHERE, NR>(m, 0.f, 0.f, ray, t); break;
+        case RF_ANNULUS: scan_c_type<RF_ANNULUS, NR>(m, vcopy(p.util1), vcopy(p.util2), ray, t); break;
+        case RF_RECTANGLE: scan_c_type<RF_RECTANGLE, NR>(m, 0.f, 0.f, ray, t); break;
+        case RF_DISC: scan_c_type<RF_DISC, NR>(m, 0.f, 0.f, ray, t); break;
         default:
 #pragma unroll
             for (int r = 0; r < NR; ++r) t[r] = finf();
@@ -1340,8 +1340,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REFINE
 // empty (all waves reach the same exit).
 // Four rays per thread with the FP64 stage fits 96 VGPRs (five waves per
 // SIMD) when asked to; left alone it takes a few more and drops to four.
+// The compat scan (four rays per visit) is held to the same five waves.
 template <int MATH, bool BV, bool TRACE, int NR, bool REFINE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 4 ? 5 : 1))) k_queue(KParams kp)
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu((NR == 4 || (MATH == KM_COMPAT && !BV)) ? 5 : 1))) k_queue(KParams kp)
 {
     // dynamic LDS: [bins: kp.bins x u32][pending re-solves: PEND_K x blockDim x u32]
     extern __shared__ uint32_t smem[];
@@ -1419,6 +1421,55 @@ __global__ void k_fill_u64(unsigned long long *p, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// Exhaustive check of the compat reciprocal / square root: every float bit
+// pattern, each compared with the compiler's IEEE operation (bit for bit, or
+// both NaN).  counts[k] = mismatches of check k, first[k] = lowest mismatching
+// input bits:
+//   0 FastOps::rcp where it reports ok, vs 1.0f / x            (must be 0)
+//   1 FastOps::sqrt where it reports ok, vs sqrtf(x)            (must be 0)
+//   2 recip_nr on every input vs 1.0f / x                       (information)
+//   3 v_sqrt_f32 alone vs sqrtf(x), for normal x > 0            (information)
+// ---------------------------------------------------------------------------
+constexpr int SELFTEST_PER_THREAD = 256;
+
+__device__ __forceinline__ bool same_bits(float a, float b)
+{
+    return __float_as_uint(a) == __float_as_uint(b) || (__builtin_isnan(a) && __builtin_isnan(b));
+}
+
+__global__ __launch_bounds__(256) void k_selftest_math(unsigned long long *counts, unsigned int *first)
+{
+    const uint64_t base = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * SELFTEST_PER_THREAD;
+    uint32_t n[4] = {0, 0, 0, 0};
+    uint32_t lo[4] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
+    for (int i = 0; i < SELFTEST_PER_THREAD; ++i) {
+        const uint32_t bits = (uint32_t)(base + i);
+        const float x = __uint_as_float(bits);
+        const float ri = recip_ieee(x), si = sqrt_ieee(x);
+        bool bad[4];
+        FastOps fr, fs;
+        const float rf = fr.rcp(x), sf = fs.sqrt(x);
+        bad[0] = fr.ok && !same_bits(rf, ri);
+        bad[1] = fs.ok && !same_bits(sf, si);
+        bad[2] = !same_bits(recip_nr(x), ri);
+        bad[3] = __builtin_amdgcn_class(x, 0x100) && !same_bits(__builtin_amdgcn_sqrtf(x), si);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (bad[k]) { ++n[k]; lo[k] = bits < lo[k] ? bits : lo[k]; }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (n[k]) { atomicAdd(&counts[k], (unsigned long long)n[k]); atomicMin(&first[k], lo[k]); }
+}
+
+hipError_t launch_selftest_math(unsigned long long *counts, unsigned int *first, hipStream_t stream)
+{
+    // 2^32 inputs: 2^16 groups x 256 threads x 256 inputs
+    hipLaunchKernelGGL(k_selftest_math, dim3(1u << 16), dim3(256), 0, stream, counts, first);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_u32(unsigned int *p, unsigned int value, size_t count, hipStream_t stream)

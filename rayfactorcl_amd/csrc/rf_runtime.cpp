@@ -215,6 +215,27 @@ extern "C" int32_t rf_device_info(const rf_ctx *c, char *name, size_t len, int32
     return RF_OK;
 }
 
+extern "C" int32_t rf_selftest_math(rf_ctx *ctx, uint64_t mismatches[4], uint32_t first[4])
+{
+    if (!ctx || !mismatches || !first) return set_err(ctx, RF_ERR_INVALID_ARG, "NULL argument");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int rc = fan_join(ctx);
+    if (rc) return rc;
+    unsigned long long *d = nullptr;
+    HIPCHK(ctx, hipMalloc((void **)&d, 4 * sizeof(unsigned long long) + 4 * sizeof(unsigned)));
+    unsigned *d_first = reinterpret_cast<unsigned *>(d + 4);
+    hipError_t e = launch_fill_u64(d, 4, ctx->stream);
+    if (e == hipSuccess) e = launch_fill_u32(d_first, 0xffffffffu, 4, ctx->stream);
+    if (e == hipSuccess) e = launch_selftest_math(d, d_first, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(mismatches, d, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(first, d_first, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return set_err(ctx, RF_ERR_HIP, std::string("selftest: ") + hipGetErrorString(e));
+    return RF_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Scene upload
 // ---------------------------------------------------------------------------

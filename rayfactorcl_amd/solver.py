@@ -218,6 +218,16 @@ def normalise(hits: np.ndarray, rays: np.ndarray) -> np.ndarray:
     return vf
 
 
+def selftest_math(ctx: Context) -> dict:
+    """Exhaustive device check of the compat reciprocal and square root
+    against IEEE 1/x and sqrt (all 2^32 inputs)."""
+    mism = np.zeros(4, dtype=np.uint64)
+    first = np.zeros(4, dtype=np.uint32)
+    check(lib().rf_selftest_math(ctx.handle, ptr(mism), ptr(first)), "rf_selftest_math", ctx.handle)
+    names = ["recip_c", "sqrt_c", "recip_newton_unguarded", "v_sqrt_unadjusted"]
+    return {k: {"mismatches": int(m), "first": int(f)} for k, m, f in zip(names, mism, first)}
+
+
 def areas(prims: np.ndarray) -> np.ndarray:
     """Surface area of each primitive (object shape under its m transform, by quadrature)."""
     prims = np.ascontiguousarray(prims, dtype=PRIM_DTYPE)

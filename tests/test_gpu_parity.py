@@ -14,6 +14,16 @@ def wi_for(sc):
     return 4096 if sc.n <= 8 else (1024 if sc.n <= 64 else 128)
 
 
+def test_compat_math_exhaustive(ctx):
+    """The compat kernels reach IEEE 1/x and sqrt through short hardware
+    sequences with an IEEE fallback; over all 2^32 inputs they must give the
+    compiler's IEEE results bit for bit (NaN for NaN)."""
+    r = rf.selftest_math(ctx)
+    print(r, flush=True)
+    assert r["recip_c"]["mismatches"] == 0, r
+    assert r["sqrt_c"]["mismatches"] == 0, r
+
+
 @pytest.mark.parametrize("name", SCENE_NAMES)
 @pytest.mark.parametrize("math", ["compat", "fp64"])
 def test_hits_match_oracle(ctx, oracle, golden_scene, name, math):
