@@ -116,7 +116,12 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--rays-per-thread", type=int, default=0,
+                    help="fixed launch shape (with --block-size) instead of the autotuner's")
+    ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--no-refine", action="store_true")
+    ap.add_argument("--per-test", action="store_true",
+                    help="evaluate every cylinder test in full (no band stacks; same tallies)")
     ap.add_argument("--traffic-file", default=None,
                     help="measured HBM bytes per launch of the solve kernel (JSON written from rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE passes); default profiles/hbm_traffic_<workload>.json")
@@ -138,16 +143,20 @@ def main() -> int:
     rays = int(args.rays)
     refine = not args.no_refine
     rpt, blk = 4, 256          # library defaults (VALU transform)
-    if not args.no_tune:
+    if args.rays_per_thread:
+        rpt, blk = args.rays_per_thread, args.block_size or 256
+    elif not args.no_tune:
         t0 = time.perf_counter()
         rpt, blk = rf.autotune(ctx, dev, rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays,
-                                                      refine=refine))
+                                                      refine=refine, per_test=args.per_test))
         log(f"autotune: {'MFMA transform' if rpt == 0 else f'{rpt} rays/thread'}, {blk} threads/group "
             f"({time.perf_counter() - t0:.1f} s)")
     kernel = (f"rf::k_queue_mfma<{str(refine).lower()}, false>" if rpt == 0 else
               f"rf::k_queue<2, false, false, {rpt}, {str(refine).lower()}>")
     opts = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", refine=refine,
-                        seed=args.seed)
+                        seed=args.seed, per_test=args.per_test,
+                        rays_per_thread=rpt if args.rays_per_thread else 0,
+                        block_size=blk if args.rays_per_thread else 0)
     for i in range(args.warmup):
         _, t = rf.batch_solve(ctx, dev, opts, readback=False)
         log(f"warmup {i}: {t['kernel_ms']:.1f} ms kernel")
@@ -201,6 +210,18 @@ def main() -> int:
         dt = time.perf_counter() - s0
         res["per_geometry"] = {"tests_per_s": tests / dt, "wall_ms": dt * 1e3, "launches": launches_g}
         log(f"sample per_geometry: {res['per_geometry']['tests_per_s']:.3e} tests/s, {launches_g} launches")
+        # the same batched solve with every cylinder test in full (no band stacks),
+        # in that form's best shape (4 rays per thread, 256-thread groups)
+        opt = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", refine=refine,
+                           seed=args.seed, per_test=True, rays_per_thread=4, block_size=256, transform="valu")
+        rf.batch_solve(ctx, sub, opt, readback=False)
+        s0 = time.perf_counter()
+        _, t = rf.batch_solve(ctx, sub, opt, readback=False)
+        dt = time.perf_counter() - s0
+        res["per_test"] = {"tests_per_s": t["tests"] / dt, "kernel_ms": t["kernel_ms"],
+                           "tflops": t["flops"] / (t["kernel_ms"] * 1e-3) / 1e12}
+        log(f"sample per_test: {res['per_test']['tests_per_s']:.3e} tests/s, "
+            f"{res['per_test']['tflops']:.1f} TFLOP/s")
         plain = {
             "geometries": len(sub),
             "batched_tests_per_s": res["batched"]["tests_per_s"],
@@ -210,13 +231,20 @@ def main() -> int:
             "batched_over_per_emitter": res["batched"]["tests_per_s"] / res["per_emitter"]["tests_per_s"],
             "batched_over_reference_form": res["batched"]["tests_per_s"] / res["reference_form"]["tests_per_s"],
             "batched_over_one_geometry_per_launch": res["batched"]["tests_per_s"] / res["per_geometry"]["tests_per_s"],
+            "per_test_form_tests_per_s": res["per_test"]["tests_per_s"],
+            "per_test_form_frac": res["per_test"]["tflops"] / PEAK_FP32_TFLOPS,
+            "batched_over_per_test_form": res["batched"]["tests_per_s"] / res["per_test"]["tests_per_s"],
         }
 
     traffic = None
-    tfile = Path(args.traffic_file) if args.traffic_file else ROOT / "profiles" / f"hbm_traffic_{args.workload}{args.geoms if args.workload == 'sweep' else ''}.json"
+    form = "per_test" if args.per_test else "stacked"
+    tname = f"hbm_traffic_{args.workload}{args.geoms if args.workload == 'sweep' else ''}"
+    tfile = Path(args.traffic_file) if args.traffic_file else ROOT / "profiles" / (
+        tname + ("_per_test" if args.per_test else "") + ".json")
     if tfile.exists():
         rec = json.loads(tfile.read_text())
-        if rec.get("kernel") == kernel and int(rec.get("rays_per_emitter", -1)) == rays:
+        if (rec.get("kernel") == kernel and int(rec.get("rays_per_emitter", -1)) == rays
+                and rec.get("form", "per_test") == form):
             traffic = rec["hbm_bytes_per_launch"]
             log(f"traffic: {traffic:.3e} B per launch from {tfile.name}")
         else:

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define RF_ABI_VERSION 1
+#define RF_ABI_VERSION 2
 
 /* ------------------------------------------------------------------ */
 /* Scene ABI                                                           */
@@ -123,6 +123,10 @@ typedef struct rf_solve_opts {
     int32_t transform;        /* RF_MATH_FAST plain form: rf_transform */
     float refine_margin;      /* RF_MATH_FAST: relative margin of the FP64 re-solve flags, 0 = 2^-14;
                                  larger re-solves more pairs (at most 0.25) */
+    int32_t per_test;         /* RF_MATH_FAST plain form: 0 = a band stack (consecutive cylinders whose
+                                 transforms differ only in the z offset, e.g. the bands of one wall)
+                                 solves its quadric once per ray and tests each band's z range;
+                                 1 = every cylinder test in full.  Same tallies either way. */
 } rf_solve_opts;
 
 typedef enum rf_transform {
@@ -142,7 +146,8 @@ typedef struct rf_timing {
     double wall_ms;           /* host steady clock around the whole call */
     int64_t rays;             /* rays traced */
     int64_t tests;            /* ray-primitive tests executed (counted on the device for BV) */
-    int64_t flops;            /* algorithmic FP32 flops: sum over tests of F_type */
+    int64_t flops;            /* algorithmic FP32 flops: sum over tests of F_type; a stacked band
+                                 test counts F_band, plus F_stack per (ray, band stack) */
     int64_t refined;          /* (ray, primitive) pairs re-solved in FP64 */
     int64_t launches;         /* kernel launches issued */
     int64_t rescans;          /* lane re-scans after a full FP64 re-solve list (no flagged pair is left unsolved) */

@@ -50,7 +50,7 @@ class SolveOpts(ctypes.Structure):
         ("n_samples", c_int32), ("form", c_int32), ("rays_per_emitter", c_int64),
         ("seed", c_uint32), ("geometry", c_uint32), ("refine", c_int32),
         ("rays_per_thread", c_int32), ("block_size", c_int32), ("transform", c_int32),
-        ("refine_margin", c_float),
+        ("refine_margin", c_float), ("per_test", c_int32),
     ]
 
 

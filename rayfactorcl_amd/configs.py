@@ -124,7 +124,7 @@ def config3(ctx, quick: bool) -> dict:
             "differing_rays": frac, "rays_compared": nr}
 
 
-def config5(ctx, quick: bool, n_req: int = 0, load: float = 0.7, throughput: float = 1.1e12) -> dict:
+def config5(ctx, quick: bool, n_req: int = 0, load: float = 0.7, throughput: float = 2.5e12) -> dict:
     """Poisson arrivals at `load` x throughput; requests of 10-2000 primitives.
     Ray budgets are log-U[1e5, 1e6] (1e5-1e8 in the configuration) so that the
     run stays within minutes; the budget range is part of the record."""

@@ -20,6 +20,26 @@ struct BvMeta {
     int32_t type;
 };
 
+// Runs of the fast form's cylinder segment, in traversal order.  A stack
+// (stack = 1) is a run of two or more consecutive cylinders whose folded
+// transform rows are bitwise equal except the z-row offset r[11] -- the bands
+// of one coaxial wall: every band shares the ray's x-y quadric and its z
+// direction, so the kernel solves the quadric once per ray and tests each
+// band's z range.  Other runs (stack = 0) are scanned one test at a time.
+struct CylRun {
+    int32_t begin;
+    int32_t end;
+    int32_t stack;
+    int32_t pad;
+};
+
+// Per-position band record of a stack: the z-row offset and the primitive's
+// index in the caller's order.
+struct BandRec {
+    float r11;
+    int32_t orig;
+};
+
 // Device-side descriptor of one geometry (all pointers are device memory).
 struct GeomDesc {
     const rf_primitive *prims;          // caller's order (AoS, 112 B)
@@ -49,7 +69,18 @@ struct GeomDesc {
     const rfd::FastPrim *bvc_fast;
     const int32_t *bvc_seg;
     int32_t bv_binned;                  // 1: no BV asks for the intermediate check (settings != 1)
-    int32_t pad2;
+    int32_t n_cyl_runs;
+    const CylRun *cyl_runs;             // cover [seg_begin[SEG_CYL], seg_end[SEG_CYL]) in order
+    const BandRec *bands;               // indexed by traversal position (cylinder positions only);
+                                        // bands[-1] is a pad record
+    // binned BV form with band stacks: runs of the children's cylinder
+    // segment of surface b are bvc_runs[bvc_run_off[b] .. bvc_run_off[b+1]),
+    // over bvc_fast positions; bvc_bands as bands; bv_cflops[2b + s] the
+    // children's algorithmic flops per ray, per test (s = 0) or stacked (s = 1)
+    const CylRun *bvc_runs;
+    const int32_t *bvc_run_off;
+    const BandRec *bvc_bands;
+    const uint32_t *bv_cflops;
 };
 
 // One work unit: work items [wi0, wi0+n_wi) of one emitter of one geometry.
@@ -65,7 +96,8 @@ struct KParams {
     const GeomDesc *geoms;
     const WorkUnit *units;              // nullptr: implicit unit list (below)
     unsigned int *queue_head;           // atomic unit counter (zeroed before launch)
-    unsigned long long *stats;          // [0] tests executed (BV kernels), [1] FP64 re-solves
+    unsigned long long *stats;          // [0] tests executed (BV kernels), [1] FP64 re-solves,
+                                        // [2] re-scans, [3] children's flops (binned BV kernel)
     int32_t *trace_out;                 // per-ray hit index: ray 4*(wi - trace_wi0) + v
     int64_t trace_wi0;
     int32_t n_units;
@@ -83,6 +115,7 @@ struct KParams {
     int64_t imp_unit;
     float tau;                          // fast form: relative margin of the FP64 re-solve flags
     int32_t bv_slots;                   // binned BV kernel: LDS buckets (>= n_bvs of every geometry)
+    int32_t stacks;                     // fast plain form: 1 = band stacks share the quadric (CylRun)
 };
 
 enum KMath { KM_COMPAT = 0, KM_FP64 = 1, KM_FAST = 2 };
@@ -105,6 +138,13 @@ hipError_t launch_solve(const LaunchCfg &cfg, const KParams &kp, hipStream_t str
 hipError_t launch_fill_u32(unsigned int *p, unsigned int value, size_t count, hipStream_t stream);
 hipError_t launch_fill_u64(unsigned long long *p, size_t count, hipStream_t stream);
 hipError_t launch_selftest_math(unsigned long long *counts, unsigned int *first, hipStream_t stream);
+
+// Algorithmic FP32 flops of one band stack, same counting as flops_per_test:
+// per (ray, stack) the x-y rows of origin and direction, the z direction, A,
+// b, the cross term, D, sqrt, rcp and both roots; per (ray, band) the band's
+// z origin row and z at both roots.
+constexpr int64_t kFlopsStack = 45;
+constexpr int64_t kFlopsBand = 10;
 
 // Algorithmic FP32 flops per ray-primitive test (counted from the reference
 // formulas: FMA = 2, mul/add = 1, rcp/sqrt = 1, compare/select = 0).

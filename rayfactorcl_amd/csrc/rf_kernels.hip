@@ -452,7 +452,7 @@ __device__ __forceinline__ void seg_loop(CPTR(FastPrim) F, int begin, int end, c
 #pragma unroll
                 for (int i = 0; i < NR; ++i) {
                     if (f[i]) {
-                        if (pd.count < PEND_K) {
+                        if (pd.count < PEND_K && pd.resume == NO_KEY) {
                             pd.buf[pd.count * blockDim.x + threadIdx.x] =
                                 ((uint32_t)q.orig << 3) | (uint32_t)(RESCAN ? slot : i);
                             ++pd.count;
@@ -467,6 +467,151 @@ __device__ __forceinline__ void seg_loop(CPTR(FastPrim) F, int begin, int end, c
 #pragma unroll
         for (int i = 0; i < NR; ++i)
             if (t[i] < tb[i]) { tb[i] = t[i]; ib[i] = q.orig; }
+    }
+}
+
+// Band stack [b, e) of the cylinder segment (CylRun.stack = 1): the records
+// are bitwise equal except the z offset r[11], so each ray's x-y quadric, its
+// roots and its z direction are computed once, with the operations of
+// fast_test<SEG_CYL>, and each band then needs only its z origin row, z at
+// both roots (and at the tangent point) and the range compares.  Every
+// per-band decision is fast_test's on that band's own record, bit for bit:
+//  - a root failing t > eps becomes NaN, so every compare on its z is false
+//    (fast_test's e1 / e2 terms); the tangent point likewise when |D| > tau A;
+//  - fast_test's candidate on a band is t1 if z1 is on it, else t2 if z2 is,
+//    and t1 <= t2 (sqrt >= 0, 1/A >= 0; t1 == t2 makes z1 == z2), so the
+//    nearest hit of the stack is t1 at the first band holding z1, else t2 at
+//    the first band holding z2 -- the bands are walked last to first and the
+//    last band seen holding each root is kept -- and it enters the running
+//    minimum with the strict < of the per-test scan;
+//  - fast_test's flag terms are evaluated on the same values; its rim term
+//    (|z| <= hi) ^ (|z| < lo) is lo <= |z| <= hi, here med3(|z|, lo, hi) == |z|
+//    (one vector compare instead of a compare and a scalar xor).
+// A flagged pair is queued as in seg_loop.  If the lane's list fills inside the
+// stack, the entries it queued from this stack are given back and the re-scan
+// starts at the stack's first band (resume = b x 8), so every flagged pair is
+// still re-solved exactly once.
+// Band loop of stack_loop, bands e-1 down to b.  AXIAL: the stack's z row has
+// no x or y term (r[8] == r[9] == 0), so the band's z origin is one FMA; the
+// per-test scan's two extra FMAs add a signed zero, which no |z| compare can
+// tell apart.  TAN: some ray of the wave is near-tangent (finite TM), so the
+// tangent-point flag is evaluated; otherwise it is false for every ray.
+template <int NR, bool REFINE, bool AXIAL, bool TAN>
+__device__ __forceinline__ void stack_bands(const FastPrim &h, CPTR(BandRec) BR, int b, int e, const Ray32 (&R)[NR],
+                                            const float (&T1)[NR], const float (&T2)[NR], const float (&TM)[NR],
+                                            const float (&CZ)[NR], int (&I1)[NR], int (&I2)[NR], float lo, float hi,
+                                            Pending &pd)
+{
+    const int cnt0 = pd.count;
+    // BR[-1] exists (the band array has a leading pad record), so the
+    // prefetch of the next record needs no clamp
+    CPTR(BandRec) p = BR + (e - 1);
+    BandRec next = ldc(p);
+    for (int w = e - 1; w >= b; --w) {
+        float r11, of;
+        vcopy2(next.r11, __int_as_float(next.orig), r11, of);
+        const int orig = __float_as_int(of);
+        --p;
+        next = ldc(p);
+        bool f[NR];
+        bool any = false;
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const float sz = AXIAL ? __builtin_fmaf(h.r[10], R[i].sz, r11)
+                                   : __builtin_fmaf(h.r[8], R[i].sx,
+                                                    __builtin_fmaf(h.r[9], R[i].sy, __builtin_fmaf(h.r[10], R[i].sz, r11)));
+            const float a1 = __builtin_fabsf(__builtin_fmaf(CZ[i], T1[i], sz));
+            const float a2 = __builtin_fabsf(__builtin_fmaf(CZ[i], T2[i], sz));
+            bool in1, in2;
+            if constexpr (REFINE) {
+                // fast_test's terms: a root is on the band for |z| < lo and a
+                // rim flag for lo <= |z| <= hi (its (|z| <= hi) ^ (|z| < lo);
+                // med3 clamps |z| to [lo, hi], a NaN fails the equality)
+                in1 = a1 < lo;
+                in2 = a2 < lo;
+                f[i] = (__builtin_amdgcn_fmed3f(a1, lo, hi) == a1) | (__builtin_amdgcn_fmed3f(a2, lo, hi) == a2);
+                if constexpr (TAN) f[i] = f[i] | (__builtin_fabsf(__builtin_fmaf(CZ[i], TM[i], sz)) <= hi);
+                any |= f[i];
+                in1 = in1 & !f[i];
+                in2 = in2 & !f[i];
+            } else {
+                in1 = a1 <= 0.5f;
+                in2 = a2 <= 0.5f;
+            }
+            I1[i] = in1 ? orig : I1[i];
+            I2[i] = in2 ? orig : I2[i];
+            // updates stay ahead of the branch below (sunk past it, the range
+            // tests lose their |z| operand modifiers)
+            asm volatile("" ::"v"(I1[i]), "v"(I2[i]));
+        }
+        if (REFINE && __builtin_amdgcn_ballot_w64(any)) {
+#pragma unroll
+            for (int i = 0; i < NR; ++i) {
+                if (f[i]) {
+                    if (pd.count < PEND_K && pd.resume == NO_KEY) {
+                        pd.buf[pd.count * blockDim.x + threadIdx.x] = ((uint32_t)orig << 3) | (uint32_t)i;
+                        ++pd.count;
+                    } else if (pd.resume == NO_KEY) {
+                        pd.resume = b * 8;
+                        pd.count = cnt0;
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int NR, bool REFINE>
+__device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, int b, int e, const Ray32 (&R)[NR],
+                                           float (&tb)[NR], int (&ib)[NR], Pending &pd)
+{
+    const FastPrim hs = ldc(F + b);
+    const FastPrim h = vcopy(hs);
+    // wave-uniform choice of the band loop: integer tests on the scalar copy (+-0)
+    const bool axial = ((__float_as_uint(hs.r[8]) | __float_as_uint(hs.r[9])) << 1) == 0u;
+    const float tau = pd.tau, eps = pd.eps;
+    const float lo = 0.5f - tau, hi = 0.5f + tau;
+    const float qnan = __builtin_nanf("");
+    float T1[NR], T2[NR], TM[NR], CZ[NR];
+    int I1[NR], I2[NR];
+    bool tan = false;
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        const float sx = ROW_S(h, 0, R[i]), sy = ROW_S(h, 1, R[i]);
+        const float cx = ROW_C(h, 0, R[i]), cy = ROW_C(h, 1, R[i]);
+        CZ[i] = ROW_C(h, 2, R[i]);
+        const float A = __builtin_fmaf(cx, cx, cy * cy);
+        const float bq = __builtin_fmaf(sx, cx, sy * cy);
+        const float w = __builtin_fmaf(sx, cy, -(sy * cx));
+        const float D = __builtin_fmaf(-w, w, A);
+        const float sq = sqrt_fast(D);
+        const float rA = rcp_fast(A);
+        const float tm = -bq * rA;
+        const float t1 = __builtin_fmaf(-sq, rA, tm);
+        const float t2 = __builtin_fmaf(sq, rA, tm);
+        T1[i] = t1 > eps ? t1 : qnan;
+        T2[i] = t2 > eps ? t2 : qnan;
+        const bool near = REFINE && __builtin_fabsf(D) <= tau * A;
+        TM[i] = near ? tm : qnan;
+        tan |= near;
+        // materialised once: keeps the selects out of the band loop
+        asm volatile("" : "+v"(T1[i]), "+v"(T2[i]), "+v"(TM[i]), "+v"(CZ[i]));
+        I1[i] = -1;
+        I2[i] = -1;
+    }
+    if (REFINE && __builtin_amdgcn_ballot_w64(tan)) {
+        if (axial) stack_bands<NR, REFINE, true, true>(h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, pd);
+        else stack_bands<NR, REFINE, false, true>(h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, pd);
+    } else {
+        if (axial) stack_bands<NR, REFINE, true, false>(h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, pd);
+        else stack_bands<NR, REFINE, false, false>(h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, pd);
+    }
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        const bool h1 = I1[i] >= 0;
+        const float c = h1 ? T1[i] : (I2[i] >= 0 ? T2[i] : finf());
+        const int ci = h1 ? I1[i] : I2[i];
+        if (c < tb[i]) { tb[i] = c; ib[i] = ci; }
     }
 }
 
@@ -546,12 +691,22 @@ __device__ __forceinline__ void finish_dropped(const KParams &kp, CPTR(GeomDesc)
     pd.resume = NO_KEY;
 }
 
+// stacks: the cylinder segment runs as its CylRun list (band stacks share the
+// quadric); otherwise as one per-test run.
 template <int NR, bool REFINE>
 __device__ __forceinline__ void scan_fast(CPTR(GeomDesc) G, const Ray32 (&R)[NR], float (&tb)[NR],
-                                          int (&ib)[NR], Pending &pd)
+                                          int (&ib)[NR], Pending &pd, int stacks)
 {
     CPTR(FastPrim) F = cst(G->fast);
-    seg_loop<SEG_CYL, NR, REFINE, false>(F, G->seg_begin[SEG_CYL], G->seg_end[SEG_CYL], R, tb, ib, pd);
+    const int n_runs = stacks ? G->n_cyl_runs : 1;
+    for (int j = 0; j < n_runs; ++j) {
+        CylRun run{G->seg_begin[SEG_CYL], G->seg_end[SEG_CYL], 0, 0};
+        if (stacks) run = ldc(cst(G->cyl_runs) + j);
+        if (run.stack)
+            stack_loop<NR, REFINE>(F, cst(G->bands), run.begin, run.end, R, tb, ib, pd);
+        else
+            seg_loop<SEG_CYL, NR, REFINE, false>(F, run.begin, run.end, R, tb, ib, pd);
+    }
     seg_loop<SEG_FRUSTUM, NR, REFINE, false>(F, G->seg_begin[SEG_FRUSTUM], G->seg_end[SEG_FRUSTUM], R, tb, ib, pd);
     seg_loop<SEG_SPHERE, NR, REFINE, false>(F, G->seg_begin[SEG_SPHERE], G->seg_end[SEG_SPHERE], R, tb, ib, pd);
     seg_loop<SEG_RING, NR, REFINE, false>(F, G->seg_begin[SEG_RING], G->seg_end[SEG_RING], R, tb, ib, pd);
@@ -687,7 +842,7 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
 #pragma unroll
             for (int i = 0; i < NR; ++i) { tb[i] = finf(); ib[i] = -1; }
             Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
-            scan_fast<NR, REFINE>(G, R, tb, ib, pd);
+            scan_fast<NR, REFINE>(G, R, tb, ib, pd, kp.stacks);
             if constexpr (REFINE) {
                 const auto ray_of = [base](int k) { return base + k; };
                 if (__builtin_amdgcn_ballot_w64(pd.count > 0))
@@ -742,7 +897,7 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                         float tb1[1] = {finf()};
                         int ib1[1] = {-1};
                         Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
-                        scan_fast<1, REFINE>(G, R1, tb1, ib1, pd);
+                        scan_fast<1, REFINE>(G, R1, tb1, ib1, pd, kp.stacks);
                         if constexpr (REFINE) {
                             const int64_t r0 = wl * 4 + v;
                             const auto ray_of = [r0](int k) { return r0 + k; };
@@ -762,17 +917,20 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
 }
 
 // stats[0]: tests counted on the device (BV forms); stats[1]: pairs re-solved
-// in FP64; stats[2]: lane re-scans after a full re-solve list.
-__device__ __forceinline__ void flush_stats(const KParams &kp, uint64_t tests, uint32_t nref, uint32_t nover)
+// in FP64; stats[2]: lane re-scans after a full re-solve list; stats[3]:
+// algorithmic flops of the binned BV kernel's child scans.
+__device__ __forceinline__ void flush_stats(const KParams &kp, uint64_t tests, uint32_t nref, uint32_t nover,
+                                            uint64_t flops = 0)
 {
-    __shared__ unsigned long long s_stats[3];
-    if (threadIdx.x < 3) s_stats[threadIdx.x] = 0;
+    __shared__ unsigned long long s_stats[4];
+    if (threadIdx.x < 4) s_stats[threadIdx.x] = 0;
     __syncthreads();
     if (tests) atomicAdd(&s_stats[0], (unsigned long long)tests);
     if (nref) atomicAdd(&s_stats[1], (unsigned long long)nref);
     if (nover) atomicAdd(&s_stats[2], (unsigned long long)nover);
+    if (flops) atomicAdd(&s_stats[3], (unsigned long long)flops);
     __syncthreads();
-    if (threadIdx.x < 3 && kp.stats && s_stats[threadIdx.x]) atomicAdd(&kp.stats[threadIdx.x], s_stats[threadIdx.x]);
+    if (threadIdx.x < 4 && kp.stats && s_stats[threadIdx.x]) atomicAdd(&kp.stats[threadIdx.x], s_stats[threadIdx.x]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1119,8 +1277,8 @@ template <bool REFINE, bool TRACE>
 __device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc) G, uint32_t gid, int emitter,
                                                int64_t wi0, int64_t n_wi, uint32_t *bins, uint32_t *pend,
                                                uint32_t *ids, uint32_t *cnt, uint32_t *off, uint32_t *cur,
-                                               uint32_t *pad, int *s_total, uint64_t &tests, uint32_t &nref,
-                                               uint32_t &nover)
+                                               uint32_t *pad, int *s_total, uint64_t &tests, uint64_t &cflops,
+                                               uint32_t &nref, uint32_t &nover)
 {
     const rf_primitive E = ldc(cst(G->prims) + emitter);
     const int64_t ray0 = (wi0 - kp.trace_wi0) * 4;
@@ -1250,7 +1408,20 @@ __device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc)
             }
             CPTR(int32_t) sg = cst(G->bvc_seg) + b * (N_SEG + 1);
             Pending pd{pend, 0, NO_KEY, 0, tau, eps};
-            seg_loop<SEG_CYL, BNR, REFINE, false>(FC, sg[SEG_CYL], sg[SEG_CYL + 1], R, tb, ib, pd);
+            if (kp.stacks) {
+                // the children's cylinder runs: band stacks share the quadric
+                CPTR(int32_t) ro = cst(G->bvc_run_off) + b;
+                const int j1 = ro[1];
+                for (int j = ro[0]; j < j1; ++j) {
+                    const CylRun run = ldc(cst(G->bvc_runs) + j);
+                    if (run.stack)
+                        stack_loop<BNR, REFINE>(FC, cst(G->bvc_bands), run.begin, run.end, R, tb, ib, pd);
+                    else
+                        seg_loop<SEG_CYL, BNR, REFINE, false>(FC, run.begin, run.end, R, tb, ib, pd);
+                }
+            } else {
+                seg_loop<SEG_CYL, BNR, REFINE, false>(FC, sg[SEG_CYL], sg[SEG_CYL + 1], R, tb, ib, pd);
+            }
             seg_loop<SEG_FRUSTUM, BNR, REFINE, false>(FC, sg[SEG_FRUSTUM], sg[SEG_FRUSTUM + 1], R, tb, ib, pd);
             seg_loop<SEG_SPHERE, BNR, REFINE, false>(FC, sg[SEG_SPHERE], sg[SEG_SPHERE + 1], R, tb, ib, pd);
             seg_loop<SEG_RING, BNR, REFINE, false>(FC, sg[SEG_RING], sg[SEG_RING + 1], R, tb, ib, pd);
@@ -1262,10 +1433,12 @@ __device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc)
                     finish_dropped<BNR>(kp, G, E, gid, emitter, wi0, ray_of, FC, sg, sg + 1, tb, ib, pd, nref, nover);
             }
             const BvMeta meta = ldc(M + b);
+            const uint32_t cfl = cst(G->bv_cflops)[2 * b + (kp.stacks ? 1 : 0)];
 #pragma unroll
             for (int i = 0; i < BNR; ++i) {
                 if (valid[i]) {
                     tests += (uint64_t)(meta.end - meta.start);
+                    cflops += cfl;
                     tally<TRACE, false>(kp, G, emitter, ray0 + pass0 + lid[i], ib[i], bins);
                 }
             }
@@ -1291,7 +1464,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REFINE
     uint32_t *cur = off + kp.bv_slots;
     uint32_t *pad = cur + kp.bv_slots;
     __shared__ int s_unit, s_total;
-    uint64_t tests = 0;
+    uint64_t tests = 0, cflops = 0;
     uint32_t nref = 0, nover = 0;
     const bool lds_tally = !TRACE && kp.bins > 0;
     for (;;) {
@@ -1322,7 +1495,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REFINE
             __syncthreads();
         }
         run_unit_bvbin<REFINE, TRACE>(kp, G, gid, emitter, wi0, n_wi, bins, pend, ids, cnt, off, cur, pad, &s_total,
-                                      tests, nref, nover);
+                                      tests, cflops, nref, nover);
         if (lds_tally) {
             __syncthreads();
             unsigned long long *row = G->hits + (int64_t)emitter * n;
@@ -1333,7 +1506,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REFINE
         }
         __syncthreads();
     }
-    flush_stats(kp, tests, nref, nover);
+    flush_stats(kp, tests, nref, nover, cflops);
 }
 
 // Persistent queue kernel: every workgroup pulls units until the queue is

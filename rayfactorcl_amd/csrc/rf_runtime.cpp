@@ -82,9 +82,17 @@ struct rf_scene {
     int32_t *d_pos2orig = nullptr, *d_orig2pos = nullptr;
     FastPrim *d_bvc_fast = nullptr;      // binned BV form: children per BV, type-segmented
     int32_t *d_bvc_seg = nullptr;
+    CylRun *d_cyl_runs = nullptr;        // fast form: cylinder runs (band stacks)
+    BandRec *d_bands = nullptr;
     GeomDesc *d_desc = nullptr;          // single-geometry descriptor
     GeomDesc desc{};                     // host copy
     int64_t flops_per_ray = 0;           // sum over primitives of F_type (plain kernels)
+    int64_t flops_per_ray_stacked = 0;   // the same with band stacks (kFlopsStack / kFlopsBand)
+    int64_t flops_bvs = 0;               // sum over bounding surfaces of F_type (BV tests per ray)
+    CylRun *d_bvc_runs = nullptr;        // binned BV form: cylinder runs of each surface's children
+    int32_t *d_bvc_run_off = nullptr;
+    BandRec *d_bvc_bands = nullptr;
+    uint32_t *d_bv_cflops = nullptr;
     double flops_per_test_avg = 0.0;     // BV accounting
 };
 
@@ -311,12 +319,43 @@ static void quad_rows(const rf_primitive *p, float rows[4][4])
     }
 }
 
+// Cylinder runs over records [cb, ce) of a type-segmented array: maximal
+// groups of consecutive cylinders whose folded rows other than the z offset
+// r[11] are bitwise equal; two or more form a band stack, consecutive
+// singletons one per-test run.  Returns the change of algorithmic flops per
+// ray that the stacks bring (kFlopsStack + L kFlopsBand instead of L tests).
+static int64_t build_cyl_runs(const std::vector<FastPrim> &recs, int32_t cb, int32_t ce, std::vector<CylRun> &runs)
+{
+    int64_t delta = 0;
+    auto stack_end = [&](int32_t p) {
+        int32_t q = p + 1;
+        while (q < ce && std::memcmp(recs[q].r, recs[p].r, 11 * sizeof(float)) == 0) ++q;
+        return q;
+    };
+    int32_t p = cb;
+    while (p < ce) {
+        int32_t q = stack_end(p);
+        if (q - p >= 2) {
+            runs.push_back(CylRun{p, q, 1, 0});
+            delta += kFlopsStack + (q - p) * (kFlopsBand - flops_per_test(RF_CYLINDER));
+            p = q;
+            continue;
+        }
+        const int32_t s0 = p;
+        while (p < ce && (q = stack_end(p)) - p < 2) p = q;
+        runs.push_back(CylRun{s0, p, 0, 0});
+    }
+    return delta;
+}
+
 extern "C" int64_t rf_scene_footprint(int32_t n, int32_t nbv)
 {
     if (n < 0 || nbv < 0) return -1;
-    return (int64_t)n * (sizeof(rf_primitive) + 2 * sizeof(FastPrim) + 64 + 8) + 5 * 256 +
-           (int64_t)nbv * (sizeof(rf_bounding_volume) + sizeof(FastPrim) + sizeof(BvMeta) + (N_SEG + 1) * 4) +
-           (nbv ? (int64_t)n * sizeof(FastPrim) : 0) +
+    return (int64_t)n * (sizeof(rf_primitive) + 2 * sizeof(FastPrim) + 64 + 8 + sizeof(BandRec) +
+                         sizeof(CylRun)) + 6 * 256 +
+           (int64_t)nbv * (sizeof(rf_bounding_volume) + sizeof(FastPrim) + sizeof(BvMeta) + (N_SEG + 1) * 4 +
+                           4 + 8) +
+           (nbv ? (int64_t)n * (sizeof(FastPrim) + sizeof(BandRec) + sizeof(CylRun)) + 6 * 256 : 0) +
            (int64_t)n * n * 8 + (int64_t)sizeof(GeomDesc);
 }
 
@@ -324,6 +363,9 @@ static void free_scene_mem(rf_scene *s)
 {
     (void)hipFree(s->d_quads); (void)hipFree(s->d_pos2orig); (void)hipFree(s->d_orig2pos);
     (void)hipFree(s->d_bvc_fast); (void)hipFree(s->d_bvc_seg);
+    (void)hipFree(s->d_cyl_runs); (void)hipFree(s->d_bands);
+    (void)hipFree(s->d_bvc_runs); (void)hipFree(s->d_bvc_run_off); (void)hipFree(s->d_bvc_bands);
+    (void)hipFree(s->d_bv_cflops);
     (void)hipFree(s->d_prims); (void)hipFree(s->d_fast); (void)hipFree(s->d_fast_orig);
     (void)hipFree(s->d_fast_bvs); (void)hipFree(s->d_bvs); (void)hipFree(s->d_bv_meta);
     (void)hipFree(s->d_hits); (void)hipFree(s->d_desc);
@@ -429,8 +471,38 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
             bvc_seg[(size_t)b * (N_SEG + 1) + N_SEG] = (int32_t)bvc.size();
         }
     }
+    // band stacks of the plain scan; band records by traversal position after
+    // one pad record (the kernel's prefetch reads the record before a stack)
+    std::vector<CylRun> runs;
+    const int64_t fstack = fsum + build_cyl_runs(fast, seg_begin[SEG_CYL], seg_end[SEG_CYL], runs);
+    std::vector<BandRec> bands(n + 1, BandRec{0.f, 0});
+    for (int32_t k = seg_begin[SEG_CYL]; k < seg_end[SEG_CYL]; ++k) bands[k + 1] = BandRec{fast[k].r[11], fast[k].orig};
+    // the same for the children of each bounding surface (binned BV form), and
+    // the children's flops per ray (per test, stacked)
+    std::vector<CylRun> bvc_runs;
+    std::vector<int32_t> bvc_run_off;
+    std::vector<BandRec> bvc_bands;
+    std::vector<uint32_t> bv_cflops;
+    if (binned) {
+        bvc_run_off.resize(nbv + 1);
+        bv_cflops.resize(2 * (size_t)nbv);
+        bvc_bands.assign(bvc.size() + 1, BandRec{0.f, 0});
+        for (int32_t b = 0; b < nbv; ++b) {
+            const int32_t *sg = &bvc_seg[(size_t)b * (N_SEG + 1)];
+            bvc_run_off[b] = (int32_t)bvc_runs.size();
+            const int64_t d = build_cyl_runs(bvc, sg[SEG_CYL], sg[SEG_CYL + 1], bvc_runs);
+            int64_t f = 0;
+            for (int32_t k = sg[0]; k < sg[N_SEG]; ++k) f += flops_per_test(bvc[k].type);
+            bv_cflops[2 * b] = (uint32_t)f;
+            bv_cflops[2 * b + 1] = (uint32_t)(f + d);
+            for (int32_t k = sg[SEG_CYL]; k < sg[SEG_CYL + 1]; ++k) bvc_bands[k + 1] = BandRec{bvc[k].r[11], bvc[k].orig};
+        }
+        bvc_run_off[nbv] = (int32_t)bvc_runs.size();
+    }
     s->flops_per_ray = fsum;
+    s->flops_per_ray_stacked = fstack;
     s->flops_per_test_avg = (double)(fsum + fbv) / (double)(n + nbv);
+    s->flops_bvs = fbv;
 
     hipError_t e = hipSuccess;
     auto up = [&](void **dst, const void *src, size_t bytes) {
@@ -449,7 +521,13 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     up((void **)&s->d_quads, quads.data(), quads.size() * sizeof(float));
     up((void **)&s->d_pos2orig, pos2orig.data(), pos2orig.size() * sizeof(int32_t));
     up((void **)&s->d_orig2pos, orig2pos.data(), orig2pos.size() * sizeof(int32_t));
+    up((void **)&s->d_cyl_runs, runs.data(), runs.size() * sizeof(CylRun));
+    up((void **)&s->d_bands, bands.data(), bands.size() * sizeof(BandRec));
     if (binned) {
+        up((void **)&s->d_bvc_runs, bvc_runs.data(), bvc_runs.size() * sizeof(CylRun));
+        up((void **)&s->d_bvc_run_off, bvc_run_off.data(), bvc_run_off.size() * sizeof(int32_t));
+        up((void **)&s->d_bvc_bands, bvc_bands.data(), bvc_bands.size() * sizeof(BandRec));
+        up((void **)&s->d_bv_cflops, bv_cflops.data(), bv_cflops.size() * sizeof(uint32_t));
         up((void **)&s->d_bvc_fast, bvc.data(), bvc.size() * sizeof(FastPrim));
         up((void **)&s->d_bvc_seg, bvc_seg.data(), bvc_seg.size() * sizeof(int32_t));
     }
@@ -463,6 +541,11 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     d.quads = s->d_quads; d.pos2orig = s->d_pos2orig; d.orig2pos = s->d_orig2pos;
     for (int k = 0; k < N_SEG; ++k) { d.qseg_begin[k] = qseg_begin[k]; d.qseg_end[k] = qseg_end[k]; }
     d.bvc_fast = s->d_bvc_fast; d.bvc_seg = s->d_bvc_seg; d.bv_binned = binned ? 1 : 0;
+    d.n_cyl_runs = (int32_t)runs.size(); d.cyl_runs = s->d_cyl_runs;
+    d.bands = s->d_bands ? s->d_bands + 1 : nullptr;
+    d.bvc_runs = s->d_bvc_runs; d.bvc_run_off = s->d_bvc_run_off;
+    d.bvc_bands = s->d_bvc_bands ? s->d_bvc_bands + 1 : nullptr;
+    d.bv_cflops = s->d_bv_cflops;
     up((void **)&s->d_desc, &d, sizeof d);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
@@ -527,6 +610,7 @@ static int check_opts(rf_ctx *ctx, const rf_solve_opts *o)
         return set_err(ctx, RF_ERR_INVALID_ARG, "bad transform");
     if (!(o->refine_margin >= 0.0f && o->refine_margin <= 0.25f))
         return set_err(ctx, RF_ERR_INVALID_ARG, "refine_margin must lie in [0, 0.25]");
+    if (o->per_test != 0 && o->per_test != 1) return set_err(ctx, RF_ERR_INVALID_ARG, "per_test must be 0 or 1");
     if (o->block_size && (o->block_size < 64 || o->block_size > 256 || o->block_size % 64))
         return set_err(ctx, RF_ERR_INVALID_ARG, "block_size must be 0 or a multiple of 64 in [64, 256]");
     return RF_OK;
@@ -584,6 +668,16 @@ static void pick_shape(const rf_ctx *ctx, const rf_solve_opts *o, int &nr, int &
     } else if (o->block_size) {
         block = o->block_size;
     }
+}
+
+// Band stacks run in the fast form with the VALU transform (plain scan, and
+// the children of the binned BV kernel) unless the caller asks for every test
+// in full.
+static bool uses_stacks(const rf_ctx *ctx, const rf_solve_opts *o)
+{
+    int nr, block, mfma;
+    pick_shape(ctx, o, nr, block, mfma);
+    return o->math == RF_MATH_FAST && !mfma && !o->per_test;
 }
 
 static int ensure_queue(rf_ctx *ctx)
@@ -671,6 +765,7 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
     kp.refine = o->refine;
     kp.tau = o->refine_margin > 0.0f ? o->refine_margin : rfd::TAU_DEFAULT;
     kp.bins = s->n <= kMaxLdsBins ? s->n : 0;
+    kp.stacks = uses_stacks(ctx, o) ? 1 : 0;
     kp.imp_emitter = emitter;
     kp.imp_geom_id = geom_id;
     kp.imp_wi0 = wi0;
@@ -790,21 +885,22 @@ static double now_ms()
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-static int read_stats(rf_ctx *ctx, unsigned long long st[3])
+static int read_stats(rf_ctx *ctx, unsigned long long st[4])
 {
-    HIPCHK(ctx, hipMemcpy(st, ctx->d_stats, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(st, ctx->d_stats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     return RF_OK;
 }
 
 // Accounting of one geometry's solve.
-static void account(const rf_scene *s, const rf_solve_opts *o, const std::vector<int64_t> &n_wi, rf_timing &t)
+static void account(const rf_ctx *ctx, const rf_scene *s, const rf_solve_opts *o, const std::vector<int64_t> &n_wi,
+                    rf_timing &t)
 {
     int64_t rays = 0;
     for (int64_t w : n_wi) rays += 4 * w;
     t.rays += rays;
     if (!(o->use_bvs && s->n_bvs > 0)) {
         t.tests += rays * s->n;
-        t.flops += rays * s->flops_per_ray;
+        t.flops += rays * (uses_stacks(ctx, o) ? s->flops_per_ray_stacked : s->flops_per_ray);
     }
 }
 
@@ -881,6 +977,7 @@ static int prepare_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms,
     kp.refine = o->refine;
     kp.tau = o->refine_margin > 0.0f ? o->refine_margin : rfd::TAU_DEFAULT;
     kp.bins = max_n <= kMaxLdsBins ? max_n : 0;
+    kp.stacks = uses_stacks(ctx, o) ? 1 : 0;
     int rc = take_counter(ctx, &kp.queue_head);
     if (rc) return rc;
     c = LaunchCfg{};
@@ -917,7 +1014,7 @@ static hipError_t enqueue_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n
     hipError_t e = hipSuccess;
     for (int g = 0; g < n_geoms && e == hipSuccess; ++g)
         e = launch_fill_u64(scenes[g]->d_hits, (size_t)scenes[g]->n * scenes[g]->n, ctx->stream);
-    if (e == hipSuccess) e = launch_fill_u64(ctx->d_stats, 3, ctx->stream);
+    if (e == hipSuccess) e = launch_fill_u64(ctx->d_stats, 4, ctx->stream);
     if (e == hipSuccess && kp.queue_head) e = launch_fill_u32(kp.queue_head, 0u, 1, ctx->stream);
     if (e == hipSuccess && kp.n_units > 0) e = launch_solve(c, kp, ctx->stream);
     return e;
@@ -1042,7 +1139,7 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
         // hit matrices and statistics start at zero
         for (int g = 0; g < n_geoms; ++g)
             HIPCHK(ctx, launch_fill_u64(scenes[g]->d_hits, (size_t)scenes[g]->n * scenes[g]->n, ctx->stream));
-        HIPCHK(ctx, launch_fill_u64(ctx->d_stats, 3, ctx->stream));
+        HIPCHK(ctx, launch_fill_u64(ctx->d_stats, 4, ctx->stream));
         HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         // one launch per emitter; queue-kernel launches of different emitters
         // are independent and fan out over the lanes, the reference shape
@@ -1061,7 +1158,7 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
     HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
     float ms = 0.f;
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    unsigned long long st[3];
+    unsigned long long st[4];
     rc = read_stats(ctx, st);
     if (rc) return rc;
     rf_timing t{};
@@ -1071,7 +1168,7 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
     t.refined = (int64_t)st[1];
     t.rescans = (int64_t)st[2];
     for (int g = 0; g < n_geoms; ++g) {
-        account(scenes[g], o, budgets[g], t);
+        account(ctx, scenes[g], o, budgets[g], t);
         if (o->use_bvs && scenes[g]->n_bvs > 0) {
             // BV kernels count executed tests on the device (all geometries together)
         }
@@ -1080,13 +1177,23 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
             for (int e = 0; e < scenes[g]->n; ++e) rays[g][e] = 4 * budgets[g][e];
     }
     if (o->use_bvs) {
-        double favg = 0.0;
-        int nbv_g = 0;
-        for (int g = 0; g < n_geoms; ++g)
-            if (scenes[g]->n_bvs > 0) { favg += scenes[g]->flops_per_test_avg; ++nbv_g; }
-        favg = nbv_g ? favg / nbv_g : 0.0;
         t.tests += (int64_t)st[0];
-        t.flops += (int64_t)((double)st[0] * favg);
+        // the binned kernel counts its children's flops on the device (stats[3],
+        // stacked or per test); the surfaces' own tests are rays x sum of F
+        bool binned = o->math == RF_MATH_FAST && o->form != RF_FORM_REFERENCE;
+        for (int g = 0; g < n_geoms; ++g) binned = binned && scenes[g]->n_bvs > 0 && scenes[g]->desc.bv_binned;
+        if (binned) {
+            t.flops += (int64_t)st[3];
+            for (int g = 0; g < n_geoms; ++g)
+                for (int64_t w : budgets[g]) t.flops += 4 * w * scenes[g]->flops_bvs;
+        } else {
+            double favg = 0.0;
+            int nbv_g = 0;
+            for (int g = 0; g < n_geoms; ++g)
+                if (scenes[g]->n_bvs > 0) { favg += scenes[g]->flops_per_test_avg; ++nbv_g; }
+            favg = nbv_g ? favg / nbv_g : 0.0;
+            t.flops += (int64_t)((double)st[0] * favg);
+        }
     }
     t.wall_ms = now_ms() - w0;
     if (timing) *timing = t;

@@ -24,9 +24,11 @@ def make_opts(math: str = "compat", rng: str = "reference", use_bvs: bool = Fals
               n_samples: int | None = None, rays_per_emitter: int | None = None,
               form: str = "persistent", refine: bool = True, seed: int = 0, geometry: int = 0,
               rays_per_thread: int = 0, block_size: int = 0, transform: str = "auto",
-              refine_margin: float = 0.0) -> SolveOpts:
+              refine_margin: float = 0.0, per_test: bool = False) -> SolveOpts:
     """Solve options.  The ray budget is the reference's launch geometry
-    (``n_samples``, main.cpp:172-173) unless ``rays_per_emitter`` is given."""
+    (``n_samples``, main.cpp:172-173) unless ``rays_per_emitter`` is given.
+    ``per_test``: fast form without band stacks (every cylinder test in full;
+    same tallies)."""
     o = SolveOpts()
     lib().rf_solve_opts_default(byref(o))
     o.math = MATH[math]
@@ -40,6 +42,7 @@ def make_opts(math: str = "compat", rng: str = "reference", use_bvs: bool = Fals
     o.block_size = block_size
     o.transform = TRANSFORM[transform]
     o.refine_margin = float(refine_margin)
+    o.per_test = int(bool(per_test))
     if rays_per_emitter is not None:
         o.budget = 1
         o.rays_per_emitter = int(rays_per_emitter)

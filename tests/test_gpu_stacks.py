@@ -1,0 +1,182 @@
+"""Band stacks of the fast form: consecutive cylinders whose transforms differ
+only in the z offset (the bands of one coaxial wall) share one quadric solve
+per ray.  Every per-band decision is the per-test scan's on that band's own
+record, so the tallies, the FP64 re-solve counts and the per-ray hit indices
+must equal the per-test form's exactly, for every launch shape, with and
+without flag-list overflow, and on stacks that overlap, leave gaps or hold a
+single band."""
+import numpy as np
+import pytest
+
+import rayfactorcl_amd as rf
+from rayfactorcl_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(ctx, sc, **kw):
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    a = rf.solve(ctx, dev, math="fast", rng="seeded", per_test=True, **kw)
+    b = rf.solve(ctx, dev, math="fast", rng="seeded", per_test=False, **kw)
+    dev.close()
+    return a, b
+
+
+def _same(a, b):
+    assert np.array_equal(a.hits, b.hits)
+    assert a.timing["refined"] == b.timing["refined"]
+
+
+@pytest.mark.parametrize("form", ["persistent", "per_emitter", "reference"])
+@pytest.mark.parametrize("rpt", [1, 2, 4, 8])
+def test_stacks_equal_per_test(ctx, form, rpt):
+    sc = scenes.parse_xml(scenes.furnace_xml(wall_bands=40, preform_bands=12))
+    a, b = _pair(ctx, sc, rays_per_emitter=20000, form=form, rays_per_thread=rpt)
+    _same(a, b)
+    # the stacked form counts fewer algorithmic flops for the same tests
+    assert b.timing["tests"] == a.timing["tests"] and b.timing["flops"] < a.timing["flops"]
+
+
+@pytest.mark.parametrize("rpt", [1, 4, 8])
+def test_stacks_wide_margin_overflow(ctx, rpt):
+    """tau = 1/4 flags most rim crossings: re-solve lists overflow inside
+    stacks, the stack's queued entries are given back and the re-scan starts
+    at its first band; every flagged pair is still re-solved exactly once."""
+    sc = scenes.parse_xml(scenes.furnace_xml(wall_bands=40, preform_bands=12))
+    a, b = _pair(ctx, sc, rays_per_emitter=20000, form="persistent", rays_per_thread=rpt, refine_margin=0.25)
+    _same(a, b)
+    assert b.timing["rescans"] > 0
+
+
+def test_stacks_without_refine(ctx):
+    sc = scenes.parse_xml(scenes.furnace_xml(wall_bands=60, preform_bands=20))
+    a, b = _pair(ctx, sc, rays_per_emitter=1 << 16, form="persistent", refine=False)
+    assert np.array_equal(a.hits, b.hits)
+
+
+def test_stacks_sweep_batched(ctx):
+    geoms = scenes.sweep(6, seed=0x52465331)
+    dev = [rf.DeviceScene(ctx, g.prims, None) for g in geoms]
+    kw = dict(math="fast", rng="seeded", rays_per_emitter=1 << 14, form="persistent")
+    a, ta = rf.batch_solve(ctx, dev, rf.make_opts(per_test=True, **kw))
+    b, tb = rf.batch_solve(ctx, dev, rf.make_opts(per_test=False, **kw))
+    for x, y in zip(a, b):
+        assert np.array_equal(x.hits, y.hits)
+    assert ta["refined"] == tb["refined"] and ta["tests"] == tb["tests"]
+    print(f"[stacks] sweep sample: flops per launch {ta['flops']:.3e} per test, {tb['flops']:.3e} stacked", flush=True)
+    for d in dev:
+        d.close()
+
+
+def test_stacks_traces(ctx):
+    sc = scenes.parse_xml(scenes.furnace_xml(wall_bands=30, preform_bands=10))
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    for e in (0, 15, sc.n - 1):
+        a = rf.trace(ctx, dev, e, 0, 4096, math="fast", rng="seeded", per_test=True)
+        b = rf.trace(ctx, dev, e, 0, 4096, math="fast", rng="seeded", per_test=False)
+        assert np.array_equal(a, b)
+    dev.close()
+
+
+def _band(lo, hi, r=0.5):
+    return scenes._prim("cylinderSurface", bounding=True, small=1.0, scale=(r, r, hi - lo), translate=(0, 0, lo))
+
+
+def _edge_scene():
+    """Cylinder runs of every kind, in the caller's order:
+    - equal-height bands with a gap (a stack whose bands do not tile);
+    - a band listed twice (a stack with exactly tied bands: the first wins);
+    - two overlapping bands of equal height (a stack whose z ranges overlap);
+    - a run of shorter bands (a stack of its own), split by a disc in the
+      caller's order (the cylinder segment keeps them consecutive);
+    - single bands of other radii and heights (per-test runs);
+    closed by a disc at each end."""
+    parts = [_band(0.0, 0.5), _band(0.5, 1.0), _band(1.1, 1.6), _band(1.6, 2.1),
+             _band(2.1, 2.6), _band(2.1, 2.6),
+             _band(2.6, 3.1), _band(2.85, 3.35),
+             _band(3.35, 3.6),
+             scenes._prim("disc", scale=(0.3, 0.3, 1), translate=(0, 0, 1.0)),
+             _band(3.6, 3.85), _band(3.85, 4.1),
+             _band(0.0, 0.7, r=0.2), _band(0.7, 1.2, r=0.25), _band(4.1, 4.3, r=0.5),
+             scenes._prim("disc", scale=(0.5, 0.5, 1)),
+             scenes._prim("disc", scale=(0.5, 0.5, 1), rotate=scenes.FLIP_X, translate=(0, 0, 4.3))]
+    return scenes.parse_xml(scenes._doc("band stack edge cases", "".join(parts)))
+
+
+@pytest.mark.parametrize("margin", [0.0, 0.25])
+@pytest.mark.parametrize("rpt", [1, 4])
+def test_stacks_edge_cases(ctx, margin, rpt):
+    sc = _edge_scene()
+    a, b = _pair(ctx, sc, rays_per_emitter=1 << 15, form="persistent", rays_per_thread=rpt, refine_margin=margin)
+    _same(a, b)
+    # the duplicated band never wins over its first copy
+    assert a.hits[:, 5].sum() == 0 and a.hits[:, 4].sum() > 0
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    for e in (0, 4, 7, 12):
+        kw = dict(math="fast", rng="seeded", refine_margin=margin)
+        ta = rf.trace(ctx, dev, e, 0, 2048, per_test=True, **kw)
+        tb = rf.trace(ctx, dev, e, 0, 2048, per_test=False, **kw)
+        assert np.array_equal(ta, tb)
+    dev.close()
+
+
+def _tilted(sc, deg=23.0, axis=(0.3, 0.8, 0.52)):
+    """The scene rotated about the origin: world -> object maps become invm * R^T
+    and object -> world maps R * m, so coaxial bands keep equal rows apart from
+    the z offset, now with x and y terms in every row (non-axial stacks)."""
+    a = np.asarray(axis, dtype=np.float64)
+    a /= np.linalg.norm(a)
+    th = np.radians(deg)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    Rm = (np.eye(3) + np.sin(th) * K + (1 - np.cos(th)) * K @ K).astype(np.float32)
+    p = sc.prims.copy()
+    for k in range(len(p)):
+        inv = p["invm"][k].reshape(3, 4)
+        m = p["m"][k].reshape(3, 4)
+        inv2 = np.empty_like(inv)
+        inv2[:, :3] = inv[:, :3] @ Rm.T
+        inv2[:, 3] = inv[:, 3]
+        m2 = np.empty_like(m)
+        m2[:, :3] = Rm @ m[:, :3]
+        m2[:, 3] = Rm @ m[:, 3]
+        p["invm"][k] = inv2.reshape(12)
+        p["m"][k] = m2.reshape(12)
+    return scenes.SceneData(p, sc.bvs, sc.globalRayDensity, sc.useBoundingVolumes)
+
+
+@pytest.mark.parametrize("margin", [0.0, 0.25])
+def test_stacks_tilted(ctx, margin):
+    sc = _tilted(scenes.parse_xml(scenes.furnace_xml(wall_bands=40, preform_bands=12)))
+    inv = sc.prims["invm"]
+    assert (inv[:40, 8] != 0).all() and (inv[:40, :11] == inv[0, :11]).all()
+    a, b = _pair(ctx, sc, rays_per_emitter=20000, form="persistent", refine_margin=margin)
+    _same(a, b)
+
+
+# ---------------------------------------------------------------------------
+# The binned bounding-surface kernel scans each surface's children with the
+# same runs (the wall surface's children are one band stack).
+
+@pytest.mark.parametrize("margin", [0.0, 0.25])
+@pytest.mark.parametrize("form", ["persistent", "per_emitter"])
+def test_stacks_bv_binned(ctx, margin, form):
+    sc = scenes.parse_xml(scenes.furnace_xml(with_bv=True, wall_bands=60, preform_bands=20))
+    dev = rf.DeviceScene(ctx, sc.prims, sc.bvs)
+    kw = dict(math="fast", rng="seeded", use_bvs=True, rays_per_emitter=1 << 14, form=form, refine_margin=margin)
+    a = rf.solve(ctx, dev, per_test=True, **kw)
+    b = rf.solve(ctx, dev, per_test=False, **kw)
+    _same(a, b)
+    assert a.timing["tests"] == b.timing["tests"]
+    assert 0 < b.timing["flops"] < a.timing["flops"]
+    dev.close()
+
+
+def test_stacks_bv_overlapping_children(ctx):
+    sc = scenes.parse_xml(scenes.furnace_xml(with_bv=True, wall_bands=60, preform_bands=20))
+    bvs = sc.bvs.copy()
+    bvs["objEndIndex"][2] = bvs["objStartIndex"][2]                      # no children
+    bvs["objStartIndex"][1] = max(0, int(bvs["objStartIndex"][1]) - 5)   # overlaps the wall's range
+    dev = rf.DeviceScene(ctx, sc.prims, bvs)
+    kw = dict(math="fast", rng="seeded", use_bvs=True, rays_per_emitter=1 << 14, form="persistent")
+    _same(rf.solve(ctx, dev, per_test=True, **kw), rf.solve(ctx, dev, per_test=False, **kw))
+    dev.close()

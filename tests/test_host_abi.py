@@ -32,7 +32,7 @@ def test_struct_layouts():
     assert rf.PRIM_DTYPE.itemsize == 112 and rf.BV_DTYPE.itemsize == 72
     assert rf.PRIM_DTYPE.fields["type"][1] == 96 and rf.PRIM_DTYPE.fields["util2"][1] == 108
     assert rf.BV_DTYPE.fields["objEndIndex"][1] == 60 and rf.BV_DTYPE.fields["util2"][1] == 68
-    assert rf.lib().rf_abi_version() == 1
+    assert rf.lib().rf_abi_version() == 2
 
 
 # work items per emitter of the sample scenes at globalRayDensity = 1e6
@@ -164,4 +164,4 @@ def test_solve_opts_defaults():
     o = _lib.SolveOpts()
     rf.lib().rf_solve_opts_default(ctypes.byref(o))
     assert (o.math, o.rng, o.use_bvs, o.budget, o.n_samples, o.form, o.refine) == (0, 0, 0, 0, 1000000, 1, 1)
-    assert (o.rays_per_thread, o.block_size, o.transform, o.refine_margin) == (0, 0, 0, 0.0)
+    assert (o.rays_per_thread, o.block_size, o.transform, o.refine_margin, o.per_test) == (0, 0, 0, 0.0, 0)
