@@ -142,9 +142,10 @@ hipError_t launch_selftest_math(unsigned long long *counts, unsigned int *first,
 // Algorithmic FP32 flops of one band stack, same counting as flops_per_test:
 // per (ray, stack) the x-y rows of origin and direction, the z direction, A,
 // b, the cross term, D, sqrt, rcp and both roots; per (ray, band) the band's
-// z origin row and z at both roots.
+// z origin row (one FMA when the z row has no x or y term) and z at both roots.
 constexpr int64_t kFlopsStack = 45;
 constexpr int64_t kFlopsBand = 10;
+constexpr int64_t kFlopsBandAxial = 6;
 
 // Algorithmic FP32 flops per ray-primitive test (counted from the reference
 // formulas: FMA = 2, mul/add = 1, rcp/sqrt = 1, compare/select = 0).

@@ -491,12 +491,18 @@ __device__ __forceinline__ void seg_loop(CPTR(FastPrim) F, int begin, int end, c
 // stack, the entries it queued from this stack are given back and the re-scan
 // starts at the stack's first band (resume = b x 8), so every flagged pair is
 // still re-solved exactly once.
-// Band loop of stack_loop, bands e-1 down to b.  AXIAL: the stack's z row has
-// no x or y term (r[8] == r[9] == 0), so the band's z origin is one FMA; the
-// per-test scan's two extra FMAs add a signed zero, which no |z| compare can
-// tell apart.  TAN: some ray of the wave is near-tangent (finite TM), so the
-// tangent-point flag is evaluated; otherwise it is false for every ray.
-template <int NR, bool REFINE, bool AXIAL, bool TAN>
+// Band loop of stack_loop, bands e-1 down to b.  Wave-uniform variants:
+//  AXIAL: the stack's z row has no x or y term (r[8] == r[9] == 0), so the
+//    band's z origin is one FMA; the per-test scan's two extra FMAs add a
+//    signed zero, which no |z| compare can tell apart;
+//  TAN: some ray of the wave is near-tangent (finite TM), so the tangent-point
+//    flag is evaluated; otherwise it is false for every ray;
+//  ROOT1: some ray of the wave has a first root ahead of it (finite T1).
+//    Otherwise root 1 is never on a band nor a rim flag (every ray starts
+//    inside a wall's cylinder), and without the tangent flag a flagged pair
+//    (a rim of root 2) can never also hold root 2 on the band (|z| < lo vs
+//    lo <= |z|), so no pair needs excluding.
+template <int NR, bool REFINE, bool AXIAL, bool TAN, bool ROOT1>
 __device__ __forceinline__ void stack_bands(const FastPrim &h, CPTR(BandRec) BR, int b, int e, const Ray32 (&R)[NR],
                                             const float (&T1)[NR], const float (&T2)[NR], const float (&TM)[NR],
                                             const float (&CZ)[NR], int (&I1)[NR], int (&I2)[NR], float lo, float hi,
@@ -520,25 +526,30 @@ __device__ __forceinline__ void stack_bands(const FastPrim &h, CPTR(BandRec) BR,
             const float sz = AXIAL ? __builtin_fmaf(h.r[10], R[i].sz, r11)
                                    : __builtin_fmaf(h.r[8], R[i].sx,
                                                     __builtin_fmaf(h.r[9], R[i].sy, __builtin_fmaf(h.r[10], R[i].sz, r11)));
-            const float a1 = __builtin_fabsf(__builtin_fmaf(CZ[i], T1[i], sz));
             const float a2 = __builtin_fabsf(__builtin_fmaf(CZ[i], T2[i], sz));
-            bool in1, in2;
+            const float a1 = ROOT1 ? __builtin_fabsf(__builtin_fmaf(CZ[i], T1[i], sz)) : 0.f;
+            bool in1 = false, in2;
             if constexpr (REFINE) {
                 // fast_test's terms: a root is on the band for |z| < lo and a
                 // rim flag for lo <= |z| <= hi (its (|z| <= hi) ^ (|z| < lo);
                 // med3 clamps |z| to [lo, hi], a NaN fails the equality)
-                in1 = a1 < lo;
                 in2 = a2 < lo;
-                f[i] = (__builtin_amdgcn_fmed3f(a1, lo, hi) == a1) | (__builtin_amdgcn_fmed3f(a2, lo, hi) == a2);
+                f[i] = __builtin_amdgcn_fmed3f(a2, lo, hi) == a2;
+                if constexpr (ROOT1) {
+                    in1 = a1 < lo;
+                    f[i] = f[i] | (__builtin_amdgcn_fmed3f(a1, lo, hi) == a1);
+                }
                 if constexpr (TAN) f[i] = f[i] | (__builtin_fabsf(__builtin_fmaf(CZ[i], TM[i], sz)) <= hi);
                 any |= f[i];
-                in1 = in1 & !f[i];
-                in2 = in2 & !f[i];
+                if constexpr (ROOT1 || TAN) {
+                    in1 = in1 & !f[i];
+                    in2 = in2 & !f[i];
+                }
             } else {
-                in1 = a1 <= 0.5f;
                 in2 = a2 <= 0.5f;
+                if constexpr (ROOT1) in1 = a1 <= 0.5f;
             }
-            I1[i] = in1 ? orig : I1[i];
+            if constexpr (ROOT1) I1[i] = in1 ? orig : I1[i];
             I2[i] = in2 ? orig : I2[i];
             // updates stay ahead of the branch below (sunk past it, the range
             // tests lose their |z| operand modifiers)
@@ -561,6 +572,21 @@ __device__ __forceinline__ void stack_bands(const FastPrim &h, CPTR(BandRec) BR,
     }
 }
 
+template <int NR, bool REFINE, bool AXIAL>
+__device__ __forceinline__ void stack_bands_sel(bool tan, bool root1, const FastPrim &h, CPTR(BandRec) BR, int b,
+                                                int e, const Ray32 (&R)[NR], const float (&T1)[NR],
+                                                const float (&T2)[NR], const float (&TM)[NR], const float (&CZ)[NR],
+                                                int (&I1)[NR], int (&I2)[NR], float lo, float hi, Pending &pd)
+{
+    if (tan) {
+        if (root1) stack_bands<NR, REFINE, AXIAL, true, true>(h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, pd);
+        else stack_bands<NR, REFINE, AXIAL, true, false>(h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, pd);
+    } else {
+        if (root1) stack_bands<NR, REFINE, AXIAL, false, true>(h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, pd);
+        else stack_bands<NR, REFINE, AXIAL, false, false>(h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, pd);
+    }
+}
+
 template <int NR, bool REFINE>
 __device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, int b, int e, const Ray32 (&R)[NR],
                                            float (&tb)[NR], int (&ib)[NR], Pending &pd)
@@ -574,7 +600,7 @@ __device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, i
     const float qnan = __builtin_nanf("");
     float T1[NR], T2[NR], TM[NR], CZ[NR];
     int I1[NR], I2[NR];
-    bool tan = false;
+    bool tan = false, root1 = false;
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
         const float sx = ROW_S(h, 0, R[i]), sy = ROW_S(h, 1, R[i]);
@@ -591,6 +617,7 @@ __device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, i
         const float t2 = __builtin_fmaf(sq, rA, tm);
         T1[i] = t1 > eps ? t1 : qnan;
         T2[i] = t2 > eps ? t2 : qnan;
+        root1 |= t1 > eps;
         const bool near = REFINE && __builtin_fabsf(D) <= tau * A;
         TM[i] = near ? tm : qnan;
         tan |= near;
@@ -599,13 +626,10 @@ __device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, i
         I1[i] = -1;
         I2[i] = -1;
     }
-    if (REFINE && __builtin_amdgcn_ballot_w64(tan)) {
-        if (axial) stack_bands<NR, REFINE, true, true>(h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, pd);
-        else stack_bands<NR, REFINE, false, true>(h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, pd);
-    } else {
-        if (axial) stack_bands<NR, REFINE, true, false>(h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, pd);
-        else stack_bands<NR, REFINE, false, false>(h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, pd);
-    }
+    const bool wtan = REFINE && __builtin_amdgcn_ballot_w64(tan);
+    const bool wroot1 = __builtin_amdgcn_ballot_w64(root1);
+    if (axial) stack_bands_sel<NR, REFINE, true>(wtan, wroot1, h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, pd);
+    else stack_bands_sel<NR, REFINE, false>(wtan, wroot1, h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, pd);
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
         const bool h1 = I1[i] >= 0;

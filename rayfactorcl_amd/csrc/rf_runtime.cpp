@@ -323,7 +323,7 @@ static void quad_rows(const rf_primitive *p, float rows[4][4])
 // groups of consecutive cylinders whose folded rows other than the z offset
 // r[11] are bitwise equal; two or more form a band stack, consecutive
 // singletons one per-test run.  Returns the change of algorithmic flops per
-// ray that the stacks bring (kFlopsStack + L kFlopsBand instead of L tests).
+// ray that the stacks bring (kFlopsStack + L band flops instead of L tests).
 static int64_t build_cyl_runs(const std::vector<FastPrim> &recs, int32_t cb, int32_t ce, std::vector<CylRun> &runs)
 {
     int64_t delta = 0;
@@ -337,7 +337,8 @@ static int64_t build_cyl_runs(const std::vector<FastPrim> &recs, int32_t cb, int
         int32_t q = stack_end(p);
         if (q - p >= 2) {
             runs.push_back(CylRun{p, q, 1, 0});
-            delta += kFlopsStack + (q - p) * (kFlopsBand - flops_per_test(RF_CYLINDER));
+            const bool axial = recs[p].r[8] == 0.0f && recs[p].r[9] == 0.0f;
+            delta += kFlopsStack + (q - p) * ((axial ? kFlopsBandAxial : kFlopsBand) - flops_per_test(RF_CYLINDER));
             p = q;
             continue;
         }
