@@ -125,25 +125,28 @@ def config3(ctx, quick: bool) -> dict:
             "differing_rays": frac, "rays_compared": nr}
 
 
-def _stream_throughput(ctx, reqs, max_rays: float) -> float:
-    """Tests/s of the streaming service on a closed batch of the first eight
-    requests (budgets capped at max_rays): the rate the offered load is set
-    against."""
-    st = Stream(ctx, 64 << 30)
-    tests = 0
-    ids = []
-    for i, (_, sc, r) in enumerate(reqs[:8]):
-        r = int(min(r, max_rays))
-        o = make_opts(math="fast", rng="seeded", rays_per_emitter=r, form="persistent", geometry=i)
-        ids.append(st.submit(sc.prims, o))
-        tests += sc.n * sc.n * 4 * ((r + 3) // 4)
-    t0 = time.perf_counter()
-    st.flush()
-    dt = time.perf_counter() - t0
-    for rid in ids:
-        st.collect(rid)
-    st.close()
-    return tests / dt
+def _stream_throughput(ctx, reqs, rays: int = 100_000) -> float:
+    """Throughput of the streaming service on this request mix: each kind of
+    scene (furnace, soup) is timed in one closed batch of all its requests at
+    `rays` per emitter, and the mix rate weights the kinds by their share of
+    the requests' tests."""
+    rate = {}
+    for kind in ("furnace", "soup"):
+        sel = [(i, sc) for i, (_, sc, _, k) in enumerate(reqs) if k == kind]
+        if not sel:
+            continue
+        st = Stream(ctx, 64 << 30)
+        ids = [st.submit(sc.prims, make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="persistent",
+                                             geometry=i)) for i, sc in sel]
+        t0 = time.perf_counter()
+        st.flush()
+        dt = time.perf_counter() - t0
+        for rid in ids:
+            st.collect(rid)
+        st.close()
+        rate[kind] = sum(sc.n * sc.n for _, sc in sel) * 4 * ((rays + 3) // 4) / dt
+    cost = [(sc.n * sc.n * 4 * ((r + 3) // 4), k) for _, sc, r, k in reqs]
+    return sum(c for c, _ in cost) / sum(c / rate[k] for c, k in cost)
 
 
 def config5(ctx, quick: bool, n_req: int = 0, load: float = 0.7) -> dict:
@@ -156,11 +159,11 @@ def config5(ctx, quick: bool, n_req: int = 0, load: float = 0.7) -> dict:
     n_req = n_req or (30 if quick else 60)
     lo, hi = 1e5, (1e6 if quick else 1e7)
     reqs = scenes.streaming_requests(n_req, rate_hz=1.0, min_rays=lo, max_rays=hi)
-    cost = np.array([sc.n * sc.n * (4 * ((r + 3) // 4)) for _, sc, r in reqs], dtype=float)
-    throughput = _stream_throughput(ctx, reqs, 1e6)
+    cost = np.array([sc.n * sc.n * (4 * ((r + 3) // 4)) for _, sc, r, _ in reqs], dtype=float)
+    throughput = _stream_throughput(ctx, reqs)
     print(f"[config5] calibration: {throughput:.3e} tests/s", file=sys.stderr, flush=True)
     rate = load * throughput / cost.mean()
-    arrivals = [t / rate for t, _, _ in reqs]
+    arrivals = [t / rate for t, _, _, _ in reqs]
     st = Stream(ctx, 64 << 30)
     t0 = time.perf_counter()
     i = 0
@@ -171,7 +174,7 @@ def config5(ctx, quick: bool, n_req: int = 0, load: float = 0.7) -> dict:
     while i < len(reqs) or queued:
         now = time.perf_counter() - t0
         while i < len(reqs) and arrivals[i] <= now:
-            _, sc, r = reqs[i]
+            _, sc, r, _ = reqs[i]
             o = make_opts(math="fast", rng="seeded", rays_per_emitter=r, form="persistent", geometry=i)
             queued[st.submit(sc.prims, o)] = (arrivals[i], sc.n)
             i += 1

@@ -76,7 +76,8 @@ struct GeomDesc {
     // binned BV form with band stacks: runs of the children's cylinder
     // segment of surface b are bvc_runs[bvc_run_off[b] .. bvc_run_off[b+1]),
     // over bvc_fast positions; bvc_bands as bands; bv_cflops[2b + s] the
-    // children's algorithmic flops per ray, per test (s = 0) or stacked (s = 1)
+    // children's algorithmic flops per ray, per test (s = 0) or, with band
+    // stacks (s = 1), those of the children outside the stacks
     const CylRun *bvc_runs;
     const int32_t *bvc_run_off;
     const BandRec *bvc_bands;
@@ -97,7 +98,8 @@ struct KParams {
     const WorkUnit *units;              // nullptr: implicit unit list (below)
     unsigned int *queue_head;           // atomic unit counter (zeroed before launch)
     unsigned long long *stats;          // [0] tests executed (BV kernels), [1] FP64 re-solves,
-                                        // [2] re-scans, [3] children's flops (binned BV kernel)
+                                        // [2] re-scans, [3] flops counted on the device (band
+                                        // stacks; the binned BV kernel's child scans)
     int32_t *trace_out;                 // per-ray hit index: ray 4*(wi - trace_wi0) + v
     int64_t trace_wi0;
     int32_t n_units;
@@ -139,13 +141,16 @@ hipError_t launch_fill_u32(unsigned int *p, unsigned int value, size_t count, hi
 hipError_t launch_fill_u64(unsigned long long *p, size_t count, hipStream_t stream);
 hipError_t launch_selftest_math(unsigned long long *counts, unsigned int *first, hipStream_t stream);
 
-// Algorithmic FP32 flops of one band stack, same counting as flops_per_test:
-// per (ray, stack) the x-y rows of origin and direction, the z direction, A,
-// b, the cross term, D, sqrt, rcp and both roots; per (ray, band) the band's
-// z origin row (one FMA when the z row has no x or y term) and z at both roots.
-constexpr int64_t kFlopsStack = 45;
-constexpr int64_t kFlopsBand = 10;
-constexpr int64_t kFlopsBandAxial = 6;
+// Algorithmic FP32 flops of one band stack, same counting as flops_per_test,
+// counted on the device from the band loop a wave runs: per (ray, stack) the
+// x-y rows of origin and direction, the z direction, A, b, the cross term, D,
+// sqrt, rcp and both roots; per (ray, band) the band's z origin row (one FMA
+// when the z row has no x or y term) and z at each root the wave evaluates
+// (root 1 is skipped when no ray of the wave has it ahead).
+constexpr uint32_t kFlopsStack = 45;
+constexpr uint32_t kFlopsBandZ0 = 6;
+constexpr uint32_t kFlopsBandZ0Axial = 2;
+constexpr uint32_t kFlopsBandRoot = 2;
 
 // Algorithmic FP32 flops per ray-primitive test (counted from the reference
 // formulas: FMA = 2, mul/add = 1, rcp/sqrt = 1, compare/select = 0).

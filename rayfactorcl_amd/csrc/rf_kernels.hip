@@ -587,9 +587,10 @@ __device__ __forceinline__ void stack_bands_sel(bool tan, bool root1, const Fast
     }
 }
 
+// Returns the stack's algorithmic flops per ray for the band loop the wave ran.
 template <int NR, bool REFINE>
-__device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, int b, int e, const Ray32 (&R)[NR],
-                                           float (&tb)[NR], int (&ib)[NR], Pending &pd)
+__device__ __forceinline__ uint32_t stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, int b, int e,
+                                               const Ray32 (&R)[NR], float (&tb)[NR], int (&ib)[NR], Pending &pd)
 {
     const FastPrim hs = ldc(F + b);
     const FastPrim h = vcopy(hs);
@@ -637,6 +638,8 @@ __device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, i
         const int ci = h1 ? I1[i] : I2[i];
         if (c < tb[i]) { tb[i] = c; ib[i] = ci; }
     }
+    return kFlopsStack + (uint32_t)(e - b) * ((axial ? kFlopsBandZ0Axial : kFlopsBandZ0) +
+                                              (wroot1 ? 2u : 1u) * kFlopsBandRoot);
 }
 
 // FP64 stage of the deferred re-solves (see Pending).  The flagged ray is
@@ -716,18 +719,20 @@ __device__ __forceinline__ void finish_dropped(const KParams &kp, CPTR(GeomDesc)
 }
 
 // stacks: the cylinder segment runs as its CylRun list (band stacks share the
-// quadric); otherwise as one per-test run.
+// quadric); otherwise as one per-test run.  Returns the band stacks'
+// algorithmic flops per ray (the rest is counted on the host).
 template <int NR, bool REFINE>
-__device__ __forceinline__ void scan_fast(CPTR(GeomDesc) G, const Ray32 (&R)[NR], float (&tb)[NR],
-                                          int (&ib)[NR], Pending &pd, int stacks)
+__device__ __forceinline__ uint32_t scan_fast(CPTR(GeomDesc) G, const Ray32 (&R)[NR], float (&tb)[NR],
+                                              int (&ib)[NR], Pending &pd, int stacks)
 {
+    uint32_t sf = 0;
     CPTR(FastPrim) F = cst(G->fast);
     const int n_runs = stacks ? G->n_cyl_runs : 1;
     for (int j = 0; j < n_runs; ++j) {
         CylRun run{G->seg_begin[SEG_CYL], G->seg_end[SEG_CYL], 0, 0};
         if (stacks) run = ldc(cst(G->cyl_runs) + j);
         if (run.stack)
-            stack_loop<NR, REFINE>(F, cst(G->bands), run.begin, run.end, R, tb, ib, pd);
+            sf += stack_loop<NR, REFINE>(F, cst(G->bands), run.begin, run.end, R, tb, ib, pd);
         else
             seg_loop<SEG_CYL, NR, REFINE, false>(F, run.begin, run.end, R, tb, ib, pd);
     }
@@ -735,6 +740,7 @@ __device__ __forceinline__ void scan_fast(CPTR(GeomDesc) G, const Ray32 (&R)[NR]
     seg_loop<SEG_SPHERE, NR, REFINE, false>(F, G->seg_begin[SEG_SPHERE], G->seg_end[SEG_SPHERE], R, tb, ib, pd);
     seg_loop<SEG_RING, NR, REFINE, false>(F, G->seg_begin[SEG_RING], G->seg_end[SEG_RING], R, tb, ib, pd);
     seg_loop<SEG_RECT, NR, REFINE, false>(F, G->seg_begin[SEG_RECT], G->seg_end[SEG_RECT], R, tb, ib, pd);
+    return sf;
 }
 
 // The NR rays of ray group `base` (unit-local ray index) of one emitter.
@@ -850,8 +856,8 @@ __device__ __forceinline__ void tally(const KParams &kp, CPTR(GeomDesc) G, int e
 template <int MATH, bool BV, bool TRACE, int NR, bool REFINE, bool DIRECT>
 __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, uint32_t gid, int emitter,
                                          int64_t wi0, int64_t n_wi, int64_t first, int64_t stride,
-                                         uint32_t *bins, uint32_t *pend, uint64_t &tests, uint32_t &nref,
-                                         uint32_t &nover)
+                                         uint32_t *bins, uint32_t *pend, uint64_t &tests, uint64_t &sflops,
+                                         uint32_t &nref, uint32_t &nover)
 {
     const rf_primitive E = ldc(cst(G->prims) + emitter);
     const int64_t ray0 = (wi0 - kp.trace_wi0) * 4;   // absolute ray index base (trace launches)
@@ -866,7 +872,9 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
 #pragma unroll
             for (int i = 0; i < NR; ++i) { tb[i] = finf(); ib[i] = -1; }
             Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
-            scan_fast<NR, REFINE>(G, R, tb, ib, pd, kp.stacks);
+            const uint32_t sf = scan_fast<NR, REFINE>(G, R, tb, ib, pd, kp.stacks);
+            const int64_t left = n_rays - base;
+            sflops += (uint64_t)sf * (uint64_t)(left < NR ? left : NR);
             if constexpr (REFINE) {
                 const auto ray_of = [base](int k) { return base + k; };
                 if (__builtin_amdgcn_ballot_w64(pd.count > 0))
@@ -921,7 +929,7 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                         float tb1[1] = {finf()};
                         int ib1[1] = {-1};
                         Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
-                        scan_fast<1, REFINE>(G, R1, tb1, ib1, pd, kp.stacks);
+                        sflops += scan_fast<1, REFINE>(G, R1, tb1, ib1, pd, kp.stacks);
                         if constexpr (REFINE) {
                             const int64_t r0 = wl * 4 + v;
                             const auto ray_of = [r0](int k) { return r0 + k; };
@@ -942,7 +950,8 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
 
 // stats[0]: tests counted on the device (BV forms); stats[1]: pairs re-solved
 // in FP64; stats[2]: lane re-scans after a full re-solve list; stats[3]:
-// algorithmic flops of the binned BV kernel's child scans.
+// algorithmic flops counted on the device (band stacks of the plain scan; all
+// child scans of the binned BV kernel).
 __device__ __forceinline__ void flush_stats(const KParams &kp, uint64_t tests, uint32_t nref, uint32_t nover,
                                             uint64_t flops = 0)
 {
@@ -1432,6 +1441,7 @@ __device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc)
             }
             CPTR(int32_t) sg = cst(G->bvc_seg) + b * (N_SEG + 1);
             Pending pd{pend, 0, NO_KEY, 0, tau, eps};
+            uint32_t csf = 0;     // the children's band stacks: flops per ray
             if (kp.stacks) {
                 // the children's cylinder runs: band stacks share the quadric
                 CPTR(int32_t) ro = cst(G->bvc_run_off) + b;
@@ -1439,7 +1449,7 @@ __device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc)
                 for (int j = ro[0]; j < j1; ++j) {
                     const CylRun run = ldc(cst(G->bvc_runs) + j);
                     if (run.stack)
-                        stack_loop<BNR, REFINE>(FC, cst(G->bvc_bands), run.begin, run.end, R, tb, ib, pd);
+                        csf += stack_loop<BNR, REFINE>(FC, cst(G->bvc_bands), run.begin, run.end, R, tb, ib, pd);
                     else
                         seg_loop<SEG_CYL, BNR, REFINE, false>(FC, run.begin, run.end, R, tb, ib, pd);
                 }
@@ -1457,7 +1467,7 @@ __device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc)
                     finish_dropped<BNR>(kp, G, E, gid, emitter, wi0, ray_of, FC, sg, sg + 1, tb, ib, pd, nref, nover);
             }
             const BvMeta meta = ldc(M + b);
-            const uint32_t cfl = cst(G->bv_cflops)[2 * b + (kp.stacks ? 1 : 0)];
+            const uint32_t cfl = cst(G->bv_cflops)[2 * b + (kp.stacks ? 1 : 0)] + csf;
 #pragma unroll
             for (int i = 0; i < BNR; ++i) {
                 if (valid[i]) {
@@ -1547,7 +1557,7 @@ __attribute__((amdgpu_waves_per_eu((NR == 4 || (MATH == KM_COMPAT && !BV)) ? 5 :
     uint32_t *bins = smem;
     uint32_t *pend = smem + (TRACE ? 0 : kp.bins);
     __shared__ int s_unit;
-    uint64_t tests = 0;
+    uint64_t tests = 0, sflops = 0;
     uint32_t nref = 0, nover = 0;
     const bool lds_tally = !TRACE && kp.bins > 0;
     for (;;) {
@@ -1578,7 +1588,7 @@ __attribute__((amdgpu_waves_per_eu((NR == 4 || (MATH == KM_COMPAT && !BV)) ? 5 :
             __syncthreads();
         }
         run_unit<MATH, BV, TRACE, NR, REFINE, false>(kp, G, gid, emitter, wi0, n_wi, threadIdx.x,
-                                                     blockDim.x, bins, pend, tests, nref, nover);
+                                                     blockDim.x, bins, pend, tests, sflops, nref, nover);
         if (lds_tally) {
             __syncthreads();
             unsigned long long *row = G->hits + (int64_t)emitter * n;
@@ -1589,7 +1599,7 @@ __attribute__((amdgpu_waves_per_eu((NR == 4 || (MATH == KM_COMPAT && !BV)) ? 5 :
         }
         __syncthreads();
     }
-    flush_stats(kp, tests, nref, nover);
+    flush_stats(kp, tests, nref, nover, sflops);
 }
 
 // Reference dispatch shape (RayFactorGPU.cl:481 launched as main.cpp:172-176):
@@ -1603,9 +1613,10 @@ __global__ void __launch_bounds__(256) k_direct(KParams kp)
     uint64_t tests = 0;
     uint32_t nref = 0, nover = 0;
     const int64_t wl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t sflops = 0;
     run_unit<MATH, BV, false, 1, true, true>(kp, G, kp.imp_geom_id, kp.imp_emitter, kp.imp_wi0, kp.imp_n_wi,
-                                             wl, kp.imp_n_wi + 1, nullptr, smem, tests, nref, nover);
-    flush_stats(kp, tests, nref, nover);
+                                             wl, kp.imp_n_wi + 1, nullptr, smem, tests, sflops, nref, nover);
+    flush_stats(kp, tests, nref, nover, sflops);
 }
 
 __global__ void k_fill_u32(unsigned int *p, unsigned int v, size_t n)

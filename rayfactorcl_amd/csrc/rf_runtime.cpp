@@ -87,7 +87,7 @@ struct rf_scene {
     GeomDesc *d_desc = nullptr;          // single-geometry descriptor
     GeomDesc desc{};                     // host copy
     int64_t flops_per_ray = 0;           // sum over primitives of F_type (plain kernels)
-    int64_t flops_per_ray_stacked = 0;   // the same with band stacks (kFlopsStack / kFlopsBand)
+    int64_t flops_per_ray_nonstack = 0;  // the same without the band stacks' cylinders (counted on the device)
     int64_t flops_bvs = 0;               // sum over bounding surfaces of F_type (BV tests per ray)
     CylRun *d_bvc_runs = nullptr;        // binned BV form: cylinder runs of each surface's children
     int32_t *d_bvc_run_off = nullptr;
@@ -322,8 +322,8 @@ static void quad_rows(const rf_primitive *p, float rows[4][4])
 // Cylinder runs over records [cb, ce) of a type-segmented array: maximal
 // groups of consecutive cylinders whose folded rows other than the z offset
 // r[11] are bitwise equal; two or more form a band stack, consecutive
-// singletons one per-test run.  Returns the change of algorithmic flops per
-// ray that the stacks bring (kFlopsStack + L band flops instead of L tests).
+// singletons one per-test run.  Returns minus the per-test flops per ray of
+// the stacked cylinders (the kernel counts a stack's flops on the device).
 static int64_t build_cyl_runs(const std::vector<FastPrim> &recs, int32_t cb, int32_t ce, std::vector<CylRun> &runs)
 {
     int64_t delta = 0;
@@ -337,8 +337,7 @@ static int64_t build_cyl_runs(const std::vector<FastPrim> &recs, int32_t cb, int
         int32_t q = stack_end(p);
         if (q - p >= 2) {
             runs.push_back(CylRun{p, q, 1, 0});
-            const bool axial = recs[p].r[8] == 0.0f && recs[p].r[9] == 0.0f;
-            delta += kFlopsStack + (q - p) * ((axial ? kFlopsBandAxial : kFlopsBand) - flops_per_test(RF_CYLINDER));
+            delta -= (q - p) * flops_per_test(RF_CYLINDER);
             p = q;
             continue;
         }
@@ -475,11 +474,11 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     // band stacks of the plain scan; band records by traversal position after
     // one pad record (the kernel's prefetch reads the record before a stack)
     std::vector<CylRun> runs;
-    const int64_t fstack = fsum + build_cyl_runs(fast, seg_begin[SEG_CYL], seg_end[SEG_CYL], runs);
+    const int64_t fnonstack = fsum + build_cyl_runs(fast, seg_begin[SEG_CYL], seg_end[SEG_CYL], runs);
     std::vector<BandRec> bands(n + 1, BandRec{0.f, 0});
     for (int32_t k = seg_begin[SEG_CYL]; k < seg_end[SEG_CYL]; ++k) bands[k + 1] = BandRec{fast[k].r[11], fast[k].orig};
     // the same for the children of each bounding surface (binned BV form), and
-    // the children's flops per ray (per test, stacked)
+    // the children's flops per ray (per test; with stacks, those outside them)
     std::vector<CylRun> bvc_runs;
     std::vector<int32_t> bvc_run_off;
     std::vector<BandRec> bvc_bands;
@@ -501,7 +500,7 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
         bvc_run_off[nbv] = (int32_t)bvc_runs.size();
     }
     s->flops_per_ray = fsum;
-    s->flops_per_ray_stacked = fstack;
+    s->flops_per_ray_nonstack = fnonstack;
     s->flops_per_test_avg = (double)(fsum + fbv) / (double)(n + nbv);
     s->flops_bvs = fbv;
 
@@ -901,7 +900,8 @@ static void account(const rf_ctx *ctx, const rf_scene *s, const rf_solve_opts *o
     t.rays += rays;
     if (!(o->use_bvs && s->n_bvs > 0)) {
         t.tests += rays * s->n;
-        t.flops += rays * (uses_stacks(ctx, o) ? s->flops_per_ray_stacked : s->flops_per_ray);
+        // band stacks add their flops from the device counter (stats[3])
+        t.flops += rays * (uses_stacks(ctx, o) ? s->flops_per_ray_nonstack : s->flops_per_ray);
     }
 }
 
@@ -1177,6 +1177,7 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
         if (rays && rays[g])
             for (int e = 0; e < scenes[g]->n; ++e) rays[g][e] = 4 * budgets[g][e];
     }
+    if (!o->use_bvs && uses_stacks(ctx, o)) t.flops += (int64_t)st[3];
     if (o->use_bvs) {
         t.tests += (int64_t)st[0];
         // the binned kernel counts its children's flops on the device (stats[3],

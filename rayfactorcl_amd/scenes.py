@@ -287,8 +287,8 @@ def sweep(n_geoms: int = 256, seed: int = 0x52465331) -> list[SceneData]:
 
 def streaming_requests(n: int, seed: int = 0x52465331, rate_hz: float = 2.0, max_prims: int = 2000,
                        min_rays: float = 1e5, max_rays: float = 1e8):
-    """Poisson-arrival solve requests: (arrival_s, scene, rays_per_emitter).
-    Scene sizes U{10..max_prims} drawn as a furnace or a soup; budgets
+    """Poisson-arrival solve requests: (arrival_s, scene, rays_per_emitter, kind).
+    Scene sizes U{10..max_prims} drawn as a furnace or a soup (kind); budgets
     log-U[min_rays, max_rays] (default [1e5, 1e8])."""
     rng = np.random.default_rng(seed)
     t = 0.0
@@ -299,9 +299,9 @@ def streaming_requests(n: int, seed: int = 0x52465331, rate_hz: float = 2.0, max
         if rng.uniform() < 0.5:
             wb = max(1, int(size * 0.75) - 3)
             pb = max(1, size - wb - 4)
-            sc = parse_xml(furnace_xml(wall_bands=wb, preform_bands=pb))
+            sc, kind = parse_xml(furnace_xml(wall_bands=wb, preform_bands=pb)), "furnace"
         else:
-            sc = parse_xml(soup_xml(size, seed=seed + i + 1))
+            sc, kind = parse_xml(soup_xml(size, seed=seed + i + 1)), "soup"
         rays = int(round(float(np.exp(rng.uniform(math.log(min_rays), math.log(max_rays))))))
-        out.append((t, sc, rays))
+        out.append((t, sc, rays, kind))
     return out
