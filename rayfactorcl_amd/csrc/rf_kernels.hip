@@ -470,27 +470,6 @@ __device__ __forceinline__ void seg_loop(CPTR(FastPrim) F, int begin, int end, c
     }
 }
 
-// Band stack [b, e) of the cylinder segment (CylRun.stack = 1): the records
-// are bitwise equal except the z offset r[11], so each ray's x-y quadric, its
-// roots and its z direction are computed once, with the operations of
-// fast_test<SEG_CYL>, and each band then needs only its z origin row, z at
-// both roots (and at the tangent point) and the range compares.  Every
-// per-band decision is fast_test's on that band's own record, bit for bit:
-//  - a root failing t > eps becomes NaN, so every compare on its z is false
-//    (fast_test's e1 / e2 terms); the tangent point likewise when |D| > tau A;
-//  - fast_test's candidate on a band is t1 if z1 is on it, else t2 if z2 is,
-//    and t1 <= t2 (sqrt >= 0, 1/A >= 0; t1 == t2 makes z1 == z2), so the
-//    nearest hit of the stack is t1 at the first band holding z1, else t2 at
-//    the first band holding z2 -- the bands are walked last to first and the
-//    last band seen holding each root is kept -- and it enters the running
-//    minimum with the strict < of the per-test scan;
-//  - fast_test's flag terms are evaluated on the same values; its rim term
-//    (|z| <= hi) ^ (|z| < lo) is lo <= |z| <= hi, here med3(|z|, lo, hi) == |z|
-//    (one vector compare instead of a compare and a scalar xor).
-// A flagged pair is queued as in seg_loop.  If the lane's list fills inside the
-// stack, the entries it queued from this stack are given back and the re-scan
-// starts at the stack's first band (resume = b x 8), so every flagged pair is
-// still re-solved exactly once.
 // Band loop of stack_loop, bands e-1 down to b.  Wave-uniform variants:
 //  AXIAL: the stack's z row has no x or y term (r[8] == r[9] == 0), so the
 //    band's z origin is one FMA; the per-test scan's two extra FMAs add a
@@ -587,6 +566,27 @@ __device__ __forceinline__ void stack_bands_sel(bool tan, bool root1, const Fast
     }
 }
 
+// Band stack [b, e) of the cylinder segment (CylRun.stack = 1): the records
+// are bitwise equal except the z offset r[11], so each ray's x-y quadric, its
+// roots and its z direction are computed once, with the operations of
+// fast_test<SEG_CYL>, and each band then needs only its z origin row, z at
+// both roots (and at the tangent point) and the range compares.  Every
+// per-band decision is fast_test's on that band's own record, bit for bit:
+//  - a root failing t > eps becomes NaN, so every compare on its z is false
+//    (fast_test's e1 / e2 terms); the tangent point likewise when |D| > tau A;
+//  - fast_test's candidate on a band is t1 if z1 is on it, else t2 if z2 is,
+//    and t1 <= t2 (sqrt >= 0, 1/A >= 0; t1 == t2 makes z1 == z2), so the
+//    nearest hit of the stack is t1 at the first band holding z1, else t2 at
+//    the first band holding z2 -- the bands are walked last to first and the
+//    last band seen holding each root is kept -- and it enters the running
+//    minimum with the strict < of the per-test scan;
+//  - fast_test's flag terms are evaluated on the same values; its rim term
+//    (|z| <= hi) ^ (|z| < lo) is lo <= |z| <= hi, here med3(|z|, lo, hi) == |z|
+//    (one vector compare instead of a compare and a scalar xor).
+// A flagged pair is queued as in seg_loop.  If the lane's list fills inside the
+// stack, the entries it queued from this stack are given back and the re-scan
+// starts at the stack's first band (resume = b x 8), so every flagged pair is
+// still re-solved exactly once.
 // Returns the stack's algorithmic flops per ray for the band loop the wave ran.
 template <int NR, bool REFINE>
 __device__ __forceinline__ uint32_t stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, int b, int e,
