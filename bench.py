@@ -2,14 +2,15 @@
 """Benchmark: ray-primitive tests/s and rays/s of the view-factor engine on one MI355X.
 
 Workload (default): the batched design sweep -- 256 coaxial furnace
-geometries (34-605 primitives each) with 1e6 rays per emitter, solved in the
-FP32 + FP64-refined form by one persistent launch per step.  A step is one
-full solve of all 256 geometries (every emitter, every ray).  Beside it the
-same math runs in the plain one-geometry-per-launch form on a sample of the
-geometries, and the CPU oracle runs a bounded sample on the host cores.
+geometries (34-557 primitives each) with 1e6 rays per emitter, solved in the
+FP32 + FP64-refined form with band stacks by one persistent launch per step.
+A step is one full solve of all 256 geometries (every emitter, every ray).
+Beside it, on a sample of the geometries: the same math in the plain
+one-launch-per-emitter forms, the per-test form (band stacks off), and the
+fast form's view-factor deviation from all-FP64; the CPU oracle runs a
+bounded sample on the host cores.
 
-Prints ONE JSON line (see the driver contract).  --gpus N > 1 is refused:
-this build runs on one GPU.
+Prints ONE JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
 from __future__ import annotations
 
@@ -112,6 +113,8 @@ def main() -> int:
     ap.add_argument("--rays", type=float, default=1e6, help="rays per emitter")
     ap.add_argument("--seed", type=int, default=0x52465331)
     ap.add_argument("--plain-geoms", type=int, default=16, help="geometries in the plain-form comparison")
+    ap.add_argument("--precision-geoms", type=int, default=4,
+                    help="geometries of the fast-vs-FP64 view-factor check (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -236,6 +239,21 @@ def main() -> int:
             "batched_over_per_test_form": res["batched"]["tests_per_s"] / res["per_test"]["tests_per_s"],
         }
 
+    # the metric's precision condition: the timed form against all-FP64 on the
+    # same Philox streams, on a sample of the geometries at 2^18 rays/emitter
+    precision = None
+    if args.precision_geoms > 0:
+        idx = np.linspace(0, len(dev) - 1, min(args.precision_geoms, len(dev))).astype(int)
+        rp = 1 << 18
+        dv = 0.0
+        for i in idx:
+            kw = dict(rng="seeded", rays_per_emitter=rp, form="persistent", seed=args.seed, geometry=int(i))
+            a = rf.solve(ctx, dev[i], math="fast", refine=refine, per_test=args.per_test, **kw)
+            b = rf.solve(ctx, dev[i], math="fp64", **kw)
+            dv = max(dv, float(np.abs(a.hits - b.hits).max()) / rp)
+        precision = {"max_dvf_vs_fp64": dv, "geometries": len(idx), "rays_per_emitter": rp}
+        log(f"precision: max |dVF| fast vs fp64 = {dv:.2e} over {len(idx)} geometries")
+
     traffic = None
     form = "per_test" if args.per_test else "stacked"
     tname = f"hbm_traffic_{args.workload}{args.geoms if args.workload == 'sweep' else ''}"
@@ -272,6 +290,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": desc, "geometries": len(geoms), "rays_per_emitter": rays,
                    "primitives_total": int(sum(g.n for g in geoms)), "form": "persistent",
+                   "cylinder_tests": ("every test in full" if args.per_test else
+                                      "band stacks: one quadric per ray and stack, a z-range test per band"),
                    "math": "fp32 fast + fp64 refine" if refine else "fp32 fast", "rng": "philox4x32-10 seeded",
                    "parallelism": "single GPU"},
         "rays_per_s": nrays / wall,
@@ -281,6 +301,7 @@ def main() -> int:
                      "kernel": kernel, "avg_launch_ms": kernel_ms / max(launches, 1),
                      "flops_per_launch": flops / max(launches, 1)},
         "plain_comparison": plain,
+        "precision": precision,
         "cpu_baseline": cpu,
         "device": {"arch": ctx.arch, "cus": ctx.n_cu, "clock_mhz": ctx.clock_mhz},
     }
