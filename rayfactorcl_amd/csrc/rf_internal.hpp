@@ -82,6 +82,13 @@ struct GeomDesc {
     const int32_t *bvc_run_off;
     const BandRec *bvc_bands;
     const uint32_t *bv_cflops;
+    // compat scan with band stacks: runs over the caller's order (stacks of
+    // consecutive cylinders whose invm differs only in invm[11]), band records
+    // {invm[11], index} by caller position after one pad record
+    const CylRun *c_runs;
+    const BandRec *c_bands;
+    int32_t n_c_runs;
+    int32_t pad3;
 };
 
 // One work unit: work items [wi0, wi0+n_wi) of one emitter of one geometry.
@@ -148,6 +155,7 @@ hipError_t launch_selftest_math(unsigned long long *counts, unsigned int *first,
 // when the z row has no x or y term) and z at each root the wave evaluates
 // (root 1 is skipped when no ray of the wave has it ahead).
 constexpr uint32_t kFlopsStack = 45;
+constexpr uint32_t kFlopsStackC = 48;     // compat: cylinderHit's own A, B, C and roots
 constexpr uint32_t kFlopsBandZ0 = 6;
 constexpr uint32_t kFlopsBandZ0Axial = 2;
 constexpr uint32_t kFlopsBandRoot = 2;

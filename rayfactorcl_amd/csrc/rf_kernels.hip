@@ -197,17 +197,15 @@ __device__ __forceinline__ void scan_c_type(const float (&m)[12], float u1, floa
     }
 }
 
-// Compat scan of the NR rays of one work item: each record is read once,
-// copied to VGPRs and tested against every ray with the reference's
-// arithmetic (hit_c_ops, IEEE-exact); caller order and strict < keep the reference's tie
-// rule (RayFactorGPU.cl:445-472), so each ray's result is find_c's.
+// Compat scan of the NR rays of one work item over records [start, end):
+// each record is read once, copied to VGPRs and tested against every ray with
+// the reference's arithmetic (hit_c_ops, IEEE-exact); caller order and strict
+// < keep the reference's tie rule (RayFactorGPU.cl:445-472), so each ray's
+// result is find_c's.  tbest / obj carry the running nearest hit.
 template <int NR>
-__device__ __forceinline__ void find_c_multi(CPTR(rf_primitive) P, int start, int end, const Ray32 (&ray)[NR],
-                                             int (&obj)[NR])
+__device__ __forceinline__ void scan_c_range(CPTR(rf_primitive) P, int start, int end, const Ray32 (&ray)[NR],
+                                             float (&tbest)[NR], int (&obj)[NR])
 {
-    float tbest[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) { tbest[r] = finf(); obj[r] = -1; }
     for (int w = start; w < end; ++w) {
         const rf_primitive p = ldc(P + w);
         float m[12];
@@ -230,6 +228,127 @@ __device__ __forceinline__ void find_c_multi(CPTR(rf_primitive) P, int start, in
         for (int r = 0; r < NR; ++r)
             if (t[r] < tbest[r]) { tbest[r] = t[r]; obj[r] = w; }
     }
+}
+
+// Band loop of stack_c, bands e-1 down to b (AXIAL: wave-uniform choice of the
+// one-FMA z origin).
+template <int NR, bool AXIAL>
+__device__ __forceinline__ void stack_c_bands(const float (&m)[12], CPTR(BandRec) BR, int b, int e,
+                                              const Ray32 (&ray)[NR], const float (&P1)[NR], const float (&P2)[NR],
+                                              int (&I1)[NR], int (&I2)[NR])
+{
+    // BR[-1] exists (pad record): the next record's load needs no clamp
+    CPTR(BandRec) q = BR + (e - 1);
+    BandRec next = ldc(q);
+    for (int wb = e - 1; wb >= b; --wb) {
+        float m11, of;
+        vcopy2(next.r11, __int_as_float(next.orig), m11, of);
+        const int orig = __float_as_int(of);
+        --q;
+        next = ldc(q);
+        // keeps the next record's load here, a band ahead of its use
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const Ray32 &w = ray[r];
+            const float sz = AXIAL ? __builtin_fmaf(m[10], w.sz, m11)
+                                   : __builtin_fmaf(m[8], w.sx, __builtin_fmaf(m[9], w.sy, __builtin_fmaf(m[10], w.sz, m11)));
+            const float z1 = sz + P1[r];
+            const float z2 = sz + P2[r];
+            const bool v1 = __builtin_amdgcn_fmed3f(z1, 0.0f, 1.0f) == z1;     // z1 in [0, 1]
+            const bool v2 = (z2 > 0.0f) & (z2 < 1.0f);
+            I1[r] = v1 ? orig : I1[r];
+            I2[r] = v2 ? orig : I2[r];
+        }
+    }
+}
+
+// Compat band stack [b, e) in the caller's order: consecutive cylinders whose
+// invm rows are bitwise equal except invm[11].  cylinderHit (:189-222, here
+// hit_c_ops / roots_select_c) then computes the same A, B, C, discriminant,
+// roots and o.cz * t for every band; a band differs only in o.sz.  Per band
+// the reference returns t1 when t1 >= EPS and z1 in [0, 1], else t2 when
+// t2 < inf, t2 > EPS and z2 in (0, 1) (its t2 < t1 test can only pass once t1
+// is set to inf: 1/A >= 0 orders t1 <= t2), else inf; a NaN t1 is returned as
+// it is and never wins.  With t1 <= t2 (and z1 == z2 when they are equal) the
+// stack's nearest hit is t1 at the first band holding z1, else t2 at the first
+// band holding z2, found walking the bands last to first as stack_loop does.
+// The root conditions are folded into NaN products, which fail every range
+// test.  An axial z row (invm[8] == invm[9] == 0) takes one FMA for o.sz: the
+// two dropped FMAs add a signed zero, which no range test can tell apart.
+// Returns the stack's algorithmic flops per ray.
+template <int NR>
+__device__ __forceinline__ uint32_t stack_c(CPTR(rf_primitive) P, CPTR(BandRec) BR, int b, int e,
+                                            const Ray32 (&ray)[NR], float (&tbest)[NR], int (&obj)[NR])
+{
+    const rf_primitive hs = ldc(P + b);
+    float m[12];
+#pragma unroll
+    for (int k = 0; k < 12; k += 2) vcopy2(hs.invm[k], hs.invm[k + 1], m[k], m[k + 1]);
+    const bool axial = ((__float_as_uint(hs.invm[8]) | __float_as_uint(hs.invm[9])) << 1) == 0u;
+    const float qnan = __builtin_nanf("");
+    float T1[NR], T2[NR], P1[NR], P2[NR];
+    int I1[NR], I2[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const Ray32 &w = ray[r];
+        const float sx = __builtin_fmaf(m[0], w.sx, __builtin_fmaf(m[1], w.sy, __builtin_fmaf(m[2], w.sz, m[3])));
+        const float cx = __builtin_fmaf(m[0], w.cx, __builtin_fmaf(m[1], w.cy, m[2] * w.cz));
+        const float sy = __builtin_fmaf(m[4], w.sx, __builtin_fmaf(m[5], w.sy, __builtin_fmaf(m[6], w.sz, m[7])));
+        const float cy = __builtin_fmaf(m[4], w.cx, __builtin_fmaf(m[5], w.cy, m[6] * w.cz));
+        const float cz = __builtin_fmaf(m[8], w.cx, __builtin_fmaf(m[9], w.cy, m[10] * w.cz));
+        const float A = __builtin_fmaf(cx, cx, cy * cy);
+        const float B = -__builtin_fmaf(sx, cx, sy * cy);
+        const float C = __builtin_fmaf(sx, sx, __builtin_fmaf(sy, sy, -1.f));
+        const float discrim = sqrt_ieee(B * B - A * C);
+        const float rA = recip_ieee(A);
+        const float t1 = rA * (B - discrim);
+        const float t2 = rA * (B + discrim);
+        float p1 = cz * t1, p2 = cz * t2;
+        const bool nan1 = t1 != t1;                              // no band returns a usable t
+        p1 = (nan1 || t1 < EPS_F) ? qnan : p1;
+        p2 = (nan1 || !(t2 < finf()) || !(t2 > EPS_F)) ? qnan : p2;
+        T1[r] = t1;
+        T2[r] = t2;
+        P1[r] = p1;
+        P2[r] = p2;
+        asm volatile("" : "+v"(P1[r]), "+v"(P2[r]));
+        I1[r] = -1;
+        I2[r] = -1;
+    }
+    if (axial) stack_c_bands<NR, true>(m, BR, b, e, ray, P1, P2, I1, I2);
+    else stack_c_bands<NR, false>(m, BR, b, e, ray, P1, P2, I1, I2);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const bool h1 = I1[r] >= 0;
+        const float c = h1 ? T1[r] : (I2[r] >= 0 ? T2[r] : finf());
+        const int ci = h1 ? I1[r] : I2[r];
+        if (c < tbest[r]) { tbest[r] = c; obj[r] = ci; }
+    }
+    return kFlopsStackC + (uint32_t)(e - b) * ((axial ? kFlopsBandZ0Axial : kFlopsBandZ0) + 2u);
+}
+
+// The compat scan of one work item's NR rays over the whole scene: band stacks
+// (G->c_runs) when `stacks`, else one per-test range.  Returns the stacks'
+// algorithmic flops per ray.
+template <int NR>
+__device__ __forceinline__ uint32_t find_c_multi(CPTR(GeomDesc) G, const Ray32 (&ray)[NR], int (&obj)[NR], int stacks)
+{
+    CPTR(rf_primitive) P = cst(G->prims);
+    float tbest[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) { tbest[r] = finf(); obj[r] = -1; }
+    uint32_t sf = 0;
+    const int n_runs = stacks ? G->n_c_runs : 1;
+    for (int j = 0; j < n_runs; ++j) {
+        CylRun run{0, G->n, 0, 0};
+        if (stacks) run = ldc(cst(G->c_runs) + j);
+        if (run.stack)
+            sf += stack_c<NR>(P, cst(G->c_bands), run.begin, run.end, ray, tbest, obj);
+        else
+            scan_c_range<NR>(P, run.begin, run.end, ray, tbest, obj);
+    }
+    return sf;
 }
 
 #define ROW_S(P, i, R) __builtin_fmaf(P.r[4 * i], R.sx, __builtin_fmaf(P.r[4 * i + 1], R.sy, __builtin_fmaf(P.r[4 * i + 2], R.sz, P.r[4 * i + 3])))
@@ -902,7 +1021,7 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                     rotate_rands(f, v, r4);
                     R4[v] = emit_c(&E, r4);
                 }
-                find_c_multi<4>(cst(G->prims), 0, G->n, R4, idx4);
+                sflops += 4u * find_c_multi<4>(G, R4, idx4, kp.stacks);
 #pragma unroll
                 for (int v = 0; v < 4; ++v) tally<TRACE, DIRECT>(kp, G, emitter, ray0 + wl * 4 + v, idx4[v], bins);
                 continue;

@@ -88,6 +88,9 @@ struct rf_scene {
     GeomDesc desc{};                     // host copy
     int64_t flops_per_ray = 0;           // sum over primitives of F_type (plain kernels)
     int64_t flops_per_ray_nonstack = 0;  // the same without the band stacks' cylinders (counted on the device)
+    int64_t flops_per_ray_cnonstack = 0; // the same for the compat scan's stacks (caller order)
+    CylRun *d_c_runs = nullptr;          // compat scan: runs over the caller's order
+    BandRec *d_c_bands = nullptr;
     int64_t flops_bvs = 0;               // sum over bounding surfaces of F_type (BV tests per ray)
     CylRun *d_bvc_runs = nullptr;        // binned BV form: cylinder runs of each surface's children
     int32_t *d_bvc_run_off = nullptr;
@@ -324,6 +327,38 @@ static void quad_rows(const rf_primitive *p, float rows[4][4])
 // r[11] are bitwise equal; two or more form a band stack, consecutive
 // singletons one per-test run.  Returns minus the per-test flops per ray of
 // the stacked cylinders (the kernel counts a stack's flops on the device).
+static int64_t build_cyl_runs(const std::vector<FastPrim> &recs, int32_t cb, int32_t ce, std::vector<CylRun> &runs);
+
+// The same over the caller's order for the compat scan: a stack is two or
+// more consecutive cylinders with bitwise equal invm[0..10]; everything else
+// forms per-test runs.  Returns minus the per-test flops of the stacked cylinders.
+static int64_t build_compat_runs(const rf_primitive *prims, int32_t n, std::vector<CylRun> &runs)
+{
+    int64_t delta = 0;
+    auto stack_end = [&](int32_t p) {
+        int32_t q = p + 1;
+        if (prims[p].type != RF_CYLINDER) return q;
+        while (q < n && prims[q].type == RF_CYLINDER &&
+               std::memcmp(prims[q].invm, prims[p].invm, 11 * sizeof(float)) == 0)
+            ++q;
+        return q;
+    };
+    int32_t p = 0;
+    while (p < n) {
+        int32_t q = stack_end(p);
+        if (q - p >= 2) {
+            runs.push_back(CylRun{p, q, 1, 0});
+            delta -= (q - p) * flops_per_test(RF_CYLINDER);
+            p = q;
+            continue;
+        }
+        const int32_t s0 = p;
+        while (p < n && (q = stack_end(p)) - p < 2) p = q;
+        runs.push_back(CylRun{s0, p, 0, 0});
+    }
+    return delta;
+}
+
 static int64_t build_cyl_runs(const std::vector<FastPrim> &recs, int32_t cb, int32_t ce, std::vector<CylRun> &runs)
 {
     int64_t delta = 0;
@@ -351,8 +386,8 @@ static int64_t build_cyl_runs(const std::vector<FastPrim> &recs, int32_t cb, int
 extern "C" int64_t rf_scene_footprint(int32_t n, int32_t nbv)
 {
     if (n < 0 || nbv < 0) return -1;
-    return (int64_t)n * (sizeof(rf_primitive) + 2 * sizeof(FastPrim) + 64 + 8 + sizeof(BandRec) +
-                         sizeof(CylRun)) + 6 * 256 +
+    return (int64_t)n * (sizeof(rf_primitive) + 2 * sizeof(FastPrim) + 64 + 8 + 2 * sizeof(BandRec) +
+                         2 * sizeof(CylRun)) + 8 * 256 +
            (int64_t)nbv * (sizeof(rf_bounding_volume) + sizeof(FastPrim) + sizeof(BvMeta) + (N_SEG + 1) * 4 +
                            4 + 8) +
            (nbv ? (int64_t)n * (sizeof(FastPrim) + sizeof(BandRec) + sizeof(CylRun)) + 6 * 256 : 0) +
@@ -364,6 +399,7 @@ static void free_scene_mem(rf_scene *s)
     (void)hipFree(s->d_quads); (void)hipFree(s->d_pos2orig); (void)hipFree(s->d_orig2pos);
     (void)hipFree(s->d_bvc_fast); (void)hipFree(s->d_bvc_seg);
     (void)hipFree(s->d_cyl_runs); (void)hipFree(s->d_bands);
+    (void)hipFree(s->d_c_runs); (void)hipFree(s->d_c_bands);
     (void)hipFree(s->d_bvc_runs); (void)hipFree(s->d_bvc_run_off); (void)hipFree(s->d_bvc_bands);
     (void)hipFree(s->d_bv_cflops);
     (void)hipFree(s->d_prims); (void)hipFree(s->d_fast); (void)hipFree(s->d_fast_orig);
@@ -477,6 +513,11 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     const int64_t fnonstack = fsum + build_cyl_runs(fast, seg_begin[SEG_CYL], seg_end[SEG_CYL], runs);
     std::vector<BandRec> bands(n + 1, BandRec{0.f, 0});
     for (int32_t k = seg_begin[SEG_CYL]; k < seg_end[SEG_CYL]; ++k) bands[k + 1] = BandRec{fast[k].r[11], fast[k].orig};
+    // compat scan: stacks over the caller's order, band records {invm[11], index}
+    std::vector<CylRun> c_runs;
+    const int64_t fcnonstack = fsum + build_compat_runs(prims, n, c_runs);
+    std::vector<BandRec> c_bands(n + 1, BandRec{0.f, 0});
+    for (int32_t k = 0; k < n; ++k) c_bands[k + 1] = BandRec{prims[k].invm[11], k};
     // the same for the children of each bounding surface (binned BV form), and
     // the children's flops per ray (per test; with stacks, those outside them)
     std::vector<CylRun> bvc_runs;
@@ -501,6 +542,7 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     }
     s->flops_per_ray = fsum;
     s->flops_per_ray_nonstack = fnonstack;
+    s->flops_per_ray_cnonstack = fcnonstack;
     s->flops_per_test_avg = (double)(fsum + fbv) / (double)(n + nbv);
     s->flops_bvs = fbv;
 
@@ -522,6 +564,8 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     up((void **)&s->d_pos2orig, pos2orig.data(), pos2orig.size() * sizeof(int32_t));
     up((void **)&s->d_orig2pos, orig2pos.data(), orig2pos.size() * sizeof(int32_t));
     up((void **)&s->d_cyl_runs, runs.data(), runs.size() * sizeof(CylRun));
+    up((void **)&s->d_c_runs, c_runs.data(), c_runs.size() * sizeof(CylRun));
+    up((void **)&s->d_c_bands, c_bands.data(), c_bands.size() * sizeof(BandRec));
     up((void **)&s->d_bands, bands.data(), bands.size() * sizeof(BandRec));
     if (binned) {
         up((void **)&s->d_bvc_runs, bvc_runs.data(), bvc_runs.size() * sizeof(CylRun));
@@ -543,6 +587,8 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     d.bvc_fast = s->d_bvc_fast; d.bvc_seg = s->d_bvc_seg; d.bv_binned = binned ? 1 : 0;
     d.n_cyl_runs = (int32_t)runs.size(); d.cyl_runs = s->d_cyl_runs;
     d.bands = s->d_bands ? s->d_bands + 1 : nullptr;
+    d.n_c_runs = (int32_t)c_runs.size(); d.c_runs = s->d_c_runs;
+    d.c_bands = s->d_c_bands ? s->d_c_bands + 1 : nullptr;
     d.bvc_runs = s->d_bvc_runs; d.bvc_run_off = s->d_bvc_run_off;
     d.bvc_bands = s->d_bvc_bands ? s->d_bvc_bands + 1 : nullptr;
     d.bv_cflops = s->d_bv_cflops;
@@ -671,13 +717,14 @@ static void pick_shape(const rf_ctx *ctx, const rf_solve_opts *o, int &nr, int &
 }
 
 // Band stacks run in the fast form with the VALU transform (plain scan, and
-// the children of the binned BV kernel) unless the caller asks for every test
-// in full.
+// the children of the binned BV kernel) and in the compat plain scan, unless
+// the caller asks for every test in full.
 static bool uses_stacks(const rf_ctx *ctx, const rf_solve_opts *o)
 {
     int nr, block, mfma;
     pick_shape(ctx, o, nr, block, mfma);
-    return o->math == RF_MATH_FAST && !mfma && !o->per_test;
+    if (o->per_test) return false;
+    return (o->math == RF_MATH_FAST && !mfma) || o->math == RF_MATH_COMPAT;
 }
 
 static int ensure_queue(rf_ctx *ctx)
@@ -901,7 +948,9 @@ static void account(const rf_ctx *ctx, const rf_scene *s, const rf_solve_opts *o
     if (!(o->use_bvs && s->n_bvs > 0)) {
         t.tests += rays * s->n;
         // band stacks add their flops from the device counter (stats[3])
-        t.flops += rays * (uses_stacks(ctx, o) ? s->flops_per_ray_nonstack : s->flops_per_ray);
+        const int64_t f = !uses_stacks(ctx, o) ? s->flops_per_ray
+                          : (o->math == RF_MATH_COMPAT ? s->flops_per_ray_cnonstack : s->flops_per_ray_nonstack);
+        t.flops += rays * f;
     }
 }
 

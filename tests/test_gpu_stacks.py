@@ -180,3 +180,47 @@ def test_stacks_bv_overlapping_children(ctx):
     kw = dict(math="fast", rng="seeded", use_bvs=True, rays_per_emitter=1 << 14, form="persistent")
     _same(rf.solve(ctx, dev, per_test=True, **kw), rf.solve(ctx, dev, per_test=False, **kw))
     dev.close()
+
+
+# ---------------------------------------------------------------------------
+# Compat (reference arithmetic): band stacks over the caller's order, checked
+# bit for bit against the CPU oracle's per-test restatement of findObject.
+
+@pytest.mark.parametrize("scene", ["edge", "tilted", "furnace"])
+def test_compat_stacks_match_oracle(ctx, oracle, scene):
+    if scene == "edge":
+        sc, n_wi = _edge_scene(), 2048
+    elif scene == "tilted":
+        sc, n_wi = _tilted(scenes.parse_xml(scenes.furnace_xml(wall_bands=24, preform_bands=8))), 256
+    else:
+        sc, n_wi = scenes.parse_xml(scenes.furnace_xml(wall_bands=60, preform_bands=20)), 128
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    got = rf.solve(ctx, dev, math="compat", rays_per_emitter=4 * n_wi)
+    want, _ = oracle.solve(sc.prims, None, n_wi=n_wi, mode="compat")
+    diff = np.argwhere(got.hits != want)
+    assert diff.size == 0, f"{len(diff)} cells differ, first {diff[:5].tolist()}"
+    assert got.timing["flops"] > 0
+    dev.close()
+
+
+@pytest.mark.parametrize("form", ["persistent", "per_emitter", "reference"])
+def test_compat_stacks_equal_per_test(ctx, form):
+    geoms = scenes.sweep(3, seed=0x52465331)
+    for g in geoms:
+        dev = rf.DeviceScene(ctx, g.prims, None)
+        kw = dict(math="compat", rng="seeded", rays_per_emitter=1 << 13, form=form)
+        a = rf.solve(ctx, dev, per_test=True, **kw)
+        b = rf.solve(ctx, dev, per_test=False, **kw)
+        assert np.array_equal(a.hits, b.hits)
+        assert b.timing["flops"] < a.timing["flops"]
+        dev.close()
+
+
+def test_compat_stacks_traces(ctx, oracle):
+    sc = _edge_scene()
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    for e in (0, 5, 8, 12):
+        a = rf.trace(ctx, dev, e, 100, 1500, math="compat")
+        b = oracle.trace(sc.prims, None, e, 100, 1500, mode="compat")
+        assert np.array_equal(a, b), (e, np.argwhere(a != b)[:5])
+    dev.close()
