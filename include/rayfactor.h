@@ -123,10 +123,11 @@ typedef struct rf_solve_opts {
     int32_t transform;        /* RF_MATH_FAST plain form: rf_transform */
     float refine_margin;      /* RF_MATH_FAST: relative margin of the FP64 re-solve flags, 0 = 2^-14;
                                  larger re-solves more pairs (at most 0.25) */
-    int32_t per_test;         /* RF_MATH_FAST plain form: 0 = a band stack (consecutive cylinders whose
-                                 transforms differ only in the z offset, e.g. the bands of one wall)
-                                 solves its quadric once per ray and tests each band's z range;
-                                 1 = every cylinder test in full.  Same tallies either way. */
+    int32_t per_test;         /* Scans without bounding surfaces (and the fast form's binned BV scan):
+                                 0 = a band stack (consecutive cylinders whose transforms differ only
+                                 in the z offset, e.g. the bands of one wall) solves its quadric once
+                                 per ray and tests each band's z range; 1 = every cylinder test in
+                                 full.  Same tallies either way, in every arithmetic form. */
 } rf_solve_opts;
 
 typedef enum rf_transform {

@@ -124,7 +124,7 @@ struct KParams {
     int64_t imp_unit;
     float tau;                          // fast form: relative margin of the FP64 re-solve flags
     int32_t bv_slots;                   // binned BV kernel: LDS buckets (>= n_bvs of every geometry)
-    int32_t stacks;                     // fast plain form: 1 = band stacks share the quadric (CylRun)
+    int32_t stacks;                     // plain (non-BV) forms: 1 = band stacks share the quadric (CylRun)
 };
 
 enum KMath { KM_COMPAT = 0, KM_FP64 = 1, KM_FAST = 2 };
@@ -159,6 +159,10 @@ constexpr uint32_t kFlopsStackC = 48;     // compat: cylinderHit's own A, B, C a
 constexpr uint32_t kFlopsBandZ0 = 6;
 constexpr uint32_t kFlopsBandZ0Axial = 2;
 constexpr uint32_t kFlopsBandRoot = 2;
+// FP64 form: per (ray, stack) the shared rows, A, B, C, D, sqrt, q, both
+// roots and c_z t at each; per (ray, band) z origin and z at both roots.
+constexpr uint32_t kFlopsStackD = 51;
+constexpr uint32_t kFlopsBandD = 3;
 
 // Algorithmic FP32 flops per ray-primitive test (counted from the reference
 // formulas: FMA = 2, mul/add = 1, rcp/sqrt = 1, compare/select = 0).

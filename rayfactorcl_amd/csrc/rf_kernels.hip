@@ -95,6 +95,96 @@ __device__ __forceinline__ int find_d(CPTR(rf_primitive) P, int start, int end, 
     return obj;
 }
 
+// FP64 band stack [b, e) in the caller's order (the compat stacks: consecutive
+// cylinders whose invm differs only in invm[11]).  hit_d's cylinder
+// (quad_select_d) has the same A, B, C, roots and o.cz * t for every band, and
+// o.sz = ((c0 sx + c1 sy) + c2 sz) + c3 differs by the last term only.  Per
+// band it returns the lower root when it lies past EPS with z in [0, 1], else
+// the higher root under the same test, else inf; with lo <= hi the stack's
+// nearest hit is lo at the first band holding z(lo), else hi at the first band
+// holding z(hi) (bands walked last to first, as stack_loop).  A root that is
+// absent or not past EPS becomes a NaN product, which fails every range test.
+__device__ __forceinline__ uint32_t stack_d(CPTR(rf_primitive) P, CPTR(BandRec) BR, int b, int e,
+                                            const Ray64 &w, double &tbest, int &obj)
+{
+    const rf_primitive hs = ldc(P + b);
+    const float *iv = hs.invm;
+    const double a0 = iv[0], a1 = iv[1], a2 = iv[2], a3 = iv[3];
+    const double b0 = iv[4], b1 = iv[5], b2 = iv[6], b3 = iv[7];
+    const double c0 = iv[8], c1 = iv[9], c2 = iv[10];
+    const double sx = ((a0 * w.sx + a1 * w.sy) + a2 * w.sz) + a3;
+    const double cx = (a0 * w.cx + a1 * w.cy) + a2 * w.cz;
+    const double sy = ((b0 * w.sx + b1 * w.sy) + b2 * w.sz) + b3;
+    const double cy = (b0 * w.cx + b1 * w.cy) + b2 * w.cz;
+    const double S = (c0 * w.sx + c1 * w.sy) + c2 * w.sz;
+    const double cz = (c0 * w.cx + c1 * w.cy) + c2 * w.cz;
+    const double A = cx * cx + cy * cy;
+    const double B = sx * cx + sy * cy;
+    const double C = (sx * sx + sy * sy) - 1.0;
+    const double qnan = __builtin_nan("");
+    double r1 = qnan, r2 = qnan;
+    if (A == 0.0) {
+        if (B != 0.0) r1 = r2 = -C / (2.0 * B);
+    } else {
+        const double D = B * B - A * C;
+        if (!(D < 0.0)) {
+            const double sq = __builtin_sqrt(D);
+            const double q = -(B + (B >= 0.0 ? sq : -sq));
+            if (q != 0.0) {
+                r1 = q / A;
+                r2 = C / q;
+            }
+        }
+    }
+    const double lo = r1 < r2 ? r1 : r2;
+    const double hi = r1 < r2 ? r2 : r1;
+    const double plo = lo > EPS_D ? cz * lo : qnan;
+    const double phi = hi > EPS_D ? cz * hi : qnan;
+    int ilo = -1, ihi = -1;
+    CPTR(BandRec) q = BR + (e - 1);
+    BandRec next = ldc(q);
+    for (int wb = e - 1; wb >= b; --wb) {
+        const double c3 = next.r11;
+        const int orig = next.orig;
+        --q;
+        next = ldc(q);
+        asm volatile("" ::: "memory");
+        const double sz = S + c3;
+        const double zlo = sz + plo;
+        const double zhi = sz + phi;
+        ilo = (zlo >= 0.0 && zlo <= 1.0) ? orig : ilo;
+        ihi = (zhi >= 0.0 && zhi <= 1.0) ? orig : ihi;
+    }
+    const double c = ilo >= 0 ? lo : (ihi >= 0 ? hi : dinf());
+    const int ci = ilo >= 0 ? ilo : ihi;
+    if (c < tbest) { tbest = c; obj = ci; }
+    return kFlopsStackD + (uint32_t)(e - b) * kFlopsBandD;
+}
+
+// findObject in the FP64 form with the compat runs (band stacks) when
+// `stacks`; adds the stacks' algorithmic flops to `sflops`.
+__device__ __forceinline__ int find_d_runs(CPTR(GeomDesc) G, const Ray64 &ray, int stacks, uint64_t &sflops)
+{
+    CPTR(rf_primitive) P = cst(G->prims);
+    if (!stacks) return find_d(P, 0, G->n, ray);
+    int obj = -1;
+    double tbest = dinf();
+    const int n_runs = G->n_c_runs;
+    for (int j = 0; j < n_runs; ++j) {
+        const CylRun run = ldc(cst(G->c_runs) + j);
+        if (run.stack) {
+            sflops += stack_d(P, cst(G->c_bands), run.begin, run.end, ray, tbest, obj);
+        } else {
+            for (int w = run.begin; w < run.end; ++w) {
+                const rf_primitive p = ldc(P + w);
+                const double t = hit_d(p.type, p.invm, p.util1, p.util2, ray);
+                if (t < tbest) { tbest = t; obj = w; }
+            }
+        }
+    }
+    return obj;
+}
+
 // BV body shared by compat and fp64 (MATH selects the arithmetic).  The
 // final child scan of lanes whose winning bounding surface differs is done
 // one distinct surface at a time: the leader's surface is read with
@@ -1038,7 +1128,7 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                     const double d4[4] = {r4[0], r4[1], r4[2], r4[3]};
                     const Ray64 ray = emit_d(&E, d4);
                     idx = BV ? find_bv_cd<KM_FP64>(cst(G->prims), cst(G->bvs), G->n_bvs, ray, tests)
-                             : find_d(cst(G->prims), 0, G->n, ray);
+                             : find_d_runs(G, ray, kp.stacks, sflops);
                 } else {
                     const Ray32 ray = emit_fast(&E, r4);
                     if constexpr (BV) {

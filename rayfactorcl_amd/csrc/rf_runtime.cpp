@@ -724,7 +724,7 @@ static bool uses_stacks(const rf_ctx *ctx, const rf_solve_opts *o)
     int nr, block, mfma;
     pick_shape(ctx, o, nr, block, mfma);
     if (o->per_test) return false;
-    return (o->math == RF_MATH_FAST && !mfma) || o->math == RF_MATH_COMPAT;
+    return (o->math == RF_MATH_FAST && !mfma) || o->math == RF_MATH_COMPAT || o->math == RF_MATH_FP64;
 }
 
 static int ensure_queue(rf_ctx *ctx)
@@ -949,7 +949,7 @@ static void account(const rf_ctx *ctx, const rf_scene *s, const rf_solve_opts *o
         t.tests += rays * s->n;
         // band stacks add their flops from the device counter (stats[3])
         const int64_t f = !uses_stacks(ctx, o) ? s->flops_per_ray
-                          : (o->math == RF_MATH_COMPAT ? s->flops_per_ray_cnonstack : s->flops_per_ray_nonstack);
+                          : (o->math == RF_MATH_FAST ? s->flops_per_ray_nonstack : s->flops_per_ray_cnonstack);
         t.flops += rays * f;
     }
 }

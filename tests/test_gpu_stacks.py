@@ -4,7 +4,8 @@ per ray.  Every per-band decision is the per-test scan's on that band's own
 record, so the tallies, the FP64 re-solve counts and the per-ray hit indices
 must equal the per-test form's exactly, for every launch shape, with and
 without flag-list overflow, and on stacks that overlap, leave gaps or hold a
-single band."""
+single band.  The compat and FP64 forms stack the caller's order and must also
+equal the CPU oracle's per-test scan bit for bit."""
 import numpy as np
 import pytest
 
@@ -183,11 +184,13 @@ def test_stacks_bv_overlapping_children(ctx):
 
 
 # ---------------------------------------------------------------------------
-# Compat (reference arithmetic): band stacks over the caller's order, checked
-# bit for bit against the CPU oracle's per-test restatement of findObject.
+# Compat (reference arithmetic) and FP64: band stacks over the caller's order,
+# checked bit for bit against the CPU oracle's per-test restatement of
+# findObject in the same arithmetic.
 
+@pytest.mark.parametrize("math", ["compat", "fp64"])
 @pytest.mark.parametrize("scene", ["edge", "tilted", "furnace"])
-def test_compat_stacks_match_oracle(ctx, oracle, scene):
+def test_compat_stacks_match_oracle(ctx, oracle, scene, math):
     if scene == "edge":
         sc, n_wi = _edge_scene(), 2048
     elif scene == "tilted":
@@ -195,20 +198,21 @@ def test_compat_stacks_match_oracle(ctx, oracle, scene):
     else:
         sc, n_wi = scenes.parse_xml(scenes.furnace_xml(wall_bands=60, preform_bands=20)), 128
     dev = rf.DeviceScene(ctx, sc.prims, None)
-    got = rf.solve(ctx, dev, math="compat", rays_per_emitter=4 * n_wi)
-    want, _ = oracle.solve(sc.prims, None, n_wi=n_wi, mode="compat")
+    got = rf.solve(ctx, dev, math=math, rays_per_emitter=4 * n_wi)
+    want, _ = oracle.solve(sc.prims, None, n_wi=n_wi, mode=math)
     diff = np.argwhere(got.hits != want)
     assert diff.size == 0, f"{len(diff)} cells differ, first {diff[:5].tolist()}"
     assert got.timing["flops"] > 0
     dev.close()
 
 
+@pytest.mark.parametrize("math", ["compat", "fp64"])
 @pytest.mark.parametrize("form", ["persistent", "per_emitter", "reference"])
-def test_compat_stacks_equal_per_test(ctx, form):
+def test_compat_stacks_equal_per_test(ctx, form, math):
     geoms = scenes.sweep(3, seed=0x52465331)
     for g in geoms:
         dev = rf.DeviceScene(ctx, g.prims, None)
-        kw = dict(math="compat", rng="seeded", rays_per_emitter=1 << 13, form=form)
+        kw = dict(math=math, rng="seeded", rays_per_emitter=1 << 13, form=form)
         a = rf.solve(ctx, dev, per_test=True, **kw)
         b = rf.solve(ctx, dev, per_test=False, **kw)
         assert np.array_equal(a.hits, b.hits)
@@ -216,11 +220,12 @@ def test_compat_stacks_equal_per_test(ctx, form):
         dev.close()
 
 
-def test_compat_stacks_traces(ctx, oracle):
+@pytest.mark.parametrize("math", ["compat", "fp64"])
+def test_compat_stacks_traces(ctx, oracle, math):
     sc = _edge_scene()
     dev = rf.DeviceScene(ctx, sc.prims, None)
     for e in (0, 5, 8, 12):
-        a = rf.trace(ctx, dev, e, 100, 1500, math="compat")
-        b = oracle.trace(sc.prims, None, e, 100, 1500, mode="compat")
+        a = rf.trace(ctx, dev, e, 100, 1500, math=math)
+        b = oracle.trace(sc.prims, None, e, 100, 1500, mode=math)
         assert np.array_equal(a, b), (e, np.argwhere(a != b)[:5])
     dev.close()
