@@ -126,8 +126,10 @@ typedef struct rf_solve_opts {
     int32_t per_test;         /* Scans without bounding surfaces (and the fast form's binned BV scan):
                                  0 = a band stack (consecutive cylinders whose transforms differ only
                                  in the z offset, e.g. the bands of one wall) solves its quadric once
-                                 per ray and tests each band's z range; 1 = every cylinder test in
-                                 full.  Same tallies either way, in every arithmetic form. */
+                                 per ray; a stack whose z offsets are sorted is then searched for the
+                                 bands a root can lie on, any other has each band's z range tested;
+                                 1 = every cylinder test in full; 2 = band stacks, each band's z range
+                                 tested.  Same tallies in all three, in every arithmetic form. */
 } rf_solve_opts;
 
 typedef enum rf_transform {

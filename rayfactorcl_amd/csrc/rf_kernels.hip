@@ -83,6 +83,28 @@ __device__ __forceinline__ int find_c(CPTR(rf_primitive) P, int start, int end, 
     return obj;
 }
 
+// First position w of [b, e) (e - b >= 2) at which pass(w, i) holds, for each
+// ray slot i, where pass is false on a prefix of the positions and true on the
+// rest (e if it never holds): binary lifting, a wave-uniform number of steps
+// with one probe per ray each.
+template <int NR, class Pass>
+__device__ __forceinline__ void band_lower_bound(int b, int e, Pass pass, int (&pos)[NR])
+{
+    int last[NR];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) last[i] = b - 1;
+    for (int step = 1 << (31 - __builtin_clz(e - b)); step > 0; step >>= 1) {
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const int q = last[i] + step;
+            const bool inside = q < e;
+            last[i] = (inside && !pass(inside ? q : e - 1, i)) ? q : last[i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NR; ++i) pos[i] = last[i] + 1;
+}
+
 __device__ __forceinline__ int find_d(CPTR(rf_primitive) P, int start, int end, const Ray64 &ray)
 {
     int obj = -1;
@@ -104,7 +126,7 @@ __device__ __forceinline__ int find_d(CPTR(rf_primitive) P, int start, int end, 
 // nearest hit is lo at the first band holding z(lo), else hi at the first band
 // holding z(hi) (bands walked last to first, as stack_loop).  A root that is
 // absent or not past EPS becomes a NaN product, which fails every range test.
-__device__ __forceinline__ uint32_t stack_d(CPTR(rf_primitive) P, CPTR(BandRec) BR, int b, int e,
+__device__ __forceinline__ uint32_t stack_d(CPTR(rf_primitive) P, CPTR(BandRec) BR, int b, int e, int mode,
                                             const Ray64 &w, double &tbest, int &obj)
 {
     const rf_primitive hs = ldc(P + b);
@@ -140,6 +162,24 @@ __device__ __forceinline__ uint32_t stack_d(CPTR(rf_primitive) P, CPTR(BandRec) 
     const double hi = r1 < r2 ? r2 : r1;
     const double plo = lo > EPS_D ? cz * lo : qnan;
     const double phi = hi > EPS_D ? cz * hi : qnan;
+    if (mode >= 2) {
+        // sorted band offsets: z at each root is monotone over the positions
+        // (an add is non-decreasing in each operand), so the first band
+        // holding it is the first position past the range's lower (mode 2)
+        // or upper (mode 3) end, if z there is in range
+        const bool asc = mode == 2;
+        uint32_t probes = 0;
+        const auto zat = [&](int k, double p) { ++probes; return (S + (double)BR[k].r11) + p; };
+        int pl[1], ph[1];
+        band_lower_bound<1>(b, e, [&](int k, int) { const double z = zat(k, plo); return asc ? z >= 0.0 : z <= 1.0; }, pl);
+        band_lower_bound<1>(b, e, [&](int k, int) { const double z = zat(k, phi); return asc ? z >= 0.0 : z <= 1.0; }, ph);
+        bool vlo = false, vhi = false;
+        if (pl[0] < e) { const double z = zat(min(pl[0], e - 1), plo); vlo = z >= 0.0 && z <= 1.0; }
+        if (ph[0] < e) { const double z = zat(min(ph[0], e - 1), phi); vhi = z >= 0.0 && z <= 1.0; }
+        const double c = vlo ? lo : (vhi ? hi : dinf());
+        if (c < tbest) { tbest = c; obj = BR[min(vlo ? pl[0] : ph[0], e - 1)].orig; }
+        return kFlopsStackD + probes * (kFlopsBandD - 1u);
+    }
     int ilo = -1, ihi = -1;
     CPTR(BandRec) q = BR + (e - 1);
     BandRec next = ldc(q);
@@ -173,7 +213,7 @@ __device__ __forceinline__ int find_d_runs(CPTR(GeomDesc) G, const Ray64 &ray, i
     for (int j = 0; j < n_runs; ++j) {
         const CylRun run = ldc(cst(G->c_runs) + j);
         if (run.stack) {
-            sflops += stack_d(P, cst(G->c_bands), run.begin, run.end, ray, tbest, obj);
+            sflops += stack_d(P, cst(G->c_bands), run.begin, run.end, stacks >= 2 ? run.stack : 1, ray, tbest, obj);
         } else {
             for (int w = run.begin; w < run.end; ++w) {
                 const rf_primitive p = ldc(P + w);
@@ -353,6 +393,46 @@ __device__ __forceinline__ void stack_c_bands(const float (&m)[12], CPTR(BandRec
     }
 }
 
+// stack_c over a stack whose band offsets invm[11] are sorted over the
+// positions (asc: non-decreasing, else non-increasing; all finite).  z at
+// each root, o.sz(w) + o.cz t with o.sz(w) an FMA chain ending in invm[11](w),
+// is then monotone over the positions (FMA and add are non-decreasing in
+// their addends), so the bands holding root 1 (z in [0, 1]) or root 2
+// (z in (0, 1)) form one run each, and its first position is the first past
+// the range's lower (asc) or upper end -- if z there is in range.  I1 / I2
+// return positions (-1: none); `probes` counts the z evaluations.
+template <int NR, bool AXIAL>
+__device__ __forceinline__ void stack_c_search(const float (&m)[12], CPTR(BandRec) BR, int b, int e, bool asc,
+                                               const Ray32 (&ray)[NR], const float (&P1)[NR], const float (&P2)[NR],
+                                               int (&I1)[NR], int (&I2)[NR], uint32_t &probes)
+{
+    const auto zat = [&](int w, int r, float p) {
+        const Ray32 &q = ray[r];
+        const float m11 = BR[w].r11;
+        const float sz = AXIAL ? __builtin_fmaf(m[10], q.sz, m11)
+                               : __builtin_fmaf(m[8], q.sx, __builtin_fmaf(m[9], q.sy, __builtin_fmaf(m[10], q.sz, m11)));
+        ++probes;
+        return sz + p;
+    };
+    int p1[NR], p2[NR];
+    band_lower_bound<NR>(b, e, [&](int w, int r) { const float z = zat(w, r, P1[r]); return asc ? z >= 0.0f : z <= 1.0f; }, p1);
+    band_lower_bound<NR>(b, e, [&](int w, int r) { const float z = zat(w, r, P2[r]); return asc ? z > 0.0f : z < 1.0f; }, p2);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        bool v1 = false, v2 = false;
+        if (p1[r] < e) {
+            const float z1 = zat(min(p1[r], e - 1), r, P1[r]);
+            v1 = __builtin_amdgcn_fmed3f(z1, 0.0f, 1.0f) == z1;
+        }
+        if (p2[r] < e) {
+            const float z2 = zat(min(p2[r], e - 1), r, P2[r]);
+            v2 = (z2 > 0.0f) & (z2 < 1.0f);
+        }
+        I1[r] = v1 ? p1[r] : -1;
+        I2[r] = v2 ? p2[r] : -1;
+    }
+}
+
 // Compat band stack [b, e) in the caller's order: consecutive cylinders whose
 // invm rows are bitwise equal except invm[11].  cylinderHit (:189-222, here
 // hit_c_ops / roots_select_c) then computes the same A, B, C, discriminant,
@@ -366,9 +446,11 @@ __device__ __forceinline__ void stack_c_bands(const float (&m)[12], CPTR(BandRec
 // The root conditions are folded into NaN products, which fail every range
 // test.  An axial z row (invm[8] == invm[9] == 0) takes one FMA for o.sz: the
 // two dropped FMAs add a signed zero, which no range test can tell apart.
-// Returns the stack's algorithmic flops per ray.
+// mode 1 walks every band; 2 and 3 (band offsets sorted, non-decreasing or
+// non-increasing) search them (stack_c_search).
+// Returns the stack's algorithmic flops summed over the NR rays.
 template <int NR>
-__device__ __forceinline__ uint32_t stack_c(CPTR(rf_primitive) P, CPTR(BandRec) BR, int b, int e,
+__device__ __forceinline__ uint32_t stack_c(CPTR(rf_primitive) P, CPTR(BandRec) BR, int b, int e, int mode,
                                             const Ray32 (&ray)[NR], float (&tbest)[NR], int (&obj)[NR])
 {
     const rf_primitive hs = ldc(P + b);
@@ -406,6 +488,18 @@ __device__ __forceinline__ uint32_t stack_c(CPTR(rf_primitive) P, CPTR(BandRec) 
         I1[r] = -1;
         I2[r] = -1;
     }
+    if (mode >= 2) {
+        uint32_t probes = 0;
+        if (axial) stack_c_search<NR, true>(m, BR, b, e, mode == 2, ray, P1, P2, I1, I2, probes);
+        else stack_c_search<NR, false>(m, BR, b, e, mode == 2, ray, P1, P2, I1, I2, probes);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const bool h1 = I1[r] >= 0;
+            const float c = h1 ? T1[r] : (I2[r] >= 0 ? T2[r] : finf());
+            if (c < tbest[r]) { tbest[r] = c; obj[r] = BR[min(h1 ? I1[r] : I2[r], e - 1)].orig; }
+        }
+        return NR * kFlopsStackC + probes * ((axial ? kFlopsBandZ0Axial : kFlopsBandZ0) + 1u);
+    }
     if (axial) stack_c_bands<NR, true>(m, BR, b, e, ray, P1, P2, I1, I2);
     else stack_c_bands<NR, false>(m, BR, b, e, ray, P1, P2, I1, I2);
 #pragma unroll
@@ -415,12 +509,12 @@ __device__ __forceinline__ uint32_t stack_c(CPTR(rf_primitive) P, CPTR(BandRec) 
         const int ci = h1 ? I1[r] : I2[r];
         if (c < tbest[r]) { tbest[r] = c; obj[r] = ci; }
     }
-    return kFlopsStackC + (uint32_t)(e - b) * ((axial ? kFlopsBandZ0Axial : kFlopsBandZ0) + 2u);
+    return NR * (kFlopsStackC + (uint32_t)(e - b) * ((axial ? kFlopsBandZ0Axial : kFlopsBandZ0) + 2u));
 }
 
 // The compat scan of one work item's NR rays over the whole scene: band stacks
-// (G->c_runs) when `stacks`, else one per-test range.  Returns the stacks'
-// algorithmic flops per ray.
+// (G->c_runs) when `stacks` (2: sorted stacks searched), else one per-test
+// range.  Returns the stacks' algorithmic flops summed over the NR rays.
 template <int NR>
 __device__ __forceinline__ uint32_t find_c_multi(CPTR(GeomDesc) G, const Ray32 (&ray)[NR], int (&obj)[NR], int stacks)
 {
@@ -434,7 +528,7 @@ __device__ __forceinline__ uint32_t find_c_multi(CPTR(GeomDesc) G, const Ray32 (
         CylRun run{0, G->n, 0, 0};
         if (stacks) run = ldc(cst(G->c_runs) + j);
         if (run.stack)
-            sf += stack_c<NR>(P, cst(G->c_bands), run.begin, run.end, ray, tbest, obj);
+            sf += stack_c<NR>(P, cst(G->c_bands), run.begin, run.end, stacks >= 2 ? run.stack : 1, ray, tbest, obj);
         else
             scan_c_range<NR>(P, run.begin, run.end, ray, tbest, obj);
     }
@@ -775,7 +869,112 @@ __device__ __forceinline__ void stack_bands_sel(bool tan, bool root1, const Fast
     }
 }
 
-// Band stack [b, e) of the cylinder segment (CylRun.stack = 1): the records
+// z origin of band record r11 of a stack with head h (fast_test's z row).
+template <bool AXIAL>
+__device__ __forceinline__ float band_sz(const FastPrim &h, float r11, const Ray32 &R)
+{
+    return AXIAL ? __builtin_fmaf(h.r[10], R.sz, r11)
+                 : __builtin_fmaf(h.r[8], R.sx, __builtin_fmaf(h.r[9], R.sy, __builtin_fmaf(h.r[10], R.sz, r11)));
+}
+
+// Band stack whose band offsets r[11] are sorted over the positions
+// (ASC: non-decreasing, else non-increasing; all finite).  Each z the band
+// loop computes, fma(cz, t, sz(w)) with sz(w) = fma(..., r[11](w)), is then a
+// monotone function of the position (an FMA is non-decreasing in its addend),
+// so the positions where a root's |z| <= hi -- the only ones where fast_test's
+// on-band or rim terms can hold -- form one run, found by a binary search for
+// its first position and a walk to its end.  The band loop's per-band work is
+// done on the positions of the runs of root 2, root 1 and the tangent point
+// (each position once), so the first band holding each root, the flagged
+// pairs and the list overflow are the band loop's.  I1 / I2 return positions
+// (e: none).  fl gains the algorithmic flops of the probes and bands of the
+// ray slots whose bit is set in vmask.
+template <int NR, bool REFINE, bool AXIAL, bool ASC>
+__device__ __forceinline__ void stack_search(const FastPrim &h, CPTR(BandRec) BR, int b, int e,
+                                             const Ray32 (&R)[NR], const float (&T1)[NR], const float (&T2)[NR],
+                                             const float (&TM)[NR], const float (&CZ)[NR], int (&I1)[NR],
+                                             int (&I2)[NR], float lo, float hi, bool wroot1, bool wtan, Pending &pd,
+                                             uint32_t &fl, uint32_t vmask)
+{
+    const int cnt0 = pd.count;
+    constexpr uint32_t zf = (AXIAL ? kFlopsBandZ0Axial : kFlopsBandZ0) + kFlopsBandRoot;
+    // the band loop's per-band work on position w of ray slot i
+    const auto band = [&](int i, int w, int orig, float sz) {
+        const float a1 = __builtin_fabsf(__builtin_fmaf(CZ[i], T1[i], sz));
+        const float a2 = __builtin_fabsf(__builtin_fmaf(CZ[i], T2[i], sz));
+        bool in1, in2, f = false;
+        if constexpr (REFINE) {
+            in1 = a1 < lo;
+            in2 = a2 < lo;
+            f = (__builtin_amdgcn_fmed3f(a1, lo, hi) == a1) | (__builtin_amdgcn_fmed3f(a2, lo, hi) == a2) |
+                (__builtin_fabsf(__builtin_fmaf(CZ[i], TM[i], sz)) <= hi);
+            in1 = in1 & !f;
+            in2 = in2 & !f;
+        } else {
+            in1 = a1 <= 0.5f;
+            in2 = a2 <= 0.5f;
+        }
+        I1[i] = (in1 && w < I1[i]) ? w : I1[i];
+        I2[i] = (in2 && w < I2[i]) ? w : I2[i];
+        fl += (vmask >> i) & 1u ? 2u * kFlopsBandRoot : 0u;
+        if (REFINE && f) {
+            if (pd.count < PEND_K && pd.resume == NO_KEY) {
+                pd.buf[pd.count * blockDim.x + threadIdx.x] = ((uint32_t)orig << 3) | (uint32_t)i;
+                ++pd.count;
+            } else if (pd.resume == NO_KEY) {
+                pd.resume = b * 8;
+                pd.count = cnt0;
+            }
+        }
+    };
+    // run of root K (2, 1, tangent point) for every ray slot: first position
+    // by search, then the walk while |z| <= hi.  A position is in the run of
+    // an earlier root exactly when that root's |z| <= hi there (monotone z),
+    // so those are skipped.
+    const auto run = [&](const int K) {
+        const auto root = [&](int i) { return K == 2 ? T2[i] : (K == 1 ? T1[i] : TM[i]); };
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            int p[1];
+            band_lower_bound<1>(b, e, [&](int w, int) {
+                const float z = __builtin_fmaf(CZ[i], root(i), band_sz<AXIAL>(h, ldc(BR + w).r11, R[i]));
+                fl += (vmask >> i) & 1u ? zf : 0u;
+                return ASC ? z >= -hi : z <= hi;
+            }, p);
+            for (int w = p[0]; w < e; ++w) {
+                const BandRec br = ldc(BR + w);
+                const float sz = band_sz<AXIAL>(h, br.r11, R[i]);
+                fl += (vmask >> i) & 1u ? zf : 0u;
+                if (!(__builtin_fabsf(__builtin_fmaf(CZ[i], root(i), sz)) <= hi)) break;
+                bool done = false;
+                if (K != 2) done = __builtin_fabsf(__builtin_fmaf(CZ[i], T2[i], sz)) <= hi;
+                if (K == 0) done = done || __builtin_fabsf(__builtin_fmaf(CZ[i], T1[i], sz)) <= hi;
+                if (!done) band(i, w, br.orig, sz);
+            }
+        }
+    };
+    run(2);
+    if (wroot1) run(1);
+    if (REFINE && wtan) run(0);
+}
+
+template <int NR, bool REFINE, bool AXIAL>
+__device__ __forceinline__ void stack_search_sel(bool asc, const FastPrim &h, CPTR(BandRec) BR, int b, int e,
+                                                 const Ray32 (&R)[NR], const float (&T1)[NR],
+                                                 const float (&T2)[NR], const float (&TM)[NR],
+                                                 const float (&CZ)[NR], int (&I1)[NR], int (&I2)[NR], float lo,
+                                                 float hi, bool wroot1, bool wtan, Pending &pd, uint32_t &fl,
+                                                 uint32_t vmask)
+{
+    if (asc)
+        stack_search<NR, REFINE, AXIAL, true>(h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1, wtan, pd, fl,
+                                              vmask);
+    else
+        stack_search<NR, REFINE, AXIAL, false>(h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1, wtan, pd, fl,
+                                               vmask);
+}
+
+// Band stack [b, e) of the cylinder segment (CylRun.stack >= 1): the records
 // are bitwise equal except the z offset r[11], so each ray's x-y quadric, its
 // roots and its z direction are computed once, with the operations of
 // fast_test<SEG_CYL>, and each band then needs only its z origin row, z at
@@ -796,10 +995,14 @@ __device__ __forceinline__ void stack_bands_sel(bool tan, bool root1, const Fast
 // stack, the entries it queued from this stack are given back and the re-scan
 // starts at the stack's first band (resume = b x 8), so every flagged pair is
 // still re-solved exactly once.
-// Returns the stack's algorithmic flops per ray for the band loop the wave ran.
+// mode 1 walks every band; 2 and 3 (a stack sorted by r[11], non-decreasing or
+// non-increasing) search the bands instead (stack_search).
+// fl gains the algorithmic flops of the band work the wave ran for the ray
+// slots whose bit is set in vmask.
 template <int NR, bool REFINE>
-__device__ __forceinline__ uint32_t stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, int b, int e,
-                                               const Ray32 (&R)[NR], float (&tb)[NR], int (&ib)[NR], Pending &pd)
+__device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, int b, int e, int mode,
+                                           const Ray32 (&R)[NR], float (&tb)[NR], int (&ib)[NR], Pending &pd,
+                                           uint32_t &fl, uint32_t vmask)
 {
     const FastPrim hs = ldc(F + b);
     const FastPrim h = vcopy(hs);
@@ -838,6 +1041,25 @@ __device__ __forceinline__ uint32_t stack_loop(CPTR(FastPrim) F, CPTR(BandRec) B
     }
     const bool wtan = REFINE && __builtin_amdgcn_ballot_w64(tan);
     const bool wroot1 = __builtin_amdgcn_ballot_w64(root1);
+    if (mode >= 2) {
+        // positions, e = none
+#pragma unroll
+        for (int i = 0; i < NR; ++i) { I1[i] = e; I2[i] = e; }
+        fl += (uint32_t)__builtin_popcount(vmask) * kFlopsStack;
+        if (axial)
+            stack_search_sel<NR, REFINE, true>(mode == 2, h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1,
+                                               wtan, pd, fl, vmask);
+        else
+            stack_search_sel<NR, REFINE, false>(mode == 2, h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1,
+                                                wtan, pd, fl, vmask);
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const bool h1 = I1[i] < e;
+            const float c = h1 ? T1[i] : (I2[i] < e ? T2[i] : finf());
+            if (c < tb[i]) { tb[i] = c; ib[i] = BR[min(h1 ? I1[i] : I2[i], e - 1)].orig; }
+        }
+        return;
+    }
     if (axial) stack_bands_sel<NR, REFINE, true>(wtan, wroot1, h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, pd);
     else stack_bands_sel<NR, REFINE, false>(wtan, wroot1, h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, pd);
 #pragma unroll
@@ -847,8 +1069,9 @@ __device__ __forceinline__ uint32_t stack_loop(CPTR(FastPrim) F, CPTR(BandRec) B
         const int ci = h1 ? I1[i] : I2[i];
         if (c < tb[i]) { tb[i] = c; ib[i] = ci; }
     }
-    return kFlopsStack + (uint32_t)(e - b) * ((axial ? kFlopsBandZ0Axial : kFlopsBandZ0) +
-                                              (wroot1 ? 2u : 1u) * kFlopsBandRoot);
+    const uint32_t sf = kFlopsStack + (uint32_t)(e - b) * ((axial ? kFlopsBandZ0Axial : kFlopsBandZ0) +
+                                                           (wroot1 ? 2u : 1u) * kFlopsBandRoot);
+    fl += (uint32_t)__builtin_popcount(vmask) * sf;
 }
 
 // FP64 stage of the deferred re-solves (see Pending).  The flagged ray is
@@ -928,20 +1151,21 @@ __device__ __forceinline__ void finish_dropped(const KParams &kp, CPTR(GeomDesc)
 }
 
 // stacks: the cylinder segment runs as its CylRun list (band stacks share the
-// quadric); otherwise as one per-test run.  Returns the band stacks'
-// algorithmic flops per ray (the rest is counted on the host).
+// quadric; 2: sorted stacks are searched); otherwise as one per-test run.
+// fl gains the band-stack flops of the ray slots set in vmask (the rest is
+// counted on the host).
 template <int NR, bool REFINE>
-__device__ __forceinline__ uint32_t scan_fast(CPTR(GeomDesc) G, const Ray32 (&R)[NR], float (&tb)[NR],
-                                              int (&ib)[NR], Pending &pd, int stacks)
+__device__ __forceinline__ void scan_fast(CPTR(GeomDesc) G, const Ray32 (&R)[NR], float (&tb)[NR], int (&ib)[NR],
+                                          Pending &pd, int stacks, uint32_t &fl, uint32_t vmask)
 {
-    uint32_t sf = 0;
     CPTR(FastPrim) F = cst(G->fast);
     const int n_runs = stacks ? G->n_cyl_runs : 1;
     for (int j = 0; j < n_runs; ++j) {
         CylRun run{G->seg_begin[SEG_CYL], G->seg_end[SEG_CYL], 0, 0};
         if (stacks) run = ldc(cst(G->cyl_runs) + j);
         if (run.stack)
-            sf += stack_loop<NR, REFINE>(F, cst(G->bands), run.begin, run.end, R, tb, ib, pd);
+            stack_loop<NR, REFINE>(F, cst(G->bands), run.begin, run.end, stacks >= 2 ? run.stack : 1, R, tb, ib,
+                                   pd, fl, vmask);
         else
             seg_loop<SEG_CYL, NR, REFINE, false>(F, run.begin, run.end, R, tb, ib, pd);
     }
@@ -949,7 +1173,6 @@ __device__ __forceinline__ uint32_t scan_fast(CPTR(GeomDesc) G, const Ray32 (&R)
     seg_loop<SEG_SPHERE, NR, REFINE, false>(F, G->seg_begin[SEG_SPHERE], G->seg_end[SEG_SPHERE], R, tb, ib, pd);
     seg_loop<SEG_RING, NR, REFINE, false>(F, G->seg_begin[SEG_RING], G->seg_end[SEG_RING], R, tb, ib, pd);
     seg_loop<SEG_RECT, NR, REFINE, false>(F, G->seg_begin[SEG_RECT], G->seg_end[SEG_RECT], R, tb, ib, pd);
-    return sf;
 }
 
 // The NR rays of ray group `base` (unit-local ray index) of one emitter.
@@ -1081,9 +1304,10 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
 #pragma unroll
             for (int i = 0; i < NR; ++i) { tb[i] = finf(); ib[i] = -1; }
             Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
-            const uint32_t sf = scan_fast<NR, REFINE>(G, R, tb, ib, pd, kp.stacks);
+            uint32_t fl = 0;
             const int64_t left = n_rays - base;
-            sflops += (uint64_t)sf * (uint64_t)(left < NR ? left : NR);
+            scan_fast<NR, REFINE>(G, R, tb, ib, pd, kp.stacks, fl, left < NR ? (1u << left) - 1u : (1u << NR) - 1u);
+            sflops += fl;
             if constexpr (REFINE) {
                 const auto ray_of = [base](int k) { return base + k; };
                 if (__builtin_amdgcn_ballot_w64(pd.count > 0))
@@ -1111,7 +1335,7 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                     rotate_rands(f, v, r4);
                     R4[v] = emit_c(&E, r4);
                 }
-                sflops += 4u * find_c_multi<4>(G, R4, idx4, kp.stacks);
+                sflops += find_c_multi<4>(G, R4, idx4, kp.stacks);
 #pragma unroll
                 for (int v = 0; v < 4; ++v) tally<TRACE, DIRECT>(kp, G, emitter, ray0 + wl * 4 + v, idx4[v], bins);
                 continue;
@@ -1138,7 +1362,9 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                         float tb1[1] = {finf()};
                         int ib1[1] = {-1};
                         Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
-                        sflops += scan_fast<1, REFINE>(G, R1, tb1, ib1, pd, kp.stacks);
+                        uint32_t fl1 = 0;
+                        scan_fast<1, REFINE>(G, R1, tb1, ib1, pd, kp.stacks, fl1, 1u);
+                        sflops += fl1;
                         if constexpr (REFINE) {
                             const int64_t r0 = wl * 4 + v;
                             const auto ray_of = [r0](int k) { return r0 + k; };
@@ -1650,7 +1876,10 @@ __device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc)
             }
             CPTR(int32_t) sg = cst(G->bvc_seg) + b * (N_SEG + 1);
             Pending pd{pend, 0, NO_KEY, 0, tau, eps};
-            uint32_t csf = 0;     // the children's band stacks: flops per ray
+            uint32_t csf = 0;     // the children's band stacks' flops (valid slots)
+            uint32_t vmask = 0;
+#pragma unroll
+            for (int i = 0; i < BNR; ++i) vmask |= valid[i] ? 1u << i : 0u;
             if (kp.stacks) {
                 // the children's cylinder runs: band stacks share the quadric
                 CPTR(int32_t) ro = cst(G->bvc_run_off) + b;
@@ -1658,7 +1887,8 @@ __device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc)
                 for (int j = ro[0]; j < j1; ++j) {
                     const CylRun run = ldc(cst(G->bvc_runs) + j);
                     if (run.stack)
-                        csf += stack_loop<BNR, REFINE>(FC, cst(G->bvc_bands), run.begin, run.end, R, tb, ib, pd);
+                        stack_loop<BNR, REFINE>(FC, cst(G->bvc_bands), run.begin, run.end,
+                                                kp.stacks >= 2 ? run.stack : 1, R, tb, ib, pd, csf, vmask);
                     else
                         seg_loop<SEG_CYL, BNR, REFINE, false>(FC, run.begin, run.end, R, tb, ib, pd);
                 }
@@ -1676,7 +1906,8 @@ __device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc)
                     finish_dropped<BNR>(kp, G, E, gid, emitter, wi0, ray_of, FC, sg, sg + 1, tb, ib, pd, nref, nover);
             }
             const BvMeta meta = ldc(M + b);
-            const uint32_t cfl = cst(G->bv_cflops)[2 * b + (kp.stacks ? 1 : 0)] + csf;
+            const uint32_t cfl = cst(G->bv_cflops)[2 * b + (kp.stacks ? 1 : 0)];
+            cflops += csf;
 #pragma unroll
             for (int i = 0; i < BNR; ++i) {
                 if (valid[i]) {

@@ -329,6 +329,23 @@ static void quad_rows(const rf_primitive *p, float rows[4][4])
 // the stacked cylinders (the kernel counts a stack's flops on the device).
 static int64_t build_cyl_runs(const std::vector<FastPrim> &recs, int32_t cb, int32_t ce, std::vector<CylRun> &runs);
 
+// CylRun.stack of a stack whose band offsets are off[0..n): 2 when they are
+// finite and non-decreasing, 3 when finite and non-increasing (the kernels
+// may then search the bands), 1 otherwise (every band is walked).
+template <class Off>
+static int32_t stack_order(int32_t n, Off off)
+{
+    bool up = true, down = true;
+    for (int32_t k = 0; k < n; ++k) {
+        if (!std::isfinite(off(k))) return 1;
+        if (k > 0) {
+            up = up && off(k - 1) <= off(k);
+            down = down && off(k - 1) >= off(k);
+        }
+    }
+    return up ? 2 : (down ? 3 : 1);
+}
+
 // The same over the caller's order for the compat scan: a stack is two or
 // more consecutive cylinders with bitwise equal invm[0..10]; everything else
 // forms per-test runs.  Returns minus the per-test flops of the stacked cylinders.
@@ -347,7 +364,7 @@ static int64_t build_compat_runs(const rf_primitive *prims, int32_t n, std::vect
     while (p < n) {
         int32_t q = stack_end(p);
         if (q - p >= 2) {
-            runs.push_back(CylRun{p, q, 1, 0});
+            runs.push_back(CylRun{p, q, stack_order(q - p, [&](int32_t k) { return prims[p + k].invm[11]; }), 0});
             delta -= (q - p) * flops_per_test(RF_CYLINDER);
             p = q;
             continue;
@@ -371,7 +388,7 @@ static int64_t build_cyl_runs(const std::vector<FastPrim> &recs, int32_t cb, int
     while (p < ce) {
         int32_t q = stack_end(p);
         if (q - p >= 2) {
-            runs.push_back(CylRun{p, q, 1, 0});
+            runs.push_back(CylRun{p, q, stack_order(q - p, [&](int32_t k) { return recs[p + k].r[11]; }), 0});
             delta -= (q - p) * flops_per_test(RF_CYLINDER);
             p = q;
             continue;
@@ -656,7 +673,7 @@ static int check_opts(rf_ctx *ctx, const rf_solve_opts *o)
         return set_err(ctx, RF_ERR_INVALID_ARG, "bad transform");
     if (!(o->refine_margin >= 0.0f && o->refine_margin <= 0.25f))
         return set_err(ctx, RF_ERR_INVALID_ARG, "refine_margin must lie in [0, 0.25]");
-    if (o->per_test != 0 && o->per_test != 1) return set_err(ctx, RF_ERR_INVALID_ARG, "per_test must be 0 or 1");
+    if (o->per_test < 0 || o->per_test > 2) return set_err(ctx, RF_ERR_INVALID_ARG, "per_test must be 0, 1 or 2");
     if (o->block_size && (o->block_size < 64 || o->block_size > 256 || o->block_size % 64))
         return set_err(ctx, RF_ERR_INVALID_ARG, "block_size must be 0 or a multiple of 64 in [64, 256]");
     return RF_OK;
@@ -723,7 +740,7 @@ static bool uses_stacks(const rf_ctx *ctx, const rf_solve_opts *o)
 {
     int nr, block, mfma;
     pick_shape(ctx, o, nr, block, mfma);
-    if (o->per_test) return false;
+    if (o->per_test == 1) return false;
     return (o->math == RF_MATH_FAST && !mfma) || o->math == RF_MATH_COMPAT || o->math == RF_MATH_FP64;
 }
 
@@ -812,7 +829,7 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
     kp.refine = o->refine;
     kp.tau = o->refine_margin > 0.0f ? o->refine_margin : rfd::TAU_DEFAULT;
     kp.bins = s->n <= kMaxLdsBins ? s->n : 0;
-    kp.stacks = uses_stacks(ctx, o) ? 1 : 0;
+    kp.stacks = !uses_stacks(ctx, o) ? 0 : (o->per_test == 2 ? 1 : 2);
     kp.imp_emitter = emitter;
     kp.imp_geom_id = geom_id;
     kp.imp_wi0 = wi0;
@@ -1027,7 +1044,7 @@ static int prepare_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms,
     kp.refine = o->refine;
     kp.tau = o->refine_margin > 0.0f ? o->refine_margin : rfd::TAU_DEFAULT;
     kp.bins = max_n <= kMaxLdsBins ? max_n : 0;
-    kp.stacks = uses_stacks(ctx, o) ? 1 : 0;
+    kp.stacks = !uses_stacks(ctx, o) ? 0 : (o->per_test == 2 ? 1 : 2);
     int rc = take_counter(ctx, &kp.queue_head);
     if (rc) return rc;
     c = LaunchCfg{};

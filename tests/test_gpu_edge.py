@@ -61,7 +61,7 @@ def test_bad_options_are_rejected(ctx, golden_scene):
         rf.solve(ctx, dev, o)
     assert e.value.code == _lib.RF_ERR_INVALID_ARG
     o = rf.make_opts(math="fast", rays_per_emitter=4096)
-    o.per_test = 2
+    o.per_test = 3
     with pytest.raises(rf.RayFactorError) as e:
         rf.solve(ctx, dev, o)
     assert e.value.code == _lib.RF_ERR_INVALID_ARG

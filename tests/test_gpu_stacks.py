@@ -3,9 +3,12 @@ only in the z offset (the bands of one coaxial wall) share one quadric solve
 per ray.  Every per-band decision is the per-test scan's on that band's own
 record, so the tallies, the FP64 re-solve counts and the per-ray hit indices
 must equal the per-test form's exactly, for every launch shape, with and
-without flag-list overflow, and on stacks that overlap, leave gaps or hold a
-single band.  The compat and FP64 forms stack the caller's order and must also
-equal the CPU oracle's per-test scan bit for bit."""
+without flag-list overflow, and on stacks that overlap, leave gaps, hold a
+single band or list their bands in any order.  Both stacked forms are
+checked: per_test=0 searches stacks whose z offsets are sorted (walking the
+others), per_test=2 tests every band's z range.  The compat and FP64 forms
+stack the caller's order and must also equal the CPU oracle's per-test scan
+bit for bit."""
 import numpy as np
 import pytest
 
@@ -15,10 +18,13 @@ from rayfactorcl_amd import scenes
 pytestmark = pytest.mark.gpu
 
 
-def _pair(ctx, sc, **kw):
+STACKED = [0, 2]     # per_test values of the two stacked forms: search, every band
+
+
+def _pair(ctx, sc, stacked=0, **kw):
     dev = rf.DeviceScene(ctx, sc.prims, None)
     a = rf.solve(ctx, dev, math="fast", rng="seeded", per_test=True, **kw)
-    b = rf.solve(ctx, dev, math="fast", rng="seeded", per_test=False, **kw)
+    b = rf.solve(ctx, dev, math="fast", rng="seeded", per_test=stacked, **kw)
     dev.close()
     return a, b
 
@@ -28,30 +34,34 @@ def _same(a, b):
     assert a.timing["refined"] == b.timing["refined"]
 
 
+@pytest.mark.parametrize("stacked", STACKED)
 @pytest.mark.parametrize("form", ["persistent", "per_emitter", "reference"])
 @pytest.mark.parametrize("rpt", [1, 2, 4, 8])
-def test_stacks_equal_per_test(ctx, form, rpt):
+def test_stacks_equal_per_test(ctx, form, rpt, stacked):
     sc = scenes.parse_xml(scenes.furnace_xml(wall_bands=40, preform_bands=12))
-    a, b = _pair(ctx, sc, rays_per_emitter=20000, form=form, rays_per_thread=rpt)
+    a, b = _pair(ctx, sc, stacked, rays_per_emitter=20000, form=form, rays_per_thread=rpt)
     _same(a, b)
     # the stacked form counts fewer algorithmic flops for the same tests
     assert b.timing["tests"] == a.timing["tests"] and b.timing["flops"] < a.timing["flops"]
 
 
+@pytest.mark.parametrize("stacked", STACKED)
 @pytest.mark.parametrize("rpt", [1, 4, 8])
-def test_stacks_wide_margin_overflow(ctx, rpt):
+def test_stacks_wide_margin_overflow(ctx, rpt, stacked):
     """tau = 1/4 flags most rim crossings: re-solve lists overflow inside
     stacks, the stack's queued entries are given back and the re-scan starts
     at its first band; every flagged pair is still re-solved exactly once."""
     sc = scenes.parse_xml(scenes.furnace_xml(wall_bands=40, preform_bands=12))
-    a, b = _pair(ctx, sc, rays_per_emitter=20000, form="persistent", rays_per_thread=rpt, refine_margin=0.25)
+    a, b = _pair(ctx, sc, stacked, rays_per_emitter=20000, form="persistent", rays_per_thread=rpt,
+                 refine_margin=0.25)
     _same(a, b)
     assert b.timing["rescans"] > 0
 
 
-def test_stacks_without_refine(ctx):
+@pytest.mark.parametrize("stacked", STACKED)
+def test_stacks_without_refine(ctx, stacked):
     sc = scenes.parse_xml(scenes.furnace_xml(wall_bands=60, preform_bands=20))
-    a, b = _pair(ctx, sc, rays_per_emitter=1 << 16, form="persistent", refine=False)
+    a, b = _pair(ctx, sc, stacked, rays_per_emitter=1 << 16, form="persistent", refine=False)
     assert np.array_equal(a.hits, b.hits)
 
 
@@ -60,11 +70,13 @@ def test_stacks_sweep_batched(ctx):
     dev = [rf.DeviceScene(ctx, g.prims, None) for g in geoms]
     kw = dict(math="fast", rng="seeded", rays_per_emitter=1 << 14, form="persistent")
     a, ta = rf.batch_solve(ctx, dev, rf.make_opts(per_test=True, **kw))
-    b, tb = rf.batch_solve(ctx, dev, rf.make_opts(per_test=False, **kw))
-    for x, y in zip(a, b):
-        assert np.array_equal(x.hits, y.hits)
-    assert ta["refined"] == tb["refined"] and ta["tests"] == tb["tests"]
-    print(f"[stacks] sweep sample: flops per launch {ta['flops']:.3e} per test, {tb['flops']:.3e} stacked", flush=True)
+    for stacked in STACKED:
+        b, tb = rf.batch_solve(ctx, dev, rf.make_opts(per_test=stacked, **kw))
+        for x, y in zip(a, b):
+            assert np.array_equal(x.hits, y.hits)
+        assert ta["refined"] == tb["refined"] and ta["tests"] == tb["tests"]
+        print(f"[stacks] sweep sample: flops per launch {ta['flops']:.3e} per test, {tb['flops']:.3e} "
+              f"stacked ({stacked})", flush=True)
     for d in dev:
         d.close()
 
@@ -74,8 +86,9 @@ def test_stacks_traces(ctx):
     dev = rf.DeviceScene(ctx, sc.prims, None)
     for e in (0, 15, sc.n - 1):
         a = rf.trace(ctx, dev, e, 0, 4096, math="fast", rng="seeded", per_test=True)
-        b = rf.trace(ctx, dev, e, 0, 4096, math="fast", rng="seeded", per_test=False)
-        assert np.array_equal(a, b)
+        for stacked in STACKED:
+            b = rf.trace(ctx, dev, e, 0, 4096, math="fast", rng="seeded", per_test=stacked)
+            assert np.array_equal(a, b)
     dev.close()
 
 
@@ -91,7 +104,11 @@ def _edge_scene():
     - a run of shorter bands (a stack of its own), split by a disc in the
       caller's order (the cylinder segment keeps them consecutive);
     - single bands of other radii and heights (per-test runs);
-    closed by a disc at each end."""
+    - bands listed top down (z offsets increasing over the stack);
+    - bands listed out of z order (a stack whose offsets are not sorted:
+      walked band by band in every stacked form);
+    closed by a disc at each end.  All stacks but the last two list their
+    bands bottom up (z offsets decreasing)."""
     parts = [_band(0.0, 0.5), _band(0.5, 1.0), _band(1.1, 1.6), _band(1.6, 2.1),
              _band(2.1, 2.6), _band(2.1, 2.6),
              _band(2.6, 3.1), _band(2.85, 3.35),
@@ -99,24 +116,30 @@ def _edge_scene():
              scenes._prim("disc", scale=(0.3, 0.3, 1), translate=(0, 0, 1.0)),
              _band(3.6, 3.85), _band(3.85, 4.1),
              _band(0.0, 0.7, r=0.2), _band(0.7, 1.2, r=0.25), _band(4.1, 4.3, r=0.5),
+             _band(2.5, 2.8, r=0.35), _band(2.2, 2.5, r=0.35), _band(1.9, 2.2, r=0.35),
+             _band(3.0, 3.3, r=0.15), _band(3.6, 3.9, r=0.15), _band(3.3, 3.6, r=0.15),
              scenes._prim("disc", scale=(0.5, 0.5, 1)),
              scenes._prim("disc", scale=(0.5, 0.5, 1), rotate=scenes.FLIP_X, translate=(0, 0, 4.3))]
     return scenes.parse_xml(scenes._doc("band stack edge cases", "".join(parts)))
 
 
+@pytest.mark.parametrize("stacked", STACKED)
 @pytest.mark.parametrize("margin", [0.0, 0.25])
 @pytest.mark.parametrize("rpt", [1, 4])
-def test_stacks_edge_cases(ctx, margin, rpt):
+def test_stacks_edge_cases(ctx, margin, rpt, stacked):
     sc = _edge_scene()
-    a, b = _pair(ctx, sc, rays_per_emitter=1 << 15, form="persistent", rays_per_thread=rpt, refine_margin=margin)
+    a, b = _pair(ctx, sc, stacked, rays_per_emitter=1 << 15, form="persistent", rays_per_thread=rpt,
+                 refine_margin=margin)
     _same(a, b)
     # the duplicated band never wins over its first copy
     assert a.hits[:, 5].sum() == 0 and a.hits[:, 4].sum() > 0
+    # the top-down and unsorted stacks are hit
+    assert a.hits[:, 15:21].sum(axis=0).min() > 0
     dev = rf.DeviceScene(ctx, sc.prims, None)
-    for e in (0, 4, 7, 12):
+    for e in (0, 4, 7, 12, 16, 19):
         kw = dict(math="fast", rng="seeded", refine_margin=margin)
         ta = rf.trace(ctx, dev, e, 0, 2048, per_test=True, **kw)
-        tb = rf.trace(ctx, dev, e, 0, 2048, per_test=False, **kw)
+        tb = rf.trace(ctx, dev, e, 0, 2048, per_test=stacked, **kw)
         assert np.array_equal(ta, tb)
     dev.close()
 
@@ -145,12 +168,13 @@ def _tilted(sc, deg=23.0, axis=(0.3, 0.8, 0.52)):
     return scenes.SceneData(p, sc.bvs, sc.globalRayDensity, sc.useBoundingVolumes)
 
 
+@pytest.mark.parametrize("stacked", STACKED)
 @pytest.mark.parametrize("margin", [0.0, 0.25])
-def test_stacks_tilted(ctx, margin):
+def test_stacks_tilted(ctx, margin, stacked):
     sc = _tilted(scenes.parse_xml(scenes.furnace_xml(wall_bands=40, preform_bands=12)))
     inv = sc.prims["invm"]
     assert (inv[:40, 8] != 0).all() and (inv[:40, :11] == inv[0, :11]).all()
-    a, b = _pair(ctx, sc, rays_per_emitter=20000, form="persistent", refine_margin=margin)
+    a, b = _pair(ctx, sc, stacked, rays_per_emitter=20000, form="persistent", refine_margin=margin)
     _same(a, b)
 
 
@@ -158,14 +182,15 @@ def test_stacks_tilted(ctx, margin):
 # The binned bounding-surface kernel scans each surface's children with the
 # same runs (the wall surface's children are one band stack).
 
+@pytest.mark.parametrize("stacked", STACKED)
 @pytest.mark.parametrize("margin", [0.0, 0.25])
 @pytest.mark.parametrize("form", ["persistent", "per_emitter"])
-def test_stacks_bv_binned(ctx, margin, form):
+def test_stacks_bv_binned(ctx, margin, form, stacked):
     sc = scenes.parse_xml(scenes.furnace_xml(with_bv=True, wall_bands=60, preform_bands=20))
     dev = rf.DeviceScene(ctx, sc.prims, sc.bvs)
     kw = dict(math="fast", rng="seeded", use_bvs=True, rays_per_emitter=1 << 14, form=form, refine_margin=margin)
     a = rf.solve(ctx, dev, per_test=True, **kw)
-    b = rf.solve(ctx, dev, per_test=False, **kw)
+    b = rf.solve(ctx, dev, per_test=stacked, **kw)
     _same(a, b)
     assert a.timing["tests"] == b.timing["tests"]
     assert 0 < b.timing["flops"] < a.timing["flops"]
@@ -179,7 +204,9 @@ def test_stacks_bv_overlapping_children(ctx):
     bvs["objStartIndex"][1] = max(0, int(bvs["objStartIndex"][1]) - 5)   # overlaps the wall's range
     dev = rf.DeviceScene(ctx, sc.prims, bvs)
     kw = dict(math="fast", rng="seeded", use_bvs=True, rays_per_emitter=1 << 14, form="persistent")
-    _same(rf.solve(ctx, dev, per_test=True, **kw), rf.solve(ctx, dev, per_test=False, **kw))
+    a = rf.solve(ctx, dev, per_test=True, **kw)
+    for stacked in STACKED:
+        _same(a, rf.solve(ctx, dev, per_test=stacked, **kw))
     dev.close()
 
 
@@ -188,9 +215,10 @@ def test_stacks_bv_overlapping_children(ctx):
 # checked bit for bit against the CPU oracle's per-test restatement of
 # findObject in the same arithmetic.
 
+@pytest.mark.parametrize("stacked", STACKED)
 @pytest.mark.parametrize("math", ["compat", "fp64"])
 @pytest.mark.parametrize("scene", ["edge", "tilted", "furnace"])
-def test_compat_stacks_match_oracle(ctx, oracle, scene, math):
+def test_compat_stacks_match_oracle(ctx, oracle, scene, math, stacked):
     if scene == "edge":
         sc, n_wi = _edge_scene(), 2048
     elif scene == "tilted":
@@ -198,7 +226,7 @@ def test_compat_stacks_match_oracle(ctx, oracle, scene, math):
     else:
         sc, n_wi = scenes.parse_xml(scenes.furnace_xml(wall_bands=60, preform_bands=20)), 128
     dev = rf.DeviceScene(ctx, sc.prims, None)
-    got = rf.solve(ctx, dev, math=math, rays_per_emitter=4 * n_wi)
+    got = rf.solve(ctx, dev, math=math, rays_per_emitter=4 * n_wi, per_test=stacked)
     want, _ = oracle.solve(sc.prims, None, n_wi=n_wi, mode=math)
     diff = np.argwhere(got.hits != want)
     assert diff.size == 0, f"{len(diff)} cells differ, first {diff[:5].tolist()}"
@@ -214,18 +242,20 @@ def test_compat_stacks_equal_per_test(ctx, form, math):
         dev = rf.DeviceScene(ctx, g.prims, None)
         kw = dict(math=math, rng="seeded", rays_per_emitter=1 << 13, form=form)
         a = rf.solve(ctx, dev, per_test=True, **kw)
-        b = rf.solve(ctx, dev, per_test=False, **kw)
-        assert np.array_equal(a.hits, b.hits)
-        assert b.timing["flops"] < a.timing["flops"]
+        for stacked in STACKED:
+            b = rf.solve(ctx, dev, per_test=stacked, **kw)
+            assert np.array_equal(a.hits, b.hits)
+            assert b.timing["flops"] < a.timing["flops"]
         dev.close()
 
 
+@pytest.mark.parametrize("stacked", STACKED)
 @pytest.mark.parametrize("math", ["compat", "fp64"])
-def test_compat_stacks_traces(ctx, oracle, math):
+def test_compat_stacks_traces(ctx, oracle, math, stacked):
     sc = _edge_scene()
     dev = rf.DeviceScene(ctx, sc.prims, None)
-    for e in (0, 5, 8, 12):
-        a = rf.trace(ctx, dev, e, 100, 1500, math=math)
+    for e in (0, 5, 8, 12, 16, 19):
+        a = rf.trace(ctx, dev, e, 100, 1500, math=math, per_test=stacked)
         b = oracle.trace(sc.prims, None, e, 100, 1500, mode=math)
         assert np.array_equal(a, b), (e, np.argwhere(a != b)[:5])
     dev.close()
