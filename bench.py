@@ -3,12 +3,13 @@
 
 Workload (default): the batched design sweep -- 256 coaxial furnace
 geometries (34-557 primitives each) with 1e6 rays per emitter, solved in the
-FP32 + FP64-refined form with band stacks by one persistent launch per step.
-A step is one full solve of all 256 geometries (every emitter, every ray).
-Beside it, on a sample of the geometries: the same math in the plain
-one-launch-per-emitter forms, the per-test form (band stacks off), and the
-fast form's view-factor deviation from all-FP64; the CPU oracle runs a
-bounded sample on the host cores.
+FP32 + FP64-refined form with band stacks (sorted stacks searched) by one
+persistent launch per step.  A step is one full solve of all 256 geometries
+(every emitter, every ray).  Beside it, on a sample of the geometries: the
+same math in the plain one-launch-per-emitter forms, the band stacks walked
+band by band, the per-test form (band stacks off), and the fast form's
+view-factor deviation from all-FP64; the CPU oracle runs a bounded sample on
+the host cores.
 
 Prints ONE JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
@@ -26,6 +27,13 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+# cylinder test forms: per_test option of the library, description
+CYL_FORMS = {
+    "search": (0, "band stacks: one quadric per ray and stack; sorted stacks searched for the bands a root "
+                  "can lie on (1-2 rays/thread), others tested band by band"),
+    "walk": (2, "band stacks: one quadric per ray and stack, a z-range test per band"),
+    "full": (1, "every test in full"),
+}
 PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector peak: 256 CUs x 4 SIMDs x 32 FMA lanes/clk x 2.4 GHz x 2
 PEAK_HBM_GBS = 8000.0
 
@@ -123,14 +131,19 @@ def main() -> int:
                     help="fixed launch shape (with --block-size) instead of the autotuner's")
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--no-refine", action="store_true")
-    ap.add_argument("--per-test", action="store_true",
-                    help="evaluate every cylinder test in full (no band stacks; same tallies)")
+    ap.add_argument("--cylinder-tests", default="search", choices=list(CYL_FORMS),
+                    help="search: band stacks, sorted ones searched; walk: band stacks, every band's z range "
+                         "tested; full: every cylinder test in full (same tallies in all three)")
+    ap.add_argument("--per-test", action="store_true", help="the same as --cylinder-tests full")
     ap.add_argument("--traffic-file", default=None,
                     help="measured HBM bytes per launch of the solve kernel (JSON written from rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE passes); default profiles/hbm_traffic_<workload>.json")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (scales the HBM byte counters)")
     args = ap.parse_args()
+    if args.per_test:
+        args.cylinder_tests = "full"
+    per_test = CYL_FORMS[args.cylinder_tests][0]
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -151,13 +164,13 @@ def main() -> int:
     elif not args.no_tune:
         t0 = time.perf_counter()
         rpt, blk = rf.autotune(ctx, dev, rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays,
-                                                      refine=refine, per_test=args.per_test))
+                                                      refine=refine, per_test=per_test))
         log(f"autotune: {'MFMA transform' if rpt == 0 else f'{rpt} rays/thread'}, {blk} threads/group "
             f"({time.perf_counter() - t0:.1f} s)")
     kernel = (f"rf::k_queue_mfma<{str(refine).lower()}, false>" if rpt == 0 else
               f"rf::k_queue<2, false, false, {rpt}, {str(refine).lower()}>")
     opts = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", refine=refine,
-                        seed=args.seed, per_test=args.per_test,
+                        seed=args.seed, per_test=per_test,
                         rays_per_thread=rpt if args.rays_per_thread else 0,
                         block_size=blk if args.rays_per_thread else 0)
     for i in range(args.warmup):
@@ -188,11 +201,11 @@ def main() -> int:
         idx = np.linspace(0, len(dev) - 1, min(args.plain_geoms, len(dev))).astype(int)
         sub = [dev[i] for i in idx]
         ob = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", refine=refine,
-                          seed=args.seed)
+                          seed=args.seed, per_test=per_test)
         op = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="per_emitter", refine=refine,
-                          seed=args.seed)
+                          seed=args.seed, per_test=per_test)
         orf = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="reference", refine=refine,
-                           seed=args.seed)
+                           seed=args.seed, per_test=per_test)
         res = {}
         for name, o in [("batched", ob), ("per_emitter", op), ("reference_form", orf)]:
             rf.batch_solve(ctx, sub, o, readback=False)
@@ -207,24 +220,25 @@ def main() -> int:
         s0 = time.perf_counter()
         for i, d in enumerate(sub):
             og = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", refine=refine,
-                              seed=args.seed, geometry=i)
+                              seed=args.seed, geometry=i, per_test=per_test)
             _, t = rf.batch_solve(ctx, [d], og, readback=False)
             tests += t["tests"]; launches_g += t["launches"]
         dt = time.perf_counter() - s0
         res["per_geometry"] = {"tests_per_s": tests / dt, "wall_ms": dt * 1e3, "launches": launches_g}
         log(f"sample per_geometry: {res['per_geometry']['tests_per_s']:.3e} tests/s, {launches_g} launches")
-        # the same batched solve with every cylinder test in full (no band stacks),
-        # in that form's best shape (4 rays per thread, 256-thread groups)
-        opt = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", refine=refine,
-                           seed=args.seed, per_test=True, rays_per_thread=4, block_size=256, transform="valu")
-        rf.batch_solve(ctx, sub, opt, readback=False)
-        s0 = time.perf_counter()
-        _, t = rf.batch_solve(ctx, sub, opt, readback=False)
-        dt = time.perf_counter() - s0
-        res["per_test"] = {"tests_per_s": t["tests"] / dt, "kernel_ms": t["kernel_ms"],
-                           "tflops": t["flops"] / (t["kernel_ms"] * 1e-3) / 1e12}
-        log(f"sample per_test: {res['per_test']['tests_per_s']:.3e} tests/s, "
-            f"{res['per_test']['tflops']:.1f} TFLOP/s")
+        # the same batched solve with the band stacks walked band by band, and
+        # with every cylinder test in full (no band stacks), each in its best
+        # shape (4 rays per thread, 256-thread groups)
+        for name, pt in [("walk", 2), ("per_test", 1)]:
+            opt = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", refine=refine,
+                               seed=args.seed, per_test=pt, rays_per_thread=4, block_size=256, transform="valu")
+            rf.batch_solve(ctx, sub, opt, readback=False)
+            s0 = time.perf_counter()
+            _, t = rf.batch_solve(ctx, sub, opt, readback=False)
+            dt = time.perf_counter() - s0
+            res[name] = {"tests_per_s": t["tests"] / dt, "kernel_ms": t["kernel_ms"],
+                         "tflops": t["flops"] / (t["kernel_ms"] * 1e-3) / 1e12}
+            log(f"sample {name}: {res[name]['tests_per_s']:.3e} tests/s, {res[name]['tflops']:.1f} TFLOP/s")
         plain = {
             "geometries": len(sub),
             "batched_tests_per_s": res["batched"]["tests_per_s"],
@@ -234,6 +248,9 @@ def main() -> int:
             "batched_over_per_emitter": res["batched"]["tests_per_s"] / res["per_emitter"]["tests_per_s"],
             "batched_over_reference_form": res["batched"]["tests_per_s"] / res["reference_form"]["tests_per_s"],
             "batched_over_one_geometry_per_launch": res["batched"]["tests_per_s"] / res["per_geometry"]["tests_per_s"],
+            "walk_form_tests_per_s": res["walk"]["tests_per_s"],
+            "walk_form_frac": res["walk"]["tflops"] / PEAK_FP32_TFLOPS,
+            "batched_over_walk_form": res["batched"]["tests_per_s"] / res["walk"]["tests_per_s"],
             "per_test_form_tests_per_s": res["per_test"]["tests_per_s"],
             "per_test_form_frac": res["per_test"]["tflops"] / PEAK_FP32_TFLOPS,
             "batched_over_per_test_form": res["batched"]["tests_per_s"] / res["per_test"]["tests_per_s"],
@@ -248,21 +265,21 @@ def main() -> int:
         dv = 0.0
         for i in idx:
             kw = dict(rng="seeded", rays_per_emitter=rp, form="persistent", seed=args.seed, geometry=int(i))
-            a = rf.solve(ctx, dev[i], math="fast", refine=refine, per_test=args.per_test, **kw)
+            a = rf.solve(ctx, dev[i], math="fast", refine=refine, per_test=per_test, **kw)
             b = rf.solve(ctx, dev[i], math="fp64", **kw)
             dv = max(dv, float(np.abs(a.hits - b.hits).max()) / rp)
         precision = {"max_dvf_vs_fp64": dv, "geometries": len(idx), "rays_per_emitter": rp}
         log(f"precision: max |dVF| fast vs fp64 = {dv:.2e} over {len(idx)} geometries")
 
     traffic = None
-    form = "per_test" if args.per_test else "stacked"
+    form = {"search": "search", "walk": "walk", "full": "per_test"}[args.cylinder_tests]
     tname = f"hbm_traffic_{args.workload}{args.geoms if args.workload == 'sweep' else ''}"
     tfile = Path(args.traffic_file) if args.traffic_file else ROOT / "profiles" / (
-        tname + ("_per_test" if args.per_test else "") + ".json")
+        tname + ("" if form == "search" else "_" + form) + ".json")
     if tfile.exists():
         rec = json.loads(tfile.read_text())
         if (rec.get("kernel") == kernel and int(rec.get("rays_per_emitter", -1)) == rays
-                and rec.get("form", "per_test") == form):
+                and rec.get("form") == form):
             traffic = rec["hbm_bytes_per_launch"]
             log(f"traffic: {traffic:.3e} B per launch from {tfile.name}")
         else:
@@ -290,8 +307,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": desc, "geometries": len(geoms), "rays_per_emitter": rays,
                    "primitives_total": int(sum(g.n for g in geoms)), "form": "persistent",
-                   "cylinder_tests": ("every test in full" if args.per_test else
-                                      "band stacks: one quadric per ray and stack, a z-range test per band"),
+                   "cylinder_tests": CYL_FORMS[args.cylinder_tests][1],
                    "math": "fp32 fast + fp64 refine" if refine else "fp32 fast", "rng": "philox4x32-10 seeded",
                    "parallelism": "single GPU"},
         "rays_per_s": nrays / wall,

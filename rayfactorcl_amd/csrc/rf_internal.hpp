@@ -21,16 +21,20 @@ struct BvMeta {
 };
 
 // Runs of the fast form's cylinder segment, in traversal order.  A stack
-// (stack = 1) is a run of two or more consecutive cylinders whose folded
+// (stack >= 1) is a run of two or more consecutive cylinders whose folded
 // transform rows are bitwise equal except the z-row offset r[11] -- the bands
 // of one coaxial wall: every band shares the ray's x-y quadric and its z
 // direction, so the kernel solves the quadric once per ray and tests each
-// band's z range.  Other runs (stack = 0) are scanned one test at a time.
+// band's z range.  stack = 2 (3): the offsets are finite and non-decreasing
+// (non-increasing) over the run, so a root's bands can be searched for; inv
+// is then (end - begin - 1) / (last offset - first offset), 0 if they are
+// equal (the search's first guess assumes even spacing).  Other runs
+// (stack = 0) are scanned one test at a time.
 struct CylRun {
     int32_t begin;
     int32_t end;
     int32_t stack;
-    int32_t pad;
+    float inv;
 };
 
 // Per-position band record of a stack: the z-row offset and the primitive's

@@ -105,6 +105,44 @@ __device__ __forceinline__ void band_lower_bound(int b, int e, Pass pass, int (&
     for (int i = 0; i < NR; ++i) pos[i] = last[i] + 1;
 }
 
+// band_lower_bound from a first guess g[i] in [b, e] per ray slot: the guess
+// is the answer when pass holds at g and fails just before it (positions b-1
+// and e count as failing and holding); both probes are issued together.
+// Slots whose guess is off fall back to the binary search.
+template <int NR, class Pass>
+__device__ __forceinline__ void band_lower_bound_guess(int b, int e, const int (&g)[NR], Pass pass,
+                                                       int (&pos)[NR])
+{
+    bool off = false;
+    bool ok[NR];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        const int gi = g[i];
+        const bool at = pass(min(gi, e - 1), i);
+        const bool before = pass(max(gi - 1, b), i);
+        ok[i] = (gi >= e || at) && (gi <= b || !before);
+        pos[i] = gi;
+        off |= !ok[i];
+    }
+    if (__builtin_amdgcn_ballot_w64(off)) {
+        int bs[NR];
+        band_lower_bound<NR>(b, e, pass, bs);
+#pragma unroll
+        for (int i = 0; i < NR; ++i) pos[i] = ok[i] ? pos[i] : bs[i];
+    }
+}
+
+// First guess of a lower bound over an evenly spaced stack: the position
+// whose offset first reaches `target` (ASC: offsets increasing, pass where
+// offset >= target; else decreasing, pass where offset <= target), from the
+// first offset r0 and inv = bands per unit offset; a NaN target guesses e.
+__device__ __forceinline__ int band_guess(int b, int e, float r0, float inv, float target)
+{
+    float f = __builtin_ceilf((target - r0) * inv);
+    f = __builtin_fminf(__builtin_fmaxf(f, 0.0f), (float)(e - b));
+    return target != target ? e : b + (int)f;
+}
+
 __device__ __forceinline__ int find_d(CPTR(rf_primitive) P, int start, int end, const Ray64 &ray)
 {
     int obj = -1;
@@ -127,7 +165,7 @@ __device__ __forceinline__ int find_d(CPTR(rf_primitive) P, int start, int end, 
 // holding z(hi) (bands walked last to first, as stack_loop).  A root that is
 // absent or not past EPS becomes a NaN product, which fails every range test.
 __device__ __forceinline__ uint32_t stack_d(CPTR(rf_primitive) P, CPTR(BandRec) BR, int b, int e, int mode,
-                                            const Ray64 &w, double &tbest, int &obj)
+                                            float inv, const Ray64 &w, double &tbest, int &obj)
 {
     const rf_primitive hs = ldc(P + b);
     const float *iv = hs.invm;
@@ -170,9 +208,13 @@ __device__ __forceinline__ uint32_t stack_d(CPTR(rf_primitive) P, CPTR(BandRec) 
         const bool asc = mode == 2;
         uint32_t probes = 0;
         const auto zat = [&](int k, double p) { ++probes; return (S + (double)BR[k].r11) + p; };
+        const float r0 = ldc(BR + b).r11;
+        const double lim = asc ? 0.0 : 1.0;
+        const int gl[1] = {band_guess(b, e, r0, inv, (float)(lim - (S + plo)))};
+        const int gh[1] = {band_guess(b, e, r0, inv, (float)(lim - (S + phi)))};
         int pl[1], ph[1];
-        band_lower_bound<1>(b, e, [&](int k, int) { const double z = zat(k, plo); return asc ? z >= 0.0 : z <= 1.0; }, pl);
-        band_lower_bound<1>(b, e, [&](int k, int) { const double z = zat(k, phi); return asc ? z >= 0.0 : z <= 1.0; }, ph);
+        band_lower_bound_guess<1>(b, e, gl, [&](int k, int) { const double z = zat(k, plo); return asc ? z >= 0.0 : z <= 1.0; }, pl);
+        band_lower_bound_guess<1>(b, e, gh, [&](int k, int) { const double z = zat(k, phi); return asc ? z >= 0.0 : z <= 1.0; }, ph);
         bool vlo = false, vhi = false;
         if (pl[0] < e) { const double z = zat(min(pl[0], e - 1), plo); vlo = z >= 0.0 && z <= 1.0; }
         if (ph[0] < e) { const double z = zat(min(ph[0], e - 1), phi); vhi = z >= 0.0 && z <= 1.0; }
@@ -213,7 +255,8 @@ __device__ __forceinline__ int find_d_runs(CPTR(GeomDesc) G, const Ray64 &ray, i
     for (int j = 0; j < n_runs; ++j) {
         const CylRun run = ldc(cst(G->c_runs) + j);
         if (run.stack) {
-            sflops += stack_d(P, cst(G->c_bands), run.begin, run.end, stacks >= 2 ? run.stack : 1, ray, tbest, obj);
+            sflops += stack_d(P, cst(G->c_bands), run.begin, run.end, stacks >= 2 ? run.stack : 1, run.inv, ray, tbest,
+                              obj);
         } else {
             for (int w = run.begin; w < run.end; ++w) {
                 const rf_primitive p = ldc(P + w);
@@ -403,7 +446,7 @@ __device__ __forceinline__ void stack_c_bands(const float (&m)[12], CPTR(BandRec
 // return positions (-1: none); `probes` counts the z evaluations.
 template <int NR, bool AXIAL>
 __device__ __forceinline__ void stack_c_search(const float (&m)[12], CPTR(BandRec) BR, int b, int e, bool asc,
-                                               const Ray32 (&ray)[NR], const float (&P1)[NR], const float (&P2)[NR],
+                                               float inv, const Ray32 (&ray)[NR], const float (&P1)[NR], const float (&P2)[NR],
                                                int (&I1)[NR], int (&I2)[NR], uint32_t &probes)
 {
     const auto zat = [&](int w, int r, float p) {
@@ -414,9 +457,20 @@ __device__ __forceinline__ void stack_c_search(const float (&m)[12], CPTR(BandRe
         ++probes;
         return sz + p;
     };
-    int p1[NR], p2[NR];
-    band_lower_bound<NR>(b, e, [&](int w, int r) { const float z = zat(w, r, P1[r]); return asc ? z >= 0.0f : z <= 1.0f; }, p1);
-    band_lower_bound<NR>(b, e, [&](int w, int r) { const float z = zat(w, r, P2[r]); return asc ? z > 0.0f : z < 1.0f; }, p2);
+    // first guesses: z is about o.sz with invm[11] = 0, plus the root's
+    // product, plus invm[11]
+    const float r0 = ldc(BR + b).r11;
+    int g1[NR], g2[NR], p1[NR], p2[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const Ray32 &q = ray[r];
+        const float s0 = AXIAL ? m[10] * q.sz
+                               : __builtin_fmaf(m[8], q.sx, __builtin_fmaf(m[9], q.sy, m[10] * q.sz));
+        g1[r] = band_guess(b, e, r0, inv, (asc ? 0.0f : 1.0f) - (s0 + P1[r]));
+        g2[r] = band_guess(b, e, r0, inv, (asc ? 0.0f : 1.0f) - (s0 + P2[r]));
+    }
+    band_lower_bound_guess<NR>(b, e, g1, [&](int w, int r) { const float z = zat(w, r, P1[r]); return asc ? z >= 0.0f : z <= 1.0f; }, p1);
+    band_lower_bound_guess<NR>(b, e, g2, [&](int w, int r) { const float z = zat(w, r, P2[r]); return asc ? z > 0.0f : z < 1.0f; }, p2);
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         bool v1 = false, v2 = false;
@@ -451,7 +505,7 @@ __device__ __forceinline__ void stack_c_search(const float (&m)[12], CPTR(BandRe
 // Returns the stack's algorithmic flops summed over the NR rays.
 template <int NR>
 __device__ __forceinline__ uint32_t stack_c(CPTR(rf_primitive) P, CPTR(BandRec) BR, int b, int e, int mode,
-                                            const Ray32 (&ray)[NR], float (&tbest)[NR], int (&obj)[NR])
+                                            float inv, const Ray32 (&ray)[NR], float (&tbest)[NR], int (&obj)[NR])
 {
     const rf_primitive hs = ldc(P + b);
     float m[12];
@@ -490,8 +544,8 @@ __device__ __forceinline__ uint32_t stack_c(CPTR(rf_primitive) P, CPTR(BandRec) 
     }
     if (mode >= 2) {
         uint32_t probes = 0;
-        if (axial) stack_c_search<NR, true>(m, BR, b, e, mode == 2, ray, P1, P2, I1, I2, probes);
-        else stack_c_search<NR, false>(m, BR, b, e, mode == 2, ray, P1, P2, I1, I2, probes);
+        if (axial) stack_c_search<NR, true>(m, BR, b, e, mode == 2, inv, ray, P1, P2, I1, I2, probes);
+        else stack_c_search<NR, false>(m, BR, b, e, mode == 2, inv, ray, P1, P2, I1, I2, probes);
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             const bool h1 = I1[r] >= 0;
@@ -528,7 +582,8 @@ __device__ __forceinline__ uint32_t find_c_multi(CPTR(GeomDesc) G, const Ray32 (
         CylRun run{0, G->n, 0, 0};
         if (stacks) run = ldc(cst(G->c_runs) + j);
         if (run.stack)
-            sf += stack_c<NR>(P, cst(G->c_bands), run.begin, run.end, stacks >= 2 ? run.stack : 1, ray, tbest, obj);
+            sf += stack_c<NR>(P, cst(G->c_bands), run.begin, run.end, stacks >= 2 ? run.stack : 1, run.inv, ray, tbest,
+                              obj);
         else
             scan_c_range<NR>(P, run.begin, run.end, ray, tbest, obj);
     }
@@ -890,8 +945,8 @@ __device__ __forceinline__ float band_sz(const FastPrim &h, float r11, const Ray
 // (e: none).  fl gains the algorithmic flops of the probes and bands of the
 // ray slots whose bit is set in vmask.
 template <int NR, bool REFINE, bool AXIAL, bool ASC>
-__device__ __forceinline__ void stack_search(const FastPrim &h, CPTR(BandRec) BR, int b, int e,
-                                             const Ray32 (&R)[NR], const float (&T1)[NR], const float (&T2)[NR],
+__device__ __forceinline__ void stack_search(const FastPrim &h, CPTR(BandRec) BR, int b, int e, float r0,
+                                             float inv, const Ray32 (&R)[NR], const float (&T1)[NR], const float (&T2)[NR],
                                              const float (&TM)[NR], const float (&CZ)[NR], int (&I1)[NR],
                                              int (&I2)[NR], float lo, float hi, bool wroot1, bool wtan, Pending &pd,
                                              uint32_t &fl, uint32_t vmask)
@@ -933,15 +988,21 @@ __device__ __forceinline__ void stack_search(const FastPrim &h, CPTR(BandRec) BR
     // so those are skipped.
     const auto run = [&](const int K) {
         const auto root = [&](int i) { return K == 2 ? T2[i] : (K == 1 ? T1[i] : TM[i]); };
+        // first guess: z(w) is about fma(cz, t, sz with r[11] = 0) + r[11](w)
+        int g[NR], p[NR];
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
-            int p[1];
-            band_lower_bound<1>(b, e, [&](int w, int) {
-                const float z = __builtin_fmaf(CZ[i], root(i), band_sz<AXIAL>(h, ldc(BR + w).r11, R[i]));
-                fl += (vmask >> i) & 1u ? zf : 0u;
-                return ASC ? z >= -hi : z <= hi;
-            }, p);
-            for (int w = p[0]; w < e; ++w) {
+            const float k0 = __builtin_fmaf(CZ[i], root(i), band_sz<AXIAL>(h, 0.0f, R[i]));
+            g[i] = band_guess(b, e, r0, inv, ASC ? -hi - k0 : hi - k0);
+        }
+        band_lower_bound_guess<NR>(b, e, g, [&](int w, int i) {
+            const float z = __builtin_fmaf(CZ[i], root(i), band_sz<AXIAL>(h, ldc(BR + w).r11, R[i]));
+            fl += (vmask >> i) & 1u ? zf : 0u;
+            return ASC ? z >= -hi : z <= hi;
+        }, p);
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            for (int w = p[i]; w < e; ++w) {
                 const BandRec br = ldc(BR + w);
                 const float sz = band_sz<AXIAL>(h, br.r11, R[i]);
                 fl += (vmask >> i) & 1u ? zf : 0u;
@@ -960,17 +1021,17 @@ __device__ __forceinline__ void stack_search(const FastPrim &h, CPTR(BandRec) BR
 
 template <int NR, bool REFINE, bool AXIAL>
 __device__ __forceinline__ void stack_search_sel(bool asc, const FastPrim &h, CPTR(BandRec) BR, int b, int e,
-                                                 const Ray32 (&R)[NR], const float (&T1)[NR],
+                                                 float r0, float inv, const Ray32 (&R)[NR], const float (&T1)[NR],
                                                  const float (&T2)[NR], const float (&TM)[NR],
                                                  const float (&CZ)[NR], int (&I1)[NR], int (&I2)[NR], float lo,
                                                  float hi, bool wroot1, bool wtan, Pending &pd, uint32_t &fl,
                                                  uint32_t vmask)
 {
     if (asc)
-        stack_search<NR, REFINE, AXIAL, true>(h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1, wtan, pd, fl,
+        stack_search<NR, REFINE, AXIAL, true>(h, BR, b, e, r0, inv, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1, wtan, pd, fl,
                                               vmask);
     else
-        stack_search<NR, REFINE, AXIAL, false>(h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1, wtan, pd, fl,
+        stack_search<NR, REFINE, AXIAL, false>(h, BR, b, e, r0, inv, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1, wtan, pd, fl,
                                                vmask);
 }
 
@@ -996,11 +1057,14 @@ __device__ __forceinline__ void stack_search_sel(bool asc, const FastPrim &h, CP
 // starts at the stack's first band (resume = b x 8), so every flagged pair is
 // still re-solved exactly once.
 // mode 1 walks every band; 2 and 3 (a stack sorted by r[11], non-decreasing or
-// non-increasing) search the bands instead (stack_search).
+// non-increasing) search the bands instead (stack_search) at 1 or 2 rays per
+// thread.  At 4 and 8 the search's registers would spill in the 96 (4) or
+// fill the 192 (8) a wave has at that shape's occupancy, so those shapes walk
+// every band; the autotuner weighs the two.
 // fl gains the algorithmic flops of the band work the wave ran for the ray
 // slots whose bit is set in vmask.
 template <int NR, bool REFINE>
-__device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, int b, int e, int mode,
+__device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, int b, int e, int mode, float inv,
                                            const Ray32 (&R)[NR], float (&tb)[NR], int (&ib)[NR], Pending &pd,
                                            uint32_t &fl, uint32_t vmask)
 {
@@ -1041,16 +1105,17 @@ __device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, i
     }
     const bool wtan = REFINE && __builtin_amdgcn_ballot_w64(tan);
     const bool wroot1 = __builtin_amdgcn_ballot_w64(root1);
-    if (mode >= 2) {
+    if (NR <= 2 && mode >= 2) {
         // positions, e = none
 #pragma unroll
         for (int i = 0; i < NR; ++i) { I1[i] = e; I2[i] = e; }
         fl += (uint32_t)__builtin_popcount(vmask) * kFlopsStack;
+        const float r0 = ldc(BR + b).r11;
         if (axial)
-            stack_search_sel<NR, REFINE, true>(mode == 2, h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1,
+            stack_search_sel<NR, REFINE, true>(mode == 2, h, BR, b, e, r0, inv, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1,
                                                wtan, pd, fl, vmask);
         else
-            stack_search_sel<NR, REFINE, false>(mode == 2, h, BR, b, e, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1,
+            stack_search_sel<NR, REFINE, false>(mode == 2, h, BR, b, e, r0, inv, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1,
                                                 wtan, pd, fl, vmask);
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
@@ -1164,8 +1229,8 @@ __device__ __forceinline__ void scan_fast(CPTR(GeomDesc) G, const Ray32 (&R)[NR]
         CylRun run{G->seg_begin[SEG_CYL], G->seg_end[SEG_CYL], 0, 0};
         if (stacks) run = ldc(cst(G->cyl_runs) + j);
         if (run.stack)
-            stack_loop<NR, REFINE>(F, cst(G->bands), run.begin, run.end, stacks >= 2 ? run.stack : 1, R, tb, ib,
-                                   pd, fl, vmask);
+            stack_loop<NR, REFINE>(F, cst(G->bands), run.begin, run.end, stacks >= 2 ? run.stack : 1, run.inv, R,
+                                   tb, ib, pd, fl, vmask);
         else
             seg_loop<SEG_CYL, NR, REFINE, false>(F, run.begin, run.end, R, tb, ib, pd);
     }
@@ -1888,7 +1953,7 @@ __device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc)
                     const CylRun run = ldc(cst(G->bvc_runs) + j);
                     if (run.stack)
                         stack_loop<BNR, REFINE>(FC, cst(G->bvc_bands), run.begin, run.end,
-                                                kp.stacks >= 2 ? run.stack : 1, R, tb, ib, pd, csf, vmask);
+                                                kp.stacks >= 2 ? run.stack : 1, run.inv, R, tb, ib, pd, csf, vmask);
                     else
                         seg_loop<SEG_CYL, BNR, REFINE, false>(FC, run.begin, run.end, R, tb, ib, pd);
                 }

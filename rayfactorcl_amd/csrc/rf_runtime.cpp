@@ -329,21 +329,26 @@ static void quad_rows(const rf_primitive *p, float rows[4][4])
 // the stacked cylinders (the kernel counts a stack's flops on the device).
 static int64_t build_cyl_runs(const std::vector<FastPrim> &recs, int32_t cb, int32_t ce, std::vector<CylRun> &runs);
 
-// CylRun.stack of a stack whose band offsets are off[0..n): 2 when they are
-// finite and non-decreasing, 3 when finite and non-increasing (the kernels
-// may then search the bands), 1 otherwise (every band is walked).
+// The stack run [p, p + n) whose band offsets are off[0..n): stack = 2 when
+// they are finite and non-decreasing, 3 when finite and non-increasing (the
+// kernels may then search the bands), 1 otherwise (every band is walked);
+// inv = (n - 1) / (off[n-1] - off[0]), the bands per unit offset of an evenly
+// spaced stack (0 when the offsets are equal).
 template <class Off>
-static int32_t stack_order(int32_t n, Off off)
+static CylRun stack_run(int32_t p, int32_t n, Off off)
 {
     bool up = true, down = true;
     for (int32_t k = 0; k < n; ++k) {
-        if (!std::isfinite(off(k))) return 1;
+        if (!std::isfinite(off(k))) return CylRun{p, p + n, 1, 0.f};
         if (k > 0) {
             up = up && off(k - 1) <= off(k);
             down = down && off(k - 1) >= off(k);
         }
     }
-    return up ? 2 : (down ? 3 : 1);
+    if (!up && !down) return CylRun{p, p + n, 1, 0.f};
+    const double span = (double)off(n - 1) - (double)off(0);
+    const float inv = span != 0.0 ? (float)((double)(n - 1) / span) : 0.f;
+    return CylRun{p, p + n, up ? 2 : 3, std::isfinite(inv) ? inv : 0.f};
 }
 
 // The same over the caller's order for the compat scan: a stack is two or
@@ -364,7 +369,7 @@ static int64_t build_compat_runs(const rf_primitive *prims, int32_t n, std::vect
     while (p < n) {
         int32_t q = stack_end(p);
         if (q - p >= 2) {
-            runs.push_back(CylRun{p, q, stack_order(q - p, [&](int32_t k) { return prims[p + k].invm[11]; }), 0});
+            runs.push_back(stack_run(p, q - p, [&](int32_t k) { return prims[p + k].invm[11]; }));
             delta -= (q - p) * flops_per_test(RF_CYLINDER);
             p = q;
             continue;
@@ -388,7 +393,7 @@ static int64_t build_cyl_runs(const std::vector<FastPrim> &recs, int32_t cb, int
     while (p < ce) {
         int32_t q = stack_end(p);
         if (q - p >= 2) {
-            runs.push_back(CylRun{p, q, stack_order(q - p, [&](int32_t k) { return recs[p + k].r[11]; }), 0});
+            runs.push_back(stack_run(p, q - p, [&](int32_t k) { return recs[p + k].r[11]; }));
             delta -= (q - p) * flops_per_test(RF_CYLINDER);
             p = q;
             continue;
