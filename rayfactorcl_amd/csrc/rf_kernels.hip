@@ -210,16 +210,29 @@ __device__ __forceinline__ uint32_t stack_d(CPTR(rf_primitive) P, CPTR(BandRec) 
         const auto zat = [&](int k, double p) { ++probes; return (S + (double)BR[k].r11) + p; };
         const float r0 = ldc(BR + b).r11;
         const double lim = asc ? 0.0 : 1.0;
-        const int gl[1] = {band_guess(b, e, r0, inv, (float)(lim - (S + plo)))};
-        const int gh[1] = {band_guess(b, e, r0, inv, (float)(lim - (S + phi)))};
-        int pl[1], ph[1];
-        band_lower_bound_guess<1>(b, e, gl, [&](int k, int) { const double z = zat(k, plo); return asc ? z >= 0.0 : z <= 1.0; }, pl);
-        band_lower_bound_guess<1>(b, e, gh, [&](int k, int) { const double z = zat(k, phi); return asc ? z >= 0.0 : z <= 1.0; }, ph);
-        bool vlo = false, vhi = false;
-        if (pl[0] < e) { const double z = zat(min(pl[0], e - 1), plo); vlo = z >= 0.0 && z <= 1.0; }
-        if (ph[0] < e) { const double z = zat(min(ph[0], e - 1), phi); vhi = z >= 0.0 && z <= 1.0; }
+        const auto pass = [&](double z) { return asc ? z >= 0.0 : z <= 1.0; };
+        // the first position past the range's end for product p, and whether
+        // z there is in range; the guess and the position before it are
+        // probed together
+        const auto root_search = [&](double p, int &pos) {
+            const int g = band_guess(b, e, r0, inv, (float)(lim - (S + p)));
+            const double zg = zat(min(g, e - 1), p);
+            const double zm = zat(max(g - 1, b), p);
+            double z = zg;
+            pos = g;
+            if (!((g >= e || pass(zg)) && (g <= b || !pass(zm)))) {
+                int bs[1];
+                band_lower_bound<1>(b, e, [&](int k, int) { return pass(zat(k, p)); }, bs);
+                pos = bs[0];
+                z = zat(min(pos, e - 1), p);
+            }
+            return pos < e && z >= 0.0 && z <= 1.0;
+        };
+        int pl, ph;
+        const bool vlo = root_search(plo, pl);
+        const bool vhi = root_search(phi, ph);
         const double c = vlo ? lo : (vhi ? hi : dinf());
-        if (c < tbest) { tbest = c; obj = BR[min(vlo ? pl[0] : ph[0], e - 1)].orig; }
+        if (c < tbest) { tbest = c; obj = BR[min(vlo ? pl : ph, e - 1)].orig; }
         return kFlopsStackD + probes * (kFlopsBandD - 1u);
     }
     int ilo = -1, ihi = -1;
@@ -460,28 +473,51 @@ __device__ __forceinline__ void stack_c_search(const float (&m)[12], CPTR(BandRe
     // first guesses: z is about o.sz with invm[11] = 0, plus the root's
     // product, plus invm[11]
     const float r0 = ldc(BR + b).r11;
-    int g1[NR], g2[NR], p1[NR], p2[NR];
+    float s0[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         const Ray32 &q = ray[r];
-        const float s0 = AXIAL ? m[10] * q.sz
-                               : __builtin_fmaf(m[8], q.sx, __builtin_fmaf(m[9], q.sy, m[10] * q.sz));
-        g1[r] = band_guess(b, e, r0, inv, (asc ? 0.0f : 1.0f) - (s0 + P1[r]));
-        g2[r] = band_guess(b, e, r0, inv, (asc ? 0.0f : 1.0f) - (s0 + P2[r]));
+        s0[r] = AXIAL ? m[10] * q.sz : __builtin_fmaf(m[8], q.sx, __builtin_fmaf(m[9], q.sy, m[10] * q.sz));
     }
-    band_lower_bound_guess<NR>(b, e, g1, [&](int w, int r) { const float z = zat(w, r, P1[r]); return asc ? z >= 0.0f : z <= 1.0f; }, p1);
-    band_lower_bound_guess<NR>(b, e, g2, [&](int w, int r) { const float z = zat(w, r, P2[r]); return asc ? z > 0.0f : z < 1.0f; }, p2);
+    // one root: the first position where z passes the range's lower (asc) or
+    // upper end, and z there; the guess and the position before it are
+    // probed together, and z at the guess is the answer's when it holds
+    const auto root_search = [&](const float (&P)[NR], bool open, int (&pos)[NR], float (&zp)[NR]) {
+        const auto pass = [&](float z) {
+            return asc ? (open ? z > 0.0f : z >= 0.0f) : (open ? z < 1.0f : z <= 1.0f);
+        };
+        bool ok[NR];
+        bool off = false;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int g = band_guess(b, e, r0, inv, (asc ? 0.0f : 1.0f) - (s0[r] + P[r]));
+            const float zg = zat(min(g, e - 1), r, P[r]);
+            const float zm = zat(max(g - 1, b), r, P[r]);
+            ok[r] = (g >= e || pass(zg)) && (g <= b || !pass(zm));
+            pos[r] = g;
+            zp[r] = zg;
+            off |= !ok[r];
+        }
+        if (__builtin_amdgcn_ballot_w64(off)) {
+            int bs[NR];
+            band_lower_bound<NR>(b, e, [&](int w, int r) { return pass(zat(w, r, P[r])); }, bs);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                if (!ok[r]) {
+                    pos[r] = bs[r];
+                    zp[r] = zat(min(bs[r], e - 1), r, P[r]);
+                }
+            }
+        }
+    };
+    int p1[NR], p2[NR];
+    float z1[NR], z2[NR];
+    root_search(P1, false, p1, z1);
+    root_search(P2, true, p2, z2);
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-        bool v1 = false, v2 = false;
-        if (p1[r] < e) {
-            const float z1 = zat(min(p1[r], e - 1), r, P1[r]);
-            v1 = __builtin_amdgcn_fmed3f(z1, 0.0f, 1.0f) == z1;
-        }
-        if (p2[r] < e) {
-            const float z2 = zat(min(p2[r], e - 1), r, P2[r]);
-            v2 = (z2 > 0.0f) & (z2 < 1.0f);
-        }
+        const bool v1 = p1[r] < e && __builtin_amdgcn_fmed3f(z1[r], 0.0f, 1.0f) == z1[r];
+        const bool v2 = p2[r] < e && (z2[r] > 0.0f) & (z2[r] < 1.0f);
         I1[r] = v1 ? p1[r] : -1;
         I2[r] = v2 ? p2[r] : -1;
     }
@@ -988,30 +1024,67 @@ __device__ __forceinline__ void stack_search(const FastPrim &h, CPTR(BandRec) BR
     // so those are skipped.
     const auto run = [&](const int K) {
         const auto root = [&](int i) { return K == 2 ? T2[i] : (K == 1 ? T1[i] : TM[i]); };
-        // first guess: z(w) is about fma(cz, t, sz with r[11] = 0) + r[11](w)
-        int g[NR], p[NR];
+        const auto zof = [&](int i, float r11) {
+            return __builtin_fmaf(CZ[i], root(i), band_sz<AXIAL>(h, r11, R[i]));
+        };
+        const auto pass = [&](float z) { return ASC ? z >= -hi : z <= hi; };
+        // first guess, z(w) being about fma(cz, t, sz with r[11] = 0) + r[11](w);
+        // the records before it, at it and after it are loaded together: the
+        // check of the guess needs the first two, and the walk usually ends
+        // at the third
+        int g[NR];
+        BandRec rg[NR], rn[NR];
+        bool ok[NR];
+        bool off = false;
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
             const float k0 = __builtin_fmaf(CZ[i], root(i), band_sz<AXIAL>(h, 0.0f, R[i]));
-            g[i] = band_guess(b, e, r0, inv, ASC ? -hi - k0 : hi - k0);
+            const int gi = band_guess(b, e, r0, inv, ASC ? -hi - k0 : hi - k0);
+            const BandRec rm = ldc(BR + max(gi - 1, b));
+            rg[i] = ldc(BR + min(gi, e - 1));
+            rn[i] = ldc(BR + min(gi + 1, e - 1));
+            fl += (vmask >> i) & 1u ? 2u * zf : 0u;
+            ok[i] = (gi >= e || pass(zof(i, rg[i].r11))) && (gi <= b || !pass(zof(i, rm.r11)));
+            g[i] = gi;
+            off |= !ok[i];
         }
-        band_lower_bound_guess<NR>(b, e, g, [&](int w, int i) {
-            const float z = __builtin_fmaf(CZ[i], root(i), band_sz<AXIAL>(h, ldc(BR + w).r11, R[i]));
-            fl += (vmask >> i) & 1u ? zf : 0u;
-            return ASC ? z >= -hi : z <= hi;
-        }, p);
+        if (__builtin_amdgcn_ballot_w64(off)) {
+            int bs[NR];
+            band_lower_bound<NR>(b, e, [&](int w, int i) {
+                fl += (vmask >> i) & 1u ? zf : 0u;
+                return pass(zof(i, ldc(BR + w).r11));
+            }, bs);
+#pragma unroll
+            for (int i = 0; i < NR; ++i) {
+                if (!ok[i]) {
+                    g[i] = bs[i];
+                    rg[i] = ldc(BR + min(bs[i], e - 1));
+                    rn[i] = ldc(BR + min(bs[i] + 1, e - 1));
+                }
+            }
+        }
+        // the walk; its first two positions use the records already loaded
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
-            for (int w = p[i]; w < e; ++w) {
-                const BandRec br = ldc(BR + w);
+            int w = g[i];
+            bool go = w < e;
+            const auto step = [&](const BandRec &br) {
                 const float sz = band_sz<AXIAL>(h, br.r11, R[i]);
                 fl += (vmask >> i) & 1u ? zf : 0u;
-                if (!(__builtin_fabsf(__builtin_fmaf(CZ[i], root(i), sz)) <= hi)) break;
+                if (!(__builtin_fabsf(__builtin_fmaf(CZ[i], root(i), sz)) <= hi)) {
+                    go = false;
+                    return;
+                }
                 bool done = false;
                 if (K != 2) done = __builtin_fabsf(__builtin_fmaf(CZ[i], T2[i], sz)) <= hi;
                 if (K == 0) done = done || __builtin_fabsf(__builtin_fmaf(CZ[i], T1[i], sz)) <= hi;
                 if (!done) band(i, w, br.orig, sz);
-            }
+                ++w;
+                go = w < e;
+            };
+            if (go) step(rg[i]);
+            if (go) step(rn[i]);
+            while (go) step(ldc(BR + w));
         }
     };
     run(2);
