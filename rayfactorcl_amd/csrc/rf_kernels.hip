@@ -789,6 +789,8 @@ struct Pending {
     int from;          // re-scan: pairs with key < from are skipped
     float tau;         // relative flag margin (VGPR copy)
     float eps;         // EPS_F (VGPR copy)
+    int slot0 = 0;     // band stacks: ray slot of the caller's first ray (a stack scanned in slices)
+    int stack0 = -1;   // band stacks: list length to give back to on overflow (-1: at the call)
 };
 
 // Wave-wide minimum of a lane value (re-scan start).
@@ -881,7 +883,7 @@ __device__ __forceinline__ void stack_bands(const FastPrim &h, CPTR(BandRec) BR,
                                             const float (&CZ)[NR], int (&I1)[NR], int (&I2)[NR], float lo, float hi,
                                             Pending &pd)
 {
-    const int cnt0 = pd.count;
+    const int cnt0 = pd.stack0 >= 0 ? pd.stack0 : pd.count;
     // BR[-1] exists (the band array has a leading pad record), so the
     // prefetch of the next record needs no clamp
     CPTR(BandRec) p = BR + (e - 1);
@@ -933,7 +935,7 @@ __device__ __forceinline__ void stack_bands(const FastPrim &h, CPTR(BandRec) BR,
             for (int i = 0; i < NR; ++i) {
                 if (f[i]) {
                     if (pd.count < PEND_K && pd.resume == NO_KEY) {
-                        pd.buf[pd.count * blockDim.x + threadIdx.x] = ((uint32_t)orig << 3) | (uint32_t)i;
+                        pd.buf[pd.count * blockDim.x + threadIdx.x] = ((uint32_t)orig << 3) | (uint32_t)(pd.slot0 + i);
                         ++pd.count;
                     } else if (pd.resume == NO_KEY) {
                         pd.resume = b * 8;
@@ -987,7 +989,7 @@ __device__ __forceinline__ void stack_search(const FastPrim &h, CPTR(BandRec) BR
                                              int (&I2)[NR], float lo, float hi, bool wroot1, bool wtan, Pending &pd,
                                              uint32_t &fl, uint32_t vmask)
 {
-    const int cnt0 = pd.count;
+    const int cnt0 = pd.stack0 >= 0 ? pd.stack0 : pd.count;
     constexpr uint32_t zf = (AXIAL ? kFlopsBandZ0Axial : kFlopsBandZ0) + kFlopsBandRoot;
     // the band loop's per-band work on position w of ray slot i
     const auto band = [&](int i, int w, int orig, float sz) {
@@ -1010,7 +1012,7 @@ __device__ __forceinline__ void stack_search(const FastPrim &h, CPTR(BandRec) BR
         fl += (vmask >> i) & 1u ? 2u * kFlopsBandRoot : 0u;
         if (REFINE && f) {
             if (pd.count < PEND_K && pd.resume == NO_KEY) {
-                pd.buf[pd.count * blockDim.x + threadIdx.x] = ((uint32_t)orig << 3) | (uint32_t)i;
+                pd.buf[pd.count * blockDim.x + threadIdx.x] = ((uint32_t)orig << 3) | (uint32_t)(pd.slot0 + i);
                 ++pd.count;
             } else if (pd.resume == NO_KEY) {
                 pd.resume = b * 8;
@@ -2024,9 +2026,28 @@ __device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc)
                 const int j1 = ro[1];
                 for (int j = ro[0]; j < j1; ++j) {
                     const CylRun run = ldc(cst(G->bvc_runs) + j);
-                    if (run.stack)
-                        stack_loop<BNR, REFINE>(FC, cst(G->bvc_bands), run.begin, run.end,
-                                                kp.stacks >= 2 ? run.stack : 1, run.inv, R, tb, ib, pd, csf, vmask);
+                    if (kp.stacks >= 2 && run.stack >= 2) {
+                        // searched in two slices of 2 rays (the search is
+                        // compiled for 1 and 2); an overflow in either gives
+                        // back every entry of the stack
+                        static_assert(BNR == 4, "slices of 2 rays");
+                        Ray32 (&R0)[2] = *reinterpret_cast<Ray32 (*)[2]>(&R[0]);
+                        Ray32 (&R1)[2] = *reinterpret_cast<Ray32 (*)[2]>(&R[2]);
+                        float (&tb0)[2] = *reinterpret_cast<float (*)[2]>(&tb[0]);
+                        float (&tb1)[2] = *reinterpret_cast<float (*)[2]>(&tb[2]);
+                        int (&ib0)[2] = *reinterpret_cast<int (*)[2]>(&ib[0]);
+                        int (&ib1)[2] = *reinterpret_cast<int (*)[2]>(&ib[2]);
+                        pd.stack0 = pd.count;
+                        stack_loop<2, REFINE>(FC, cst(G->bvc_bands), run.begin, run.end, run.stack, run.inv, R0,
+                                              tb0, ib0, pd, csf, vmask & 3u);
+                        pd.slot0 = 2;
+                        stack_loop<2, REFINE>(FC, cst(G->bvc_bands), run.begin, run.end, run.stack, run.inv, R1,
+                                              tb1, ib1, pd, csf, vmask >> 2);
+                        pd.slot0 = 0;
+                        pd.stack0 = -1;
+                    } else if (run.stack)
+                        stack_loop<BNR, REFINE>(FC, cst(G->bvc_bands), run.begin, run.end, 1, run.inv, R, tb, ib, pd,
+                                                csf, vmask);
                     else
                         seg_loop<SEG_CYL, BNR, REFINE, false>(FC, run.begin, run.end, R, tb, ib, pd);
                 }

@@ -730,7 +730,9 @@ static void pick_shape(const rf_ctx *ctx, const rf_solve_opts *o, int &nr, int &
     if (o->math == RF_MATH_FAST && !o->use_bvs) {
         const int tr = o->transform ? o->transform : (ctx->tuned_transform ? ctx->tuned_transform : RF_TRANSFORM_VALU);
         mfma = tr == RF_TRANSFORM_MFMA;
-        nr = o->rays_per_thread ? o->rays_per_thread : (ctx->tuned_rpt ? ctx->tuned_rpt : 4);
+        // untuned default: 2 rays per thread where band stacks are searched
+        // (the search is compiled for 1 and 2), else 4
+        nr = o->rays_per_thread ? o->rays_per_thread : (ctx->tuned_rpt ? ctx->tuned_rpt : (o->per_test == 0 ? 2 : 4));
         block = o->block_size ? o->block_size : (ctx->tuned_block ? ctx->tuned_block : 256);
         if (mfma) nr = 4;   // 64 rays per wave and pass: one work item per lane-quarter
     } else if (o->block_size) {
