@@ -88,9 +88,11 @@ struct GeomDesc {
     const uint32_t *bv_cflops;
     // compat scan with band stacks: runs over the caller's order (stacks of
     // consecutive cylinders whose invm differs only in invm[11]), band records
-    // {invm[11], index} by caller position after one pad record
+    // {invm[11], index} by caller position after one pad record; c_run_of
+    // locates the run of a bounding surface's first child
     const CylRun *c_runs;
     const BandRec *c_bands;
+    const int32_t *c_run_of;            // run index of each caller position
     int32_t n_c_runs;
     int32_t pad3;
 };

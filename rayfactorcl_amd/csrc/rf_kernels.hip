@@ -256,22 +256,26 @@ __device__ __forceinline__ uint32_t stack_d(CPTR(rf_primitive) P, CPTR(BandRec) 
     return kFlopsStackD + (uint32_t)(e - b) * kFlopsBandD;
 }
 
-// findObject in the FP64 form with the compat runs (band stacks) when
-// `stacks`; adds the stacks' algorithmic flops to `sflops`.
-__device__ __forceinline__ int find_d_runs(CPTR(GeomDesc) G, const Ray64 &ray, int stacks, uint64_t &sflops)
+// findObject in the FP64 form over the caller's range [s, e) with the compat
+// runs when `stacks`: a stack cut by the range keeps the bands inside it (two
+// or more stay a stack, and a sorted stack stays sorted); adds the stacks'
+// algorithmic flops to `sflops`.
+__device__ __forceinline__ int find_d_runs(CPTR(GeomDesc) G, int s, int e, const Ray64 &ray, int stacks,
+                                           uint64_t &sflops)
 {
     CPTR(rf_primitive) P = cst(G->prims);
-    if (!stacks) return find_d(P, 0, G->n, ray);
+    if (!stacks || s >= e) return find_d(P, s, e, ray);
     int obj = -1;
     double tbest = dinf();
     const int n_runs = G->n_c_runs;
-    for (int j = 0; j < n_runs; ++j) {
+    for (int j = cst(G->c_run_of)[s]; j < n_runs; ++j) {
         const CylRun run = ldc(cst(G->c_runs) + j);
-        if (run.stack) {
-            sflops += stack_d(P, cst(G->c_bands), run.begin, run.end, stacks >= 2 ? run.stack : 1, run.inv, ray, tbest,
-                              obj);
+        if (run.begin >= e) break;
+        const int rb = max(run.begin, s), re = min(run.end, e);
+        if (run.stack && re - rb >= 2) {
+            sflops += stack_d(P, cst(G->c_bands), rb, re, stacks >= 2 ? run.stack : 1, run.inv, ray, tbest, obj);
         } else {
-            for (int w = run.begin; w < run.end; ++w) {
+            for (int w = rb; w < re; ++w) {
                 const rf_primitive p = ldc(P + w);
                 const double t = hit_d(p.type, p.invm, p.util1, p.util2, ray);
                 if (t < tbest) { tbest = t; obj = w; }
@@ -279,50 +283,6 @@ __device__ __forceinline__ int find_d_runs(CPTR(GeomDesc) G, const Ray64 &ray, i
         }
     }
     return obj;
-}
-
-// BV body shared by compat and fp64 (MATH selects the arithmetic).  The
-// final child scan of lanes whose winning bounding surface differs is done
-// one distinct surface at a time: the leader's surface is read with
-// readlane, so every scan runs over a wave-uniform range.
-template <int MATH, typename RayT>
-__device__ __forceinline__ int find_bv_cd(CPTR(rf_primitive) P, CPTR(rf_bounding_volume) B, int nbv,
-                                          const RayT &ray, uint64_t &tests)
-{
-    int bvIndex = -1, objectIndex = -1;
-    double tbest = dinf();
-    for (int w = 0; w < nbv; ++w) {
-        const rf_bounding_volume b = ldc(B + w);
-        double t;
-        if constexpr (MATH == KM_COMPAT) t = hit_c(b.type, b.invm, b.util1, b.util2, ray);
-        else t = hit_d(b.type, b.invm, b.util1, b.util2, ray);
-        ++tests;
-        if (t < tbest) {
-            if (b.settings == 1) {
-                if constexpr (MATH == KM_COMPAT) objectIndex = find_c(P, b.objStartIndex, b.objEndIndex, ray);
-                else objectIndex = find_d(P, b.objStartIndex, b.objEndIndex, ray);
-                tests += (uint64_t)(b.objEndIndex - b.objStartIndex);
-                if (objectIndex != -1) { tbest = t; bvIndex = w; }
-            } else {
-                tbest = t; bvIndex = w;
-            }
-        }
-    }
-    bool pending = (bvIndex != -1 && objectIndex == -1);
-    for (;;) {
-        const uint64_t mask = __ballot(pending);
-        if (mask == 0) break;
-        const int leader = __ffsll((unsigned long long)mask) - 1;
-        const int bsel = __builtin_amdgcn_readlane(bvIndex, leader);
-        if (pending && bvIndex == bsel) {
-            const int s = B[bsel].objStartIndex, e = B[bsel].objEndIndex;
-            if constexpr (MATH == KM_COMPAT) objectIndex = find_c(P, s, e, ray);
-            else objectIndex = find_d(P, s, e, ray);
-            tests += (uint64_t)(e - s);
-            pending = false;
-        }
-    }
-    return objectIndex;
 }
 
 // ---------------------------------------------------------------------------
@@ -624,6 +584,78 @@ __device__ __forceinline__ uint32_t find_c_multi(CPTR(GeomDesc) G, const Ray32 (
             scan_c_range<NR>(P, run.begin, run.end, ray, tbest, obj);
     }
     return sf;
+}
+
+// The compat child scan of one ray over the caller's range [s, e): the
+// compat runs cut to the range when `stacks` (as find_d_runs), else per test.
+__device__ __forceinline__ int find_c_runs(CPTR(GeomDesc) G, int s, int e, const Ray32 &ray, int stacks)
+{
+    CPTR(rf_primitive) P = cst(G->prims);
+    if (!stacks || s >= e) return find_c(P, s, e, ray);
+    const Ray32 R1[1] = {ray};
+    float tbest[1] = {finf()};
+    int obj[1] = {-1};
+    const int n_runs = G->n_c_runs;
+    for (int j = cst(G->c_run_of)[s]; j < n_runs; ++j) {
+        const CylRun run = ldc(cst(G->c_runs) + j);
+        if (run.begin >= e) break;
+        const int rb = max(run.begin, s), re = min(run.end, e);
+        if (run.stack && re - rb >= 2)
+            stack_c<1>(P, cst(G->c_bands), rb, re, stacks >= 2 ? run.stack : 1, run.inv, R1, tbest, obj);
+        else
+            scan_c_range<1>(P, rb, re, R1, tbest, obj);
+    }
+    return obj[0];
+}
+
+// BV body shared by compat and fp64 (MATH selects the arithmetic).  The
+// final child scan of lanes whose winning bounding surface differs is done
+// one distinct surface at a time: the leader's surface is read with
+// readlane, so every scan runs over a wave-uniform range.  Child scans use
+// the band stacks of the caller's order when `stacks` (cut to the child
+// range; their flops are not counted apart: tests x the scene's mean F).
+template <int MATH, typename RayT>
+__device__ __forceinline__ int find_bv_cd(CPTR(GeomDesc) G, const RayT &ray, uint64_t &tests, int stacks)
+{
+    CPTR(rf_bounding_volume) B = cst(G->bvs);
+    const int nbv = G->n_bvs;
+    uint64_t sf = 0;
+    const auto child = [&](int cs, int ce) {
+        if constexpr (MATH == KM_COMPAT) return find_c_runs(G, cs, ce, ray, stacks);
+        else return find_d_runs(G, cs, ce, ray, stacks, sf);
+    };
+    int bvIndex = -1, objectIndex = -1;
+    double tbest = dinf();
+    for (int w = 0; w < nbv; ++w) {
+        const rf_bounding_volume b = ldc(B + w);
+        double t;
+        if constexpr (MATH == KM_COMPAT) t = hit_c(b.type, b.invm, b.util1, b.util2, ray);
+        else t = hit_d(b.type, b.invm, b.util1, b.util2, ray);
+        ++tests;
+        if (t < tbest) {
+            if (b.settings == 1) {
+                objectIndex = child(b.objStartIndex, b.objEndIndex);
+                tests += (uint64_t)(b.objEndIndex - b.objStartIndex);
+                if (objectIndex != -1) { tbest = t; bvIndex = w; }
+            } else {
+                tbest = t; bvIndex = w;
+            }
+        }
+    }
+    bool pending = (bvIndex != -1 && objectIndex == -1);
+    for (;;) {
+        const uint64_t mask = __ballot(pending);
+        if (mask == 0) break;
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        const int bsel = __builtin_amdgcn_readlane(bvIndex, leader);
+        if (pending && bvIndex == bsel) {
+            const int s = B[bsel].objStartIndex, e = B[bsel].objEndIndex;
+            objectIndex = child(s, e);
+            tests += (uint64_t)(e - s);
+            pending = false;
+        }
+    }
+    return objectIndex;
 }
 
 #define ROW_S(P, i, R) __builtin_fmaf(P.r[4 * i], R.sx, __builtin_fmaf(P.r[4 * i + 1], R.sy, __builtin_fmaf(P.r[4 * i + 2], R.sz, P.r[4 * i + 3])))
@@ -1486,13 +1518,13 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                 int idx;
                 if constexpr (MATH == KM_COMPAT) {
                     const Ray32 ray = emit_c(&E, r4);
-                    idx = BV ? find_bv_cd<KM_COMPAT>(cst(G->prims), cst(G->bvs), G->n_bvs, ray, tests)
+                    idx = BV ? find_bv_cd<KM_COMPAT>(G, ray, tests, kp.stacks)
                              : find_c(cst(G->prims), 0, G->n, ray);
                 } else if constexpr (MATH == KM_FP64) {
                     const double d4[4] = {r4[0], r4[1], r4[2], r4[3]};
                     const Ray64 ray = emit_d(&E, d4);
-                    idx = BV ? find_bv_cd<KM_FP64>(cst(G->prims), cst(G->bvs), G->n_bvs, ray, tests)
-                             : find_d_runs(G, ray, kp.stacks, sflops);
+                    idx = BV ? find_bv_cd<KM_FP64>(G, ray, tests, kp.stacks)
+                             : find_d_runs(G, 0, G->n, ray, kp.stacks, sflops);
                 } else {
                     const Ray32 ray = emit_fast(&E, r4);
                     if constexpr (BV) {

@@ -91,6 +91,7 @@ struct rf_scene {
     int64_t flops_per_ray_cnonstack = 0; // the same for the compat scan's stacks (caller order)
     CylRun *d_c_runs = nullptr;          // compat scan: runs over the caller's order
     BandRec *d_c_bands = nullptr;
+    int32_t *d_c_run_of = nullptr;       // run index of each caller position (BV child scans)
     int64_t flops_bvs = 0;               // sum over bounding surfaces of F_type (BV tests per ray)
     CylRun *d_bvc_runs = nullptr;        // binned BV form: cylinder runs of each surface's children
     int32_t *d_bvc_run_off = nullptr;
@@ -409,7 +410,7 @@ extern "C" int64_t rf_scene_footprint(int32_t n, int32_t nbv)
 {
     if (n < 0 || nbv < 0) return -1;
     return (int64_t)n * (sizeof(rf_primitive) + 2 * sizeof(FastPrim) + 64 + 8 + 2 * sizeof(BandRec) +
-                         2 * sizeof(CylRun)) + 8 * 256 +
+                         2 * sizeof(CylRun) + 4) + 9 * 256 +
            (int64_t)nbv * (sizeof(rf_bounding_volume) + sizeof(FastPrim) + sizeof(BvMeta) + (N_SEG + 1) * 4 +
                            4 + 8) +
            (nbv ? (int64_t)n * (sizeof(FastPrim) + sizeof(BandRec) + sizeof(CylRun)) + 6 * 256 : 0) +
@@ -421,7 +422,7 @@ static void free_scene_mem(rf_scene *s)
     (void)hipFree(s->d_quads); (void)hipFree(s->d_pos2orig); (void)hipFree(s->d_orig2pos);
     (void)hipFree(s->d_bvc_fast); (void)hipFree(s->d_bvc_seg);
     (void)hipFree(s->d_cyl_runs); (void)hipFree(s->d_bands);
-    (void)hipFree(s->d_c_runs); (void)hipFree(s->d_c_bands);
+    (void)hipFree(s->d_c_runs); (void)hipFree(s->d_c_bands); (void)hipFree(s->d_c_run_of);
     (void)hipFree(s->d_bvc_runs); (void)hipFree(s->d_bvc_run_off); (void)hipFree(s->d_bvc_bands);
     (void)hipFree(s->d_bv_cflops);
     (void)hipFree(s->d_prims); (void)hipFree(s->d_fast); (void)hipFree(s->d_fast_orig);
@@ -588,6 +589,10 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     up((void **)&s->d_cyl_runs, runs.data(), runs.size() * sizeof(CylRun));
     up((void **)&s->d_c_runs, c_runs.data(), c_runs.size() * sizeof(CylRun));
     up((void **)&s->d_c_bands, c_bands.data(), c_bands.size() * sizeof(BandRec));
+    std::vector<int32_t> c_run_of(n > 0 ? n : 1, 0);
+    for (size_t j = 0; j < c_runs.size(); ++j)
+        for (int32_t k = c_runs[j].begin; k < c_runs[j].end; ++k) c_run_of[k] = (int32_t)j;
+    up((void **)&s->d_c_run_of, c_run_of.data(), c_run_of.size() * sizeof(int32_t));
     up((void **)&s->d_bands, bands.data(), bands.size() * sizeof(BandRec));
     if (binned) {
         up((void **)&s->d_bvc_runs, bvc_runs.data(), bvc_runs.size() * sizeof(CylRun));
@@ -611,6 +616,7 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     d.bands = s->d_bands ? s->d_bands + 1 : nullptr;
     d.n_c_runs = (int32_t)c_runs.size(); d.c_runs = s->d_c_runs;
     d.c_bands = s->d_c_bands ? s->d_c_bands + 1 : nullptr;
+    d.c_run_of = s->d_c_run_of;
     d.bvc_runs = s->d_bvc_runs; d.bvc_run_off = s->d_bvc_run_off;
     d.bvc_bands = s->d_bvc_bands ? s->d_bvc_bands + 1 : nullptr;
     d.bv_cflops = s->d_bv_cflops;

@@ -259,3 +259,23 @@ def test_compat_stacks_traces(ctx, oracle, math, stacked):
         b = oracle.trace(sc.prims, None, e, 100, 1500, mode=math)
         assert np.array_equal(a, b), (e, np.argwhere(a != b)[:5])
     dev.close()
+
+
+@pytest.mark.parametrize("stacked", STACKED)
+@pytest.mark.parametrize("math", ["compat", "fp64"])
+def test_compat_stacks_bv_match_oracle(ctx, oracle, math, stacked):
+    """runBVGPU semantics: each surface's child range is scanned with the band
+    stacks of the caller's order cut to that range.  One surface's range is
+    moved to start five bands inside the wall's stack."""
+    sc = scenes.parse_xml(scenes.furnace_xml(with_bv=True, wall_bands=60, preform_bands=20))
+    bvs = sc.bvs.copy()
+    bvs["objStartIndex"][1] = max(0, int(bvs["objStartIndex"][1]) - 5)
+    dev = rf.DeviceScene(ctx, sc.prims, bvs)
+    n_wi = 128
+    got = rf.solve(ctx, dev, math=math, use_bvs=True, rays_per_emitter=4 * n_wi, per_test=stacked)
+    want, _ = oracle.solve(sc.prims, bvs, n_wi=n_wi, mode=math, use_bvs=True)
+    diff = np.argwhere(got.hits != want)
+    assert diff.size == 0, f"{len(diff)} cells differ, first {diff[:5].tolist()}"
+    a = rf.solve(ctx, dev, math=math, use_bvs=True, rays_per_emitter=4 * n_wi, per_test=True)
+    assert np.array_equal(a.hits, got.hits) and a.timing["tests"] == got.timing["tests"]
+    dev.close()
