@@ -118,7 +118,8 @@ typedef struct rf_solve_opts {
     uint32_t seed;            /* RF_RNG_SEEDED */
     uint32_t geometry;        /* RF_RNG_SEEDED: geometry id (batched solves use their index) */
     int32_t refine;           /* RF_MATH_FAST: 1 = FP64 re-solve of flagged roots */
-    int32_t rays_per_thread;  /* 0 = autotuned, else the default (2 when per_test = 0, else 4); or 1, 2, 4, 8 */
+    int32_t rays_per_thread;  /* 0 = autotuned, else the default: 2 when per_test = 0 and a scene has a
+                                 sorted band stack, else 4; or 1, 2, 4, 8 */
     int32_t block_size;       /* 0 = autotuned/default; else 64..256, multiple of 64 */
     int32_t transform;        /* RF_MATH_FAST plain form: rf_transform */
     float refine_margin;      /* RF_MATH_FAST: relative margin of the FP64 re-solve flags, 0 = 2^-14;

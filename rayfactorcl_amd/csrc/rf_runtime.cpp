@@ -98,6 +98,7 @@ struct rf_scene {
     BandRec *d_bvc_bands = nullptr;
     uint32_t *d_bv_cflops = nullptr;
     double flops_per_test_avg = 0.0;     // BV accounting
+    bool has_search = false;             // a fast-form band stack with sorted offsets
 };
 
 // Binned BV form: at most this many bounding surfaces per scene (LDS buckets;
@@ -534,6 +535,7 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     // one pad record (the kernel's prefetch reads the record before a stack)
     std::vector<CylRun> runs;
     const int64_t fnonstack = fsum + build_cyl_runs(fast, seg_begin[SEG_CYL], seg_end[SEG_CYL], runs);
+    for (const CylRun &r : runs) s->has_search = s->has_search || r.stack >= 2;
     std::vector<BandRec> bands(n + 1, BandRec{0.f, 0});
     for (int32_t k = seg_begin[SEG_CYL]; k < seg_end[SEG_CYL]; ++k) bands[k + 1] = BandRec{fast[k].r[11], fast[k].orig};
     // compat scan: stacks over the caller's order, band records {invm[11], index}
@@ -728,7 +730,10 @@ static size_t bvbin_lds(int block, int bv_slots, int bins)
            (size_t)bins * 4;
 }
 
-static void pick_shape(const rf_ctx *ctx, const rf_solve_opts *o, int &nr, int &block, int &mfma)
+// searched: some scene of the solve has a band stack to search (the untuned
+// default shape depends on it).
+static void pick_shape(const rf_ctx *ctx, const rf_solve_opts *o, int &nr, int &block, int &mfma,
+                       bool searched = false)
 {
     nr = 1;
     block = 256;
@@ -738,7 +743,8 @@ static void pick_shape(const rf_ctx *ctx, const rf_solve_opts *o, int &nr, int &
         mfma = tr == RF_TRANSFORM_MFMA;
         // untuned default: 2 rays per thread where band stacks are searched
         // (the search is compiled for 1 and 2), else 4
-        nr = o->rays_per_thread ? o->rays_per_thread : (ctx->tuned_rpt ? ctx->tuned_rpt : (o->per_test == 0 ? 2 : 4));
+        nr = o->rays_per_thread ? o->rays_per_thread
+                                : (ctx->tuned_rpt ? ctx->tuned_rpt : (o->per_test == 0 && searched ? 2 : 4));
         block = o->block_size ? o->block_size : (ctx->tuned_block ? ctx->tuned_block : 256);
         if (mfma) nr = 4;   // 64 rays per wave and pass: one work item per lane-quarter
     } else if (o->block_size) {
@@ -830,7 +836,7 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
     const hipStream_t st = lane_stream(ctx, lane);
     if (n_wi <= 0) return RF_OK;
     int nr, block, mfma;
-    pick_shape(ctx, o, nr, block, mfma);
+    pick_shape(ctx, o, nr, block, mfma, s->has_search);
     KParams kp{};
     kp.geoms = s->d_desc;
     kp.units = nullptr;
@@ -993,8 +999,10 @@ static int prepare_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms,
                               const std::vector<std::vector<int64_t>> &budgets, const std::vector<uint32_t> &ids,
                               LaunchCfg &c, KParams &kp, bool &empty)
 {
+    bool searched = false;
+    for (int g = 0; g < n_geoms; ++g) searched = searched || scenes[g]->has_search;
     int nr, block, mfma;
-    pick_shape(ctx, o, nr, block, mfma);
+    pick_shape(ctx, o, nr, block, mfma, searched);
     const bool use_bv = o->use_bvs != 0;
     // geometry descriptors
     std::vector<GeomDesc> descs(n_geoms);
