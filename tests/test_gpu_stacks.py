@@ -279,3 +279,40 @@ def test_compat_stacks_bv_match_oracle(ctx, oracle, math, stacked):
     a = rf.solve(ctx, dev, math=math, use_bvs=True, rays_per_emitter=4 * n_wi, per_test=True)
     assert np.array_equal(a.hits, got.hits) and a.timing["tests"] == got.timing["tests"]
     dev.close()
+
+
+def _uneven_scene(seed=7):
+    """Stacks whose bands are in z order but unevenly spaced (random gaps and
+    overlaps), so the even-spacing guess often misses and the binary search
+    decides: 80 wall bands of height 0.05 listed bottom up, 30 inner bands of
+    height 0.1 listed top down, closed by a disc at each end."""
+    rng = np.random.default_rng(seed)
+    z_wall = np.sort(rng.uniform(0.0, 3.95, 80))
+    z_in = np.sort(rng.uniform(0.5, 3.0, 30))[::-1]
+    parts = [_band(z, z + 0.05) for z in z_wall]
+    parts += [_band(z, z + 0.1, r=0.2) for z in z_in]
+    parts += [scenes._prim("disc", scale=(0.5, 0.5, 1)),
+              scenes._prim("disc", scale=(0.5, 0.5, 1), rotate=scenes.FLIP_X, translate=(0, 0, 4.0))]
+    return scenes.parse_xml(scenes._doc("unevenly spaced band stacks", "".join(parts)))
+
+
+@pytest.mark.parametrize("margin", [0.0, 0.25])
+@pytest.mark.parametrize("rpt", [1, 2])
+def test_stacks_uneven_spacing(ctx, margin, rpt):
+    sc = _uneven_scene()
+    a, b = _pair(ctx, sc, 0, rays_per_emitter=1 << 14, form="persistent", rays_per_thread=rpt,
+                 refine_margin=margin)
+    _same(a, b)
+    assert a.hits[:, :80].sum() > 0 and a.hits[:, 80:110].sum() > 0
+
+
+@pytest.mark.parametrize("math", ["compat", "fp64"])
+def test_compat_stacks_uneven_spacing(ctx, oracle, math):
+    sc = _uneven_scene()
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    n_wi = 256
+    got = rf.solve(ctx, dev, math=math, rays_per_emitter=4 * n_wi)
+    want, _ = oracle.solve(sc.prims, None, n_wi=n_wi, mode=math)
+    diff = np.argwhere(got.hits != want)
+    assert diff.size == 0, f"{len(diff)} cells differ, first {diff[:5].tolist()}"
+    dev.close()
