@@ -2181,7 +2181,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REFINE
 // The compat scan (four rays per visit) is held to the same five waves.
 template <int MATH, bool BV, bool TRACE, int NR, bool REFINE>
 __global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu((NR == 4 || NR == 2 || (MATH == KM_COMPAT && !BV)) ? 5 : 1))) k_queue(KParams kp)
+__attribute__((amdgpu_waves_per_eu((NR == 4 || NR == 2 || (MATH == KM_COMPAT && !BV)) ? 5 : (MATH == KM_FP64 ? 4 : 1)))) k_queue(KParams kp)
 {
     // dynamic LDS: [bins: kp.bins x u32][pending re-solves: PEND_K x blockDim x u32]
     extern __shared__ uint32_t smem[];
