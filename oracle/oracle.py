@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/librf_oracle.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker and by bench.py's cpu_baseline leg.  The product never uses it.
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to
+check its result and by bench.py's cpu_baseline leg.  The product never uses it.
 """
 from __future__ import annotations
 

@@ -1,6 +1,6 @@
 """Shared fixtures.  Tests marked ``gpu`` need an MI355X; everything else runs on CPU.
 
-The CPU oracle (oracle/) is imported here only as the checker."""
+The CPU oracle (oracle/) is imported here only to check results."""
 from __future__ import annotations
 
 import sys

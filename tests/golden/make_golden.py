@@ -1,4 +1,4 @@
-"""Writes the golden fixtures of tests/golden/ from the reference-built checkers.
+"""Writes the golden fixtures of tests/golden/ with the programs built from the reference sources.
 
 Runs ONLY in a container that holds the reference sources (``make -C oracle ref``
 builds ``oracle/_ref/`` from them).  Outputs, all small and committed:
