@@ -142,11 +142,24 @@ def find_f32(prims, start: int, end: int, ray) -> int:
     return lib().rfo_find_f32(_p(p), start, end, _p(r))
 
 
+def find_f64(prims, start: int, end: int, ray) -> int:
+    p = np.ascontiguousarray(prims, dtype=PRIM_DTYPE)
+    r = np.ascontiguousarray(ray, dtype=np.float64)
+    return lib().rfo_find_f64(_p(p), start, end, _p(r))
+
+
 def find_bv_f32(prims, bvs, ray) -> int:
     p = np.ascontiguousarray(prims, dtype=PRIM_DTYPE)
     b = np.ascontiguousarray(bvs, dtype=BV_DTYPE)
     r = np.ascontiguousarray(ray, dtype=np.float32)
     return lib().rfo_find_bv_f32(_p(p), _p(b), len(b), _p(r))
+
+
+def find_bv_f64(prims, bvs, ray) -> int:
+    p = np.ascontiguousarray(prims, dtype=PRIM_DTYPE)
+    b = np.ascontiguousarray(bvs, dtype=BV_DTYPE)
+    r = np.ascontiguousarray(ray, dtype=np.float64)
+    return lib().rfo_find_bv_f64(_p(p), _p(b), len(b), _p(r))
 
 
 def nthreads(ray_density: int, n_samples: int, count: int = 4, wgsize: int = 128) -> int:

@@ -6,6 +6,7 @@ import math
 import numpy as np
 import pytest
 
+import oracle_pins
 from conftest import GOLDEN
 
 INF = float("inf")
@@ -129,47 +130,18 @@ def test_stable_roots_agree_with_textbook(oracle):
         assert (t == INF) if not good else t == pytest.approx(good[0], rel=1e-9, abs=1e-12)
 
 
-def test_find_object_tie_lowest_index(oracle):
-    import rayfactorcl_amd as rf
-    p = np.zeros(3, dtype=rf.PRIM_DTYPE)
-    p["type"] = DISC
-    p["m"] = I12
-    p["invm"] = I12
-    r = np.array([0.2, 0.1, 3, 0, 0, -1], dtype=np.float32)
-    assert oracle.find_f32(p, 0, 3, r) == 0          # exact tie: lowest index (Q10)
-    assert oracle.find_f32(p, 1, 3, r) == 1
-    # brute force agrees with the per-primitive minimum on random rays
-    rng = np.random.default_rng(3)
-    p["invm"][:, 11] = -np.arange(3.0)                # discs at z = 0, 1, 2
-    p["m"][:, 11] = np.arange(3.0)
-    for _ in range(100):
-        s = np.concatenate([rng.uniform(-0.5, 0.5, 2), [rng.uniform(-1, 3)]])
-        c = np.array([0, 0, rng.choice([-1.0, 1.0])])
-        rr = np.concatenate([s, c]).astype(np.float32)
-        ts = [oracle.hit_f32(DISC, p["invm"][i], 0, 0, rr) for i in range(3)]
-        best = int(np.argmin(ts)) if min(ts) < INF else -1
-        assert oracle.find_f32(p, 0, 3, rr) == best
+@pytest.mark.parametrize("mode", ["compat", "fp64"])
+def test_find_object_tie_lowest_index(oracle, mode):
+    oracle_pins.pin_find_tie(oracle, mode)
 
 
-def test_bv_stale_index_quirk_q12(oracle):
+@pytest.mark.parametrize("mode", ["compat", "fp64"])
+def test_bv_stale_index_quirk_q12(oracle, mode):
     """runBVGPU: an intermediate-check BV sets objectIndex; a later plain BV that
-    wins keeps that stale index (RayFactorGPU.cl:596-619)."""
-    import rayfactorcl_amd as rf
-    p = np.zeros(2, dtype=rf.PRIM_DTYPE)
-    b = np.zeros(2, dtype=rf.BV_DTYPE)
-    p["type"] = DISC
-    p["m"] = I12
-    p["invm"] = I12
-    p["invm"][:, 11] = [-1.0, -2.0]                     # discs at z = 1 and z = 2
-    b["type"] = DISC
-    b["invm"] = p["invm"]
-    b["settings"] = [1, 0]
-    b["objStartIndex"] = [0, 1]
-    b["objEndIndex"] = [1, 2]
-    # ray from z=3 downward: BV1 (z=2) is nearer than BV0 (z=1), but BV0 is tested first
-    # with intermediate check -> objectIndex 0; BV1 then wins on t -> stale index 0 survives
-    r = np.array([0, 0, 3, 0, 0, -1], dtype=np.float32)
-    assert oracle.find_bv_f32(p, b, r) == 0
-    # without the intermediate check the child scan of the winner gives 1
-    b["settings"] = [0, 0]
-    assert oracle.find_bv_f32(p, b, r) == 1
+    wins keeps that stale index (RayFactorGPU.cl:596-619); control without it."""
+    oracle_pins.pin_bv_stale_index(oracle, mode)
+
+
+@pytest.mark.parametrize("mode", ["compat", "fp64"])
+def test_bv_intermediate_check_miss(oracle, mode):
+    oracle_pins.pin_bv_intermediate_miss(oracle, mode)

@@ -1,59 +1,56 @@
 """Whole-solve checks of the oracle against what the mathematics fixes:
-Howell-catalogue closed forms (sample scenes and the config-1 scene),
-reciprocity and closure.  Errors use batch means over blocks of work items
-(the 4 rays of a work item share one random block).  The reference discards
-hits closer than EPS = 1e-3 (RayFactorGPU.cl:7), which removes a small
-fraction of rays that graze a touching edge: scenes with touching surfaces get
-an allowance of 2e-3 for that loss."""
+Howell-catalogue closed forms (sample scenes, their bounding-surface variants
+and the config-1 scene), reciprocity and closure.
+
+Each measured view factor must equal the closed form less the first-order
+seam loss that the EPS = 1e-3 self-intersection cut-off causes at touching
+edges (derived in closed_forms.py), within 3 standard errors computed from
+per-work-item hit counts (the 4 rays of a work item share one random block)."""
 import math
 
 import numpy as np
 import pytest
 
-BLOCKS, BLOCK_WI = 16, 1 << 13
+from closed_forms import CASES, K, c122_quadrature, check, seam_loss, vf_and_sigma
+
+N_WI = 1 << 19            # 2^21 rays per emitter: sigma ~ 3e-4 at F = 0.2, 3e-5 at F = 0.999
 
 
-def vf_with_error(oracle, sc, i, j, mode):
-    est = []
-    for b in range(BLOCKS):
-        row = oracle.solve_range(sc.prims, sc.bvs, i, b * BLOCK_WI, BLOCK_WI, mode=mode,
-                                 use_bvs=sc.useBoundingVolumes)
-        est.append(row[j] / (4.0 * BLOCK_WI))
-    est = np.asarray(est)
-    return est.mean(), est.std(ddof=1) / math.sqrt(BLOCKS)
+def measure(oracle, sc, i, j, mode):
+    idx = oracle.trace(sc.prims, sc.bvs, i, 0, N_WI, mode=mode, use_bvs=sc.useBoundingVolumes)
+    return vf_and_sigma(idx, j)
 
 
-def c122_quadrature():
-    # sphere R=1 at height h=1.5 above the corner of a 2x2 rectangle: F = (1/4pi) int h/d^3 dA
-    from numpy.polynomial.legendre import leggauss
-    x, w = leggauss(200)
-    X = 1 - (x + 1)          # [-1, 1] (rectangle), corner at (1,1)
-    dx = (X[:, None] - 1) ** 2 + (X[None, :] - 1) ** 2
-    d = np.sqrt(dx + 1.5 ** 2)
-    return float((w[:, None] * w[None, :] * 1.5 / d ** 3).sum() / (4 * math.pi))
-
-
-CASES = [
-    # scene, emitter, receiver, closed form, edge allowance
-    ("input-C14", 0, 1, 0.2000438, 2e-3),                 # Howell C-14, H = W = 1
-    ("input-C47", 0, 1, 0.3819660, 0.0),                  # disc -> coaxial annulus
-    ("input-C52", 0, 1, 0.1534047, 0.0),                  # ring -> coaxial ring
-    ("input-C109", 1, 0, 1.0, 2e-3),                      # disc -> cone over it
-    ("input-C109", 0, 1, 1 / math.sqrt(2), 2e-3),         # cone -> base disc (reciprocity)
-    ("config1", 1, 2, 0.1715729, 0.0),                    # end disc -> opposite end disc
-    ("config1", 1, 0, 0.8284271, 2e-3),                   # end disc -> wall
-    ("config1", 0, 1, 0.2071068, 2e-3),                   # wall -> end disc
-    ("config1", 0, 0, 0.5857864, 2e-3),                   # wall -> wall
-    ("config1", 3, 4, 0.3819660, 0.0),                    # coaxial unit discs, spacing 1
-]
+def test_clamped_cosine_closed_form():
+    """K(b) = E[max(0, d.v)] over cosine-weighted d, against 2-D quadrature."""
+    from scipy import integrate
+    for b in (0.0, math.pi / 6, math.pi / 4, math.pi / 2, 2.0, math.pi):
+        v = (-math.sin(b), 0.0, math.cos(b))
+        f = lambda ph, th: max(0.0, v[0] * math.sin(th) * math.cos(ph) + v[2] * math.cos(th)) * \
+            math.cos(th) * math.sin(th) / math.pi
+        q, _ = integrate.dblquad(f, 0.0, math.pi / 2, 0.0, 2 * math.pi, epsabs=1e-10)
+        assert q == pytest.approx(K(b), abs=1e-8), b
+    assert K(math.pi / 2) == pytest.approx(2 / (3 * math.pi))
 
 
 @pytest.mark.parametrize("mode", ["compat", "fp64"])
-@pytest.mark.parametrize("scene,i,j,exact,allow", CASES)
-def test_closed_forms(oracle, golden_scene, mode, scene, i, j, exact, allow):
+@pytest.mark.parametrize("scene,i,j,exact,loss", CASES)
+def test_closed_forms(oracle, golden_scene, mode, scene, i, j, exact, loss):
     sc = golden_scene(scene)
-    vf, se = vf_with_error(oracle, sc, i, j, mode)
-    assert abs(vf - exact) <= 4 * se + allow + 1e-4, (vf, exact, se)
+    assert sc.useBoundingVolumes == scene.endswith("-BV")
+    vf, se = measure(oracle, sc, i, j, mode)
+    assert check(vf, se, exact, loss), (vf, exact - loss, se)
+
+
+def test_seam_loss_is_resolved(oracle, golden_scene):
+    """C-109 disc -> cone: the predicted loss (1.42e-3) is ~50 standard errors;
+    the closed form without it fails, with it passes (both forms)."""
+    sc = golden_scene("input-C109")
+    _, _, _, exact, loss = next(c for c in CASES if c[0] == "input-C109" and c[1] == 1)
+    for mode in ("compat", "fp64"):
+        vf, se = measure(oracle, sc, 1, 0, mode)
+        assert loss > 40 * se
+        assert check(vf, se, exact, loss) and not check(vf, se, exact, 0.0)
 
 
 def test_c122_sphere_to_rectangle(oracle, golden_scene):
@@ -65,12 +62,12 @@ def test_c122_sphere_to_rectangle(oracle, golden_scene):
     sc = golden_scene("input-C122")
     exact = c122_quadrature()
     assert exact == pytest.approx(math.atan(16 / 9 / math.sqrt(41 / 9)) / (4 * math.pi), rel=1e-6)
-    vf, se = vf_with_error(oracle, sc, 0, 1, "fp64")
-    assert abs(vf - exact) <= 4 * se + 1e-4
-    vf, se = vf_with_error(oracle, sc, 1, 0, "fp64")
-    assert abs(vf - math.pi * exact) <= 4 * se + 1e-4          # reciprocity, A_s / A_r = pi
-    vfc, sec = vf_with_error(oracle, sc, 1, 0, "compat")
-    assert abs(vfc - math.pi * exact) <= 4 * sec + 1e-4
+    vf, se = measure(oracle, sc, 0, 1, "fp64")
+    assert check(vf, se, exact, 0.0)
+    vf, se = measure(oracle, sc, 1, 0, "fp64")
+    assert check(vf, se, math.pi * exact, 0.0)                   # reciprocity, A_s / A_r = pi
+    vfc, sec = measure(oracle, sc, 1, 0, "compat")
+    assert check(vfc, sec, math.pi * exact, 0.0)
 
 
 def test_reciprocity_and_closure(oracle, golden_scene):
@@ -81,7 +78,15 @@ def test_reciprocity_and_closure(oracle, golden_scene):
     for a in range(3):
         for b in range(3):
             assert A[a] * F[a, b] == pytest.approx(A[b] * F[b, a], rel=0.02, abs=2e-3)
-    assert np.all(F.sum(axis=1) > 0.997) and np.all(F.sum(axis=1) <= 1.0)
+    # closure: each row of the closed enclosure sums to 1 less the rays lost
+    # at its rims (wall: two rims, end disc: one), i.e. the fraction of rays
+    # that hit nothing is that seam loss
+    lost = [2 * seam_loss(2 * math.pi, 4 * math.pi, math.pi / 2), seam_loss(2 * math.pi, math.pi, math.pi / 2),
+            seam_loss(2 * math.pi, math.pi, math.pi / 2)]
+    for e in range(3):
+        miss, se = measure(oracle, sc, e, -1, "fp64")
+        assert check(miss, se, lost[e], 0.0), (e, miss, lost[e], se)
+        assert miss > 0.0
 
 
 def test_compat_and_fp64_use_the_same_stream(oracle, golden_scene):
