@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define RF_ABI_VERSION 2
+#define RF_ABI_VERSION 3
 
 /* ------------------------------------------------------------------ */
 /* Scene ABI                                                           */
@@ -263,19 +263,46 @@ typedef struct rf_stream rf_stream;
 /* A service that keeps scenes and tallies resident on the device and batches
  * queued requests of unequal size into persistent launches.  Admission
  * rejects (RF_ERR_CAPACITY) a request whose scene footprint would push the
- * resident total past hbm_limit_bytes. */
+ * resident total past hbm_limit_bytes.  No reference counterpart (the
+ * reference solves one scene per process run, main.cpp:275-284). */
 int32_t rf_stream_create(rf_ctx *ctx, int64_t hbm_limit_bytes, rf_stream **out);
 int32_t rf_stream_destroy(rf_stream *st);
+/* Cost limits, in estimated ray-primitive evaluations (rays x the plain
+ * scan's evaluations per ray: one per primitive outside band stacks, per stack
+ * one quadric plus ~6 band evaluations when searched or one per band when
+ * walked).  max_queued_tests: admission rejects (RF_ERR_CAPACITY) a submit
+ * that would push the queued, not yet flushed, cost past it; a request
+ * costlier than the limit on its own is admitted only onto an empty queue.
+ * max_launch_tests: a flush packs each option group's requests, cheapest
+ * first, into persistent launches of at most this cost (a costlier request
+ * runs alone).  0 = no limit (the defaults). */
+int32_t rf_stream_set_cost_limits(rf_stream *st, double max_queued_tests, double max_launch_tests);
+/* Estimated cost of the queued requests (-1 for a NULL stream). */
+double rf_stream_queued_cost(const rf_stream *st);
 /* Enqueues one solve; the primitives are copied.  *request_id identifies it. */
 int32_t rf_stream_submit(rf_stream *st, const rf_primitive *prims, int32_t n_prims,
                          const rf_solve_opts *opts, int64_t *request_id);
-/* Runs every queued request in one persistent launch; returns how many ran. */
+/* Runs the queued requests: requests share a persistent launch only when
+ * rf_opts_same_tallies holds for their options; launches are packed as set by
+ * rf_stream_set_cost_limits and run in order, each request completing with
+ * its launch.  On an error, the requests not completed stay queued (a later
+ * flush runs them) and *n_completed counts those that did. */
 int32_t rf_stream_flush(rf_stream *st, int32_t *n_completed);
 /* Copies a completed request's n*n hits (and per-emitter rays) out and
  * releases its device memory.  RF_ERR_INVALID_ARG if not completed. */
 int32_t rf_stream_collect(rf_stream *st, int64_t request_id, int64_t *hits, int64_t *rays,
                           double *latency_ms);
 int64_t rf_stream_resident_bytes(const rf_stream *st);
+/* 1 if two solves may share one persistent launch: equal math, RNG, bounding
+ * surface use, budget rule (and n_samples under the reference budget), seed,
+ * launch shape and scan form, and for the fast form equal FP64 re-solve,
+ * re-solve margin and transform unit -- every option that can change a
+ * tally.  0 otherwise; RF_ERR_INVALID_ARG for a NULL argument.  Host only. */
+int32_t rf_opts_same_tallies(const rf_solve_opts *a, const rf_solve_opts *b);
+/* Test hook (no reference counterpart): the nth solve issued on the context
+ * from now (1 = the next) fails with RF_ERR_HIP before any device work; 0
+ * cancels.  Used to check the streaming service's recovery. */
+int32_t rf_debug_fail_solve(rf_ctx *ctx, int32_t nth);
 
 /* ------------------------------------------------------------------ */
 /* Host-side scene input and result output                             */

@@ -118,6 +118,10 @@ SIGNATURES = {
     "rf_stream_flush": (c_int32, [_P, POINTER(c_int32)]),
     "rf_stream_collect": (c_int32, [_P, c_int64, _P, _P, POINTER(c_double)]),
     "rf_stream_resident_bytes": (c_int64, [_P]),
+    "rf_stream_set_cost_limits": (c_int32, [_P, c_double, c_double]),
+    "rf_stream_queued_cost": (c_double, [_P]),
+    "rf_opts_same_tallies": (c_int32, [POINTER(SolveOpts), POINTER(SolveOpts)]),
+    "rf_debug_fail_solve": (c_int32, [_P, c_int32]),
     "rf_scene_load_xml": (c_int32, [c_char_p, POINTER(HostScene)]),
     "rf_scene_parse_xml": (c_int32, [c_char_p, c_size_t, POINTER(HostScene)]),
     "rf_host_scene_free": (None, [POINTER(HostScene)]),

@@ -62,9 +62,16 @@ struct rf_ctx {
     std::vector<uint64_t> graph_key;     // solve captured in graph_exec
     std::vector<uint64_t> last_key;      // last persistent solve run eagerly
     int64_t graph_replays = 0;
+    int fail_in = 0;                     // test hook: the fail_in-th next solve fails (0 = off)
     // live objects owned by the context (released by rf_ctx_destroy)
     std::set<rf_scene *> scenes;
     std::set<rf_stream *> streams;
+};
+
+// A band stack of the fast scan, for cost estimates.
+struct StackInfo {
+    int32_t bands;
+    bool sorted;                         // z offsets sorted: searched by default
 };
 
 struct rf_scene {
@@ -99,6 +106,8 @@ struct rf_scene {
     uint32_t *d_bv_cflops = nullptr;
     double flops_per_test_avg = 0.0;     // BV accounting
     bool has_search = false;             // a fast-form band stack with sorted offsets
+    std::vector<StackInfo> stack_info;   // the fast scan's band stacks
+    int32_t stack_prims = 0;             // primitives inside them
 };
 
 // Binned BV form: at most this many bounding surfaces per scene (LDS buckets;
@@ -535,7 +544,13 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     // one pad record (the kernel's prefetch reads the record before a stack)
     std::vector<CylRun> runs;
     const int64_t fnonstack = fsum + build_cyl_runs(fast, seg_begin[SEG_CYL], seg_end[SEG_CYL], runs);
-    for (const CylRun &r : runs) s->has_search = s->has_search || r.stack >= 2;
+    for (const CylRun &r : runs) {
+        s->has_search = s->has_search || r.stack >= 2;
+        if (r.stack) {
+            s->stack_info.push_back(StackInfo{r.end - r.begin, r.stack >= 2});
+            s->stack_prims += r.end - r.begin;
+        }
+    }
     std::vector<BandRec> bands(n + 1, BandRec{0.f, 0});
     for (int32_t k = seg_begin[SEG_CYL]; k < seg_end[SEG_CYL]; ++k) bands[k + 1] = BandRec{fast[k].r[11], fast[k].orig};
     // compat scan: stacks over the caller's order, band records {invm[11], index}
@@ -1206,6 +1221,8 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
     const double w0 = now_ms();
     int rc = check_opts(ctx, o);
     if (rc) return rc;
+    if (ctx->fail_in > 0 && --ctx->fail_in == 0)
+        return set_err(ctx, RF_ERR_HIP, "injected failure (rf_debug_fail_solve)");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     std::vector<std::vector<int64_t>> budgets(n_geoms);
     std::vector<uint32_t> ids(n_geoms);
@@ -1416,9 +1433,53 @@ extern "C" int32_t rf_autotune(rf_ctx *ctx, rf_scene *const *scenes, int32_t n_g
 }
 
 // ---------------------------------------------------------------------------
-// Streaming service: resident scenes and tallies, HBM-footprint admission,
-// queued requests of unequal size batched into persistent launches.
+// Streaming service: resident scenes and tallies, admission by HBM footprint
+// and by estimated cost, queued requests of unequal size packed shortest
+// first into persistent launches.
 // ---------------------------------------------------------------------------
+
+// Two requests may share one persistent launch only if every option that can
+// change a tally is equal: the arithmetic form, random stream, budget rule,
+// FP64 re-solve and its margin, the transform unit and the scan form.  The
+// launch shape (rays per thread, group size) is compared too, so a request
+// runs in exactly the shape it asked for.  Per-request: the geometry word and
+// the explicit ray budget.
+static bool same_tallies(const rf_solve_opts &a, const rf_solve_opts &b)
+{
+    const auto margin = [](const rf_solve_opts &o) { return o.refine_margin > 0.0f ? o.refine_margin : 0x1p-14f; };
+    const bool fast = a.math == RF_MATH_FAST;
+    return a.math == b.math && a.rng == b.rng && a.use_bvs == b.use_bvs && a.budget == b.budget &&
+           (a.budget != RF_BUDGET_REFERENCE || a.n_samples == b.n_samples) && a.seed == b.seed &&
+           a.rays_per_thread == b.rays_per_thread && a.block_size == b.block_size && a.per_test == b.per_test &&
+           (!fast || (a.refine == b.refine && a.transform == b.transform &&
+                      (!a.refine || margin(a) == margin(b))));
+}
+
+extern "C" int32_t rf_opts_same_tallies(const rf_solve_opts *a, const rf_solve_opts *b)
+{
+    if (!a || !b) return RF_ERR_INVALID_ARG;
+    return same_tallies(*a, *b) ? 1 : 0;
+}
+
+extern "C" int32_t rf_debug_fail_solve(rf_ctx *ctx, int32_t nth)
+{
+    if (!ctx || nth < 0) return set_err(ctx, RF_ERR_INVALID_ARG, "bad failure-injection count");
+    ctx->fail_in = nth;
+    return RF_OK;
+}
+
+// Estimated ray-primitive evaluations per ray of the plain scan: one per
+// primitive outside the band stacks; per stack one quadric plus about six band
+// evaluations when it is searched (guess, check, walk), one per band when it
+// is walked.  Admission and packing weigh requests with it.
+static double scan_tests_per_ray(const rf_scene *s, const rf_solve_opts &o)
+{
+    if (o.per_test == 1) return (double)s->n;
+    double t = 0.0;
+    for (const StackInfo &k : s->stack_info) t += 1.0 + (o.per_test == 0 && k.sorted ? 6.0 : (double)k.bands);
+    return t + (double)(s->n - s->stack_prims);
+}
+
 struct StreamReq {
     rf_scene *scene = nullptr;
     rf_solve_opts opts{};
@@ -1426,6 +1487,7 @@ struct StreamReq {
     bool done = false;
     std::vector<int64_t> hits, rays;
     int64_t footprint = 0;
+    double cost = 0.0;                   // estimated ray-primitive evaluations
 };
 
 struct rf_stream {
@@ -1433,8 +1495,11 @@ struct rf_stream {
     int64_t limit = 0;
     int64_t resident = 0;
     int64_t next_id = 1;
+    double max_queued = 0.0;             // admission: queued cost limit (0 = none)
+    double max_launch = 0.0;             // packing: cost per launch (0 = one launch per option group)
+    double queued_cost = 0.0;
     std::map<int64_t, StreamReq> reqs;
-    std::vector<int64_t> queued;
+    std::vector<int64_t> queued;         // submission order
 };
 
 extern "C" int32_t rf_stream_create(rf_ctx *ctx, int64_t limit, rf_stream **out)
@@ -1447,6 +1512,18 @@ extern "C" int32_t rf_stream_create(rf_ctx *ctx, int64_t limit, rf_stream **out)
     *out = st;
     return RF_OK;
 }
+
+extern "C" int32_t rf_stream_set_cost_limits(rf_stream *st, double max_queued_tests, double max_launch_tests)
+{
+    if (!st) return RF_ERR_INVALID_ARG;
+    if (!(max_queued_tests >= 0.0) || !(max_launch_tests >= 0.0))
+        return set_err(st->ctx, RF_ERR_INVALID_ARG, "cost limits must be >= 0");
+    st->max_queued = max_queued_tests;
+    st->max_launch = max_launch_tests;
+    return RF_OK;
+}
+
+extern "C" double rf_stream_queued_cost(const rf_stream *st) { return st ? st->queued_cost : -1.0; }
 
 extern "C" int32_t rf_stream_destroy(rf_stream *st)
 {
@@ -1475,10 +1552,21 @@ extern "C" int32_t rf_stream_submit(rf_stream *st, const rf_primitive *prims, in
     StreamReq r;
     rc = rf_scene_create(ctx, prims, n, nullptr, 0, &r.scene);
     if (rc) return rc;
+    std::vector<int64_t> wi;
+    rc = emitter_budget(ctx, r.scene, o, wi);
+    double rays = 0.0;
+    for (int64_t w : wi) rays += 4.0 * (double)w;
+    r.cost = rays * scan_tests_per_ray(r.scene, *o);
+    // a request costlier than the limit on its own is admitted onto an empty
+    // queue only, so it cannot starve
+    if (!rc && st->max_queued > 0.0 && !st->queued.empty() && st->queued_cost + r.cost > st->max_queued)
+        rc = set_err(ctx, RF_ERR_CAPACITY, "admission: queued cost would exceed the stream's cost limit");
+    if (rc) { rf_scene_destroy(r.scene); return rc; }
     r.opts = *o;
     r.footprint = fp;
     r.t_submit = now_ms();
     st->resident += fp;
+    st->queued_cost += r.cost;
     const int64_t rid = st->next_id++;
     st->reqs.emplace(rid, std::move(r));
     st->queued.push_back(rid);
@@ -1486,58 +1574,91 @@ extern "C" int32_t rf_stream_submit(rf_stream *st, const rf_primitive *prims, in
     return RF_OK;
 }
 
+// One persistent launch over the requests `ids` (same tally options).
+static int stream_launch(rf_stream *st, const std::vector<int64_t> &ids)
+{
+    std::vector<rf_scene *> sc;
+    std::vector<uint32_t> gids;
+    std::vector<int64_t> grays;
+    std::vector<std::vector<int64_t>> hbuf(ids.size()), rbuf(ids.size());
+    std::vector<int64_t *> hp, rp;
+    for (size_t i = 0; i < ids.size(); ++i) {
+        StreamReq &r = st->reqs.at(ids[i]);
+        sc.push_back(r.scene);
+        gids.push_back(r.opts.geometry);
+        grays.push_back(r.opts.rays_per_emitter);
+        hbuf[i].resize((size_t)r.scene->n * r.scene->n);
+        rbuf[i].resize(r.scene->n);
+        hp.push_back(hbuf[i].data());
+        rp.push_back(rbuf[i].data());
+    }
+    rf_solve_opts o = st->reqs.at(ids[0]).opts;
+    o.form = RF_FORM_PERSISTENT;
+    const int rc = solve_impl(st->ctx, sc.data(), (int)sc.size(), &o, hp.data(), rp.data(), nullptr, gids.data(),
+                              grays.data());
+    if (rc) return rc;
+    const double t = now_ms();
+    for (size_t i = 0; i < ids.size(); ++i) {
+        StreamReq &r = st->reqs.at(ids[i]);
+        r.hits = std::move(hbuf[i]);
+        r.rays = std::move(rbuf[i]);
+        r.done = true;
+        r.t_done = t;
+        st->queued_cost -= r.cost;
+    }
+    return RF_OK;
+}
+
 extern "C" int32_t rf_stream_flush(rf_stream *st, int32_t *n_completed)
 {
     if (!st) return RF_ERR_INVALID_ARG;
-    rf_ctx *ctx = st->ctx;
     if (n_completed) *n_completed = 0;
     if (st->queued.empty()) return RF_OK;
-    // group queued requests by their solve options; one persistent launch per group
-    std::vector<int64_t> pending = st->queued;
-    st->queued.clear();
-    int done = 0;
-    while (!pending.empty()) {
-        const rf_solve_opts ref = st->reqs[pending[0]].opts;
-        std::vector<int64_t> group, rest;
-        for (int64_t rid : pending) {
-            const rf_solve_opts &q = st->reqs[rid].opts;
-            if (q.math == ref.math && q.rng == ref.rng && q.budget == ref.budget && q.n_samples == ref.n_samples &&
-                q.seed == ref.seed && q.refine == ref.refine)
-                group.push_back(rid);
-            else
-                rest.push_back(rid);
-        }
-        std::vector<rf_scene *> sc;
-        std::vector<uint32_t> gids;
-        std::vector<int64_t> grays;
-        std::vector<std::vector<int64_t>> hbuf(group.size()), rbuf(group.size());
-        std::vector<int64_t *> hp, rp;
-        for (size_t i = 0; i < group.size(); ++i) {
-            StreamReq &r = st->reqs[group[i]];
-            sc.push_back(r.scene);
-            gids.push_back(r.opts.geometry);
-            grays.push_back(r.opts.rays_per_emitter);
-            hbuf[i].resize((size_t)r.scene->n * r.scene->n);
-            rbuf[i].resize(r.scene->n);
-            hp.push_back(hbuf[i].data());
-            rp.push_back(rbuf[i].data());
-        }
-        rf_solve_opts o = ref;
-        o.form = RF_FORM_PERSISTENT;
-        int rc = solve_impl(ctx, sc.data(), (int)sc.size(), &o, hp.data(), rp.data(), nullptr, gids.data(),
-                            grays.data());
-        if (rc) return rc;
-        const double t = now_ms();
-        for (size_t i = 0; i < group.size(); ++i) {
-            StreamReq &r = st->reqs[group[i]];
-            r.hits = std::move(hbuf[i]);
-            r.rays = std::move(rbuf[i]);
-            r.done = true;
-            r.t_done = t;
-            ++done;
-        }
-        pending.swap(rest);
+    // option groups in order of their first request; inside a group the
+    // cheapest requests first, packed into launches of at most max_launch
+    // estimated evaluations (a request above it runs alone), so small
+    // requests finish in the first launches
+    std::vector<std::vector<int64_t>> groups;
+    for (int64_t rid : st->queued) {
+        const rf_solve_opts &q = st->reqs.at(rid).opts;
+        auto g = std::find_if(groups.begin(), groups.end(),
+                              [&](const std::vector<int64_t> &v) { return same_tallies(st->reqs.at(v[0]).opts, q); });
+        if (g == groups.end()) groups.push_back({rid});
+        else g->push_back(rid);
     }
+    std::vector<std::vector<int64_t>> launches;
+    for (std::vector<int64_t> &g : groups) {
+        std::stable_sort(g.begin(), g.end(),
+                         [&](int64_t a, int64_t b) { return st->reqs.at(a).cost < st->reqs.at(b).cost; });
+        std::vector<int64_t> cur;
+        double c = 0.0;
+        for (int64_t rid : g) {
+            const double rc = st->reqs.at(rid).cost;
+            if (!cur.empty() && st->max_launch > 0.0 && c + rc > st->max_launch) {
+                launches.push_back(std::move(cur));
+                cur.clear();
+                c = 0.0;
+            }
+            cur.push_back(rid);
+            c += rc;
+        }
+        if (!cur.empty()) launches.push_back(std::move(cur));
+    }
+    int done = 0;
+    for (const std::vector<int64_t> &l : launches) {
+        const int rc = stream_launch(st, l);
+        if (rc) {
+            // the requests not completed stay queued, in submission order
+            std::vector<int64_t> left;
+            for (int64_t rid : st->queued)
+                if (!st->reqs.at(rid).done) left.push_back(rid);
+            st->queued.swap(left);
+            if (n_completed) *n_completed = done;
+            return rc;
+        }
+        done += (int)l.size();
+    }
+    st->queued.clear();
     if (n_completed) *n_completed = done;
     return RF_OK;
 }

@@ -302,6 +302,15 @@ class Stream:
     def resident_bytes(self) -> int:
         return lib().rf_stream_resident_bytes(self.handle)
 
+    def set_cost_limits(self, max_queued_tests: float = 0.0, max_launch_tests: float = 0.0) -> None:
+        """Admission limit on the queued cost and packing limit per launch, in
+        estimated ray-primitive evaluations (0 = none)."""
+        check(lib().rf_stream_set_cost_limits(self.handle, float(max_queued_tests), float(max_launch_tests)),
+              "rf_stream_set_cost_limits", self.ctx.handle)
+
+    def queued_cost(self) -> float:
+        return lib().rf_stream_queued_cost(self.handle)
+
     def close(self) -> None:
         if self.handle and self.ctx.handle:
             lib().rf_stream_destroy(self.handle)
@@ -312,6 +321,20 @@ class Stream:
             self.close()
         except Exception:
             pass
+
+
+def same_tallies(a: SolveOpts, b: SolveOpts) -> bool:
+    """Whether two solves may share one persistent launch (every option that can
+    change a tally is equal); host only."""
+    r = lib().rf_opts_same_tallies(byref(a), byref(b))
+    if r < 0:
+        raise L.RayFactorError(r, "rf_opts_same_tallies")
+    return bool(r)
+
+
+def fail_solve(ctx: Context, nth: int) -> None:
+    """Test hook: the nth solve issued on ctx from now fails (0 cancels)."""
+    check(lib().rf_debug_fail_solve(ctx.handle, int(nth)), "rf_debug_fail_solve", ctx.handle)
 
 
 def scene_footprint(n_prims: int, n_bvs: int = 0) -> int:

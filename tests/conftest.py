@@ -51,3 +51,14 @@ def batch_mean_sigma(hits_per_block: np.ndarray, rays_per_block: int) -> float:
     """Standard error of a VF from per-block estimates (blocks of whole work items)."""
     est = hits_per_block / rays_per_block
     return float(est.std(ddof=1) / np.sqrt(len(est)))
+
+
+def oracle_trace_mt(O, prims, bvs, emitter: int, n_wi: int, chunks: int = 64, threads: int = 16, **kw):
+    """Per-ray oracle trace of work items [0, n_wi) of one emitter, split over
+    host threads (the ctypes call releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    step = -(-n_wi // chunks)
+    spans = [(w, min(step, n_wi - w)) for w in range(0, n_wi, step)]
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        parts = list(ex.map(lambda s: O.trace(prims, bvs, emitter, s[0], s[1], **kw), spans))
+    return np.concatenate(parts)

@@ -95,21 +95,6 @@ def test_launch_emitter_accumulates(ctx, oracle, golden_scene):
     assert np.array_equal(h[2], row2) and np.array_equal(h[0], row0) and h[[1, 3, 4]].sum() == 0
 
 
-def test_config1_closed_forms_at_scale(ctx, golden_scene):
-    sc = golden_scene("config1")
-    dev = rf.DeviceScene(ctx, sc.prims, None)
-    rays = 10_000_000
-    r = rf.solve(ctx, dev, math="fast", rays_per_emitter=rays, rng="seeded", form="persistent")
-    F = r.vf
-    sig = lambda f: np.sqrt(f * (1 - f) / rays) * 2.0   # 4-ray work items share a block: x2 margin
-    for (i, j, exact, allow) in [(1, 2, 0.1715729, 0), (1, 0, 0.8284271, 1.5e-3), (0, 1, 0.2071068, 1.5e-3),
-                                 (0, 0, 0.5857864, 1.5e-3), (3, 4, 0.3819660, 0)]:
-        assert abs(F[i, j] - exact) <= 4 * sig(exact) + allow, (i, j, F[i, j])
-    assert np.all(F[:3].sum(axis=1) > 0.998)
-    # reciprocity A_i F_ij = A_j F_ji (wall area 4 pi, discs pi)
-    assert 4 * F[0, 1] == pytest.approx(F[1, 0], rel=2e-3)
-
-
 @pytest.mark.parametrize("form", ["per_emitter", "reference"])
 def test_launch_emitter_fan_out_and_implicit_join(ctx, form):
     """Per-emitter launches of the queue kernel run on several streams; reading

@@ -43,6 +43,80 @@ def test_stream_admission(ctx):
     st.close()
 
 
+def test_stream_mixed_options_each_equal_their_own_solve(ctx):
+    """Requests that differ only in the FP64 re-solve margin or the transform
+    unit are never solved with another request's options."""
+    sc = scenes.parse_xml(scenes.soup_xml(300, seed=11))
+    st = rf.Stream(ctx, 1 << 30)
+    variants = [dict(), dict(refine_margin=0.25), dict(transform="mfma"), dict(refine=False),
+                dict(refine_margin=0.25, transform="mfma"), dict(seed=6), dict(math="fp64")]
+    base = dict(math="fast", rng="seeded", rays_per_emitter=1 << 16, seed=5)
+    ids = [st.submit(sc.prims, rf.make_opts(**{**base, **v})) for v in variants]
+    assert st.flush() == len(variants)
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    got = [st.collect(rid)[0] for rid in ids]
+    for v, h in zip(variants, got):
+        assert np.array_equal(h, rf.solve(ctx, dev, **{**base, **v}).hits), v
+    # another seed or arithmetic form certainly changes the tallies; on 2e7 soup
+    # rays so does a margin, transform or re-solve change (a few rays each)
+    assert not np.array_equal(got[0], got[5]) and not np.array_equal(got[0], got[6])
+    assert not np.array_equal(got[0], got[2]) or not np.array_equal(got[0], got[3])
+    print(f"[stream] tallies differ from the default: margin {not np.array_equal(got[0], got[1])}, "
+          f"mfma {not np.array_equal(got[0], got[2])}, unrefined {not np.array_equal(got[0], got[3])}", flush=True)
+    st.close()
+
+
+def test_stream_failed_flush_keeps_requests(ctx):
+    """A flush whose launch fails leaves every uncompleted request queued and
+    its footprint accounted; the next flush completes them all."""
+    st = rf.Stream(ctx, 1 << 30)
+    o1 = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=4000, seed=1)
+    o2 = rf.make_opts(math="compat", rng="seeded", rays_per_emitter=4000, seed=1)
+    scs = [scenes.parse_xml(scenes.soup_xml(n, seed=20 + n)) for n in (20, 40, 60)]
+    ids = [st.submit(scs[0].prims, o1), st.submit(scs[1].prims, o2), st.submit(scs[2].prims, o1)]
+    resident = st.resident_bytes()
+    rf.fail_solve(ctx, 2)                       # the first launch runs, the second fails
+    with pytest.raises(rf.RayFactorError) as e:
+        st.flush()
+    assert e.value.code == _lib.RF_ERR_HIP and "injected" in str(e.value)
+    assert st.resident_bytes() == resident
+    assert st.queued_cost() > 0
+    assert st.flush() == 1                      # the request left over from the failed launch
+    assert st.queued_cost() == 0
+    for rid, sc, o in zip(ids, scs, (o1, o2, o1)):
+        h, _, _ = st.collect(rid)
+        dev = rf.DeviceScene(ctx, sc.prims, None)
+        assert np.array_equal(h, rf.solve(ctx, dev, o).hits)
+    assert st.resident_bytes() == 0
+    st.close()
+
+
+def test_stream_cost_admission_and_packing(ctx):
+    """Cost admission rejects a submit past the queued-cost limit; packing
+    runs the cheapest requests first, so they complete no later than the
+    costly one."""
+    sc_small = scenes.parse_xml(scenes.soup_xml(20, seed=1))
+    sc_big = scenes.parse_xml(scenes.soup_xml(300, seed=2))
+    o_small = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=4000)
+    o_big = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=40000)
+    small_cost = 4000 * 20 * 20
+    big_cost = 40000 * 300 * 300
+    st = rf.Stream(ctx, 1 << 30)
+    st.set_cost_limits(max_queued_tests=big_cost + 3 * small_cost, max_launch_tests=4 * small_cost)
+    big = st.submit(sc_big.prims, o_big)
+    assert st.queued_cost() == pytest.approx(big_cost)
+    small = [st.submit(sc_small.prims, o_small) for _ in range(3)]
+    with pytest.raises(rf.RayFactorError) as e:
+        st.submit(sc_small.prims, o_small)
+    assert e.value.code == _lib.RF_ERR_CAPACITY
+    assert st.flush() == 4
+    lat_big = st.collect(big)[2]
+    lat_small = [st.collect(r)[2] for r in small]
+    assert max(lat_small) <= lat_big
+    st.submit(sc_small.prims, o_small)          # admitted again once the queue drained
+    st.close()
+
+
 def test_autotune_keeps_results(ctx):
     """Tuning picks a launch shape and a transform; the VALU kernel's tallies
     do not depend on the shape, and AUTO afterwards equals the chosen
@@ -64,9 +138,9 @@ def test_autotune_keeps_results(ctx):
 def test_configs_report_quick(ctx):
     """The per-configuration report runs end to end (reduced budgets)."""
     from rayfactorcl_amd import configs
-    r1 = configs.config1(ctx, quick=True)
-    assert all(c["within_4sigma_plus_edge"] for c in r1["fast"]["closed_forms"])
-    assert all(c["within_4sigma_plus_edge"] for c in r1["compat"]["closed_forms"])
+    r1 = configs.config1(ctx, quick=True, cpu_seconds=0)
+    assert all(c["within_3se"] for c in r1["fast"]["closed_forms"])
+    assert all(c["within_3se"] for c in r1["compat"]["closed_forms"])
     assert r1["fast_vs_fp64_max_dvf"] <= 1e-4 * 10     # 1e6 rays: binomial noise of differing rays
     r5 = configs.config5(ctx, quick=True, n_req=6)
     assert r5["requests"] == 6 and r5["latency_p99_s"] >= r5["latency_p50_s"] > 0
@@ -85,9 +159,14 @@ def test_closure_reciprocity_report(ctx):
     dev.close()
     c = rep["closure"]
     # rays leaving within EPS of a seam meet the neighbouring surface at
-    # t <= EPS and are not counted (the reference's self-hit guard): an edge
-    # loss of a few 1e-4, bounded as the closed-form checks bound it
-    assert np.all(c[:3] <= 1.0) and np.all(c[:3] >= 1.0 - 1.5e-3)
+    # t <= EPS and are not counted (the reference's self-hit guard): each row
+    # of the enclosure closes to 1 less its rims' seam losses.  The standard
+    # error of a miss fraction m is at most 2 sqrt(m / rays): the 4 rays of a
+    # work item share a random block, which at worst doubles a binomial sigma
+    from closed_forms import REMAINDER, seam_loss
+    lost = np.array([2 * seam_loss(2 * np.pi, 4 * np.pi, np.pi / 2), seam_loss(2 * np.pi, np.pi, np.pi / 2),
+                     seam_loss(2 * np.pi, np.pi, np.pi / 2)])
+    assert np.all(np.abs(c[:3] - (1.0 - lost)) <= 3 * 2 * np.sqrt(lost / rays) + REMAINDER), (c[:3], 1 - lost)
     f_pp = 0.5 * (3.0 - 5.0 ** 0.5)
     sig = 2.0 * (f_pp * (1 - f_pp) / rays) ** 0.5
     # the pair also sees the enclosure's outside at grazing angles (< 1e-2)

@@ -32,7 +32,7 @@ def test_struct_layouts():
     assert rf.PRIM_DTYPE.itemsize == 112 and rf.BV_DTYPE.itemsize == 72
     assert rf.PRIM_DTYPE.fields["type"][1] == 96 and rf.PRIM_DTYPE.fields["util2"][1] == 108
     assert rf.BV_DTYPE.fields["objEndIndex"][1] == 60 and rf.BV_DTYPE.fields["util2"][1] == 68
-    assert rf.lib().rf_abi_version() == 2
+    assert rf.lib().rf_abi_version() == 3
 
 
 # work items per emitter of the sample scenes at globalRayDensity = 1e6
@@ -165,3 +165,25 @@ def test_solve_opts_defaults():
     rf.lib().rf_solve_opts_default(ctypes.byref(o))
     assert (o.math, o.rng, o.use_bvs, o.budget, o.n_samples, o.form, o.refine) == (0, 0, 0, 0, 1000000, 1, 1)
     assert (o.rays_per_thread, o.block_size, o.transform, o.refine_margin, o.per_test) == (0, 0, 0, 0.0, 0)
+
+
+def test_same_tallies_grouping_key():
+    """Streaming flushes share a launch only between requests whose options
+    give the same tallies (host-only check of the grouping key)."""
+    from rayfactorcl_amd.solver import same_tallies
+    base = dict(math="fast", rng="seeded", rays_per_emitter=1000, seed=3)
+    a = rf.make_opts(**base)
+    assert same_tallies(a, rf.make_opts(**base))
+    # per-request fields: geometry word, explicit budget, dispatch form
+    assert same_tallies(a, rf.make_opts(**{**base, "geometry": 7, "rays_per_emitter": 5000, "form": "per_emitter"}))
+    # every option that changes a tally (or the launch shape) separates
+    for kw in (dict(refine_margin=0.01), dict(transform="mfma"), dict(refine=False), dict(seed=4),
+               dict(math="fp64"), dict(rng="reference"), dict(per_test=1), dict(rays_per_thread=4),
+               dict(block_size=128)):
+        assert not same_tallies(a, rf.make_opts(**{**base, **kw})), kw
+    # the default margin is 2^-14 whether given as 0 or explicitly
+    assert same_tallies(a, rf.make_opts(**{**base, "refine_margin": 2.0 ** -14}))
+    # margins and transforms do not matter outside the fast form
+    c = dict(math="compat", n_samples=1000)
+    assert same_tallies(rf.make_opts(**c), rf.make_opts(**c, refine_margin=0.01, transform="mfma"))
+    assert not same_tallies(rf.make_opts(**c), rf.make_opts(math="compat", n_samples=2000))
