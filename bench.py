@@ -1,15 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark: ray-primitive tests/s and rays/s of the view-factor engine on one MI355X.
+"""Benchmark: rays/s and ray-primitive tests/s of the view-factor engine on one MI355X.
 
 Workload (default): the batched design sweep -- 256 coaxial furnace
 geometries (34-557 primitives each) with 1e6 rays per emitter, solved in the
 FP32 + FP64-refined form with band stacks (sorted stacks searched) by one
 persistent launch per step.  A step is one full solve of all 256 geometries
-(every emitter, every ray).  Beside it, on a sample of the geometries: the
-same math in the plain one-launch-per-emitter forms, the band stacks walked
-band by band, the per-test form (band stacks off), and the fast form's
-view-factor deviation from all-FP64; the CPU oracle runs a bounded sample on
-the host cores.
+(every emitter, every ray).
+
+Headline: rays/s over the timed steps.  Beside it:
+- executed ray-primitive evaluations per second: what the kernel runs (one per
+  primitive tested in full; per band stack one quadric per ray plus one z-range
+  check per band it looks at), counted on the device in one extra launch of the
+  same solve -- the timed launches run without the counters, which cost ~3%,
+  and every launch of the solve does the same work;
+- brute-force-equivalent tests/s: rays x primitives per second, the tests the
+  reference's runGPU scan would run for the same rays;
+- the FP32 roofline fraction from the counted launch's algorithmic flops over
+  the timed launches' average duration.
+On a sample of the geometries: the plain one-launch-per-emitter forms, one
+persistent launch per geometry (also at smaller ray budgets, where batching
+geometries pays), the band stacks walked and every test in full, and the fast
+form's view-factor deviation from all-FP64.  The CPU oracle runs a bounded
+sample on the host cores, beside the GPU running the same algorithm.
 
 Prints ONE JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
@@ -68,10 +80,39 @@ def cpu_model() -> str:
     return "unknown CPU"
 
 
-def cpu_baseline(geoms, seconds: float, threads: int):
-    """Oracle (compat FP32 restatement of the reference kernel) on host cores,
-    a bounded sample: every emitter of the median-size geometry, ray budget
-    scaled (two calibration passes) so the timed run takes about `seconds`."""
+def kernel_name(t: dict, refine: bool) -> str:
+    """The persistent kernel the library launched, from the reported shape."""
+    from rayfactorcl_amd import _lib as L
+    f = t["shape_flags"]
+    r = str(refine).lower()
+    if f & L.RF_SHAPE_MFMA:
+        return f"rf::k_queue_mfma<{r}, false>"
+    nr = t["shape_rays_per_thread"]
+    if nr <= 2 and f & L.RF_SHAPE_SEARCHED:
+        return (f"rf::k_queue<2, false, false, {nr}, {r}, {str(bool(f & L.RF_SHAPE_COUNTED)).lower()}, "
+                f"{str(bool(f & L.RF_SHAPE_BAND_TILE)).lower()}>")
+    return f"rf::k_queue<2, false, false, {nr}, {r}>"
+
+
+def cylinder_form(t: dict) -> str:
+    from rayfactorcl_amd import _lib as L
+    f = t["shape_flags"]
+    if not f & L.RF_SHAPE_STACKS:
+        return "every test in full"
+    if f & L.RF_SHAPE_SEARCHED:
+        return ("band stacks: one quadric per ray and stack; sorted stacks searched for the bands a root can lie on"
+                + (", band records in LDS" if f & L.RF_SHAPE_BAND_TILE else ", band records read from global memory"))
+    return "band stacks: one quadric per ray and stack, a z-range test per band"
+
+
+def cpu_baseline(ctx, geoms, seconds: float, threads: int, seed: int):
+    """Oracle (compat FP32 restatement of the reference kernel, brute force
+    over every primitive) on host cores, a bounded sample: every emitter of the
+    median-size geometry, ray budget scaled (two calibration passes) so the
+    timed run takes about `seconds`.  Beside it the GPU runs the same
+    algorithm (compat arithmetic, every test in full) on the same geometry, and
+    the library's default compat form (band stacks searched)."""
+    import rayfactorcl_amd as rf
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     sizes = [g.n for g in geoms]
@@ -80,20 +121,33 @@ def cpu_baseline(geoms, seconds: float, threads: int):
     n_wi = 16
     for budget in (0.2, 2.0):                   # calibration passes (seconds)
         t0 = time.perf_counter()
-        O.solve(g.prims, None, n_wi=n_wi, mode="compat", rng="seeded", threads=threads)
+        O.solve(g.prims, None, n_wi=n_wi, mode="compat", rng="seeded", seed=seed, threads=threads)
         dt = max(time.perf_counter() - t0, 1e-6)
         n_wi = max(16, int(n_wi * budget / dt))
     n_wi = max(16, int(n_wi * seconds / 2.0))
     t0 = time.perf_counter()
-    hits, rays = O.solve(g.prims, None, n_wi=n_wi, mode="compat", rng="seeded", threads=threads)
+    hits, rays = O.solve(g.prims, None, n_wi=n_wi, mode="compat", rng="seeded", seed=seed, threads=threads)
     dt = time.perf_counter() - t0
     total_rays = int(rays.sum())
-    tests = total_rays * n
+    # the GPU on the same geometry and arithmetic, at 2^20 rays per emitter
+    d = rf.DeviceScene(ctx, g.prims, None)
+    gpu = {}
+    for name, pt in (("per_test", 1), ("searched", 0)):
+        kw = dict(math="compat", rng="seeded", seed=seed, rays_per_emitter=1 << 20, form="persistent", per_test=pt)
+        rf.solve(ctx, d, **kw)
+        t = rf.solve(ctx, d, **kw).timing
+        gpu[name] = t["rays"] / (t["kernel_ms"] * 1e-3)
+    d.close()
+    cpu_rays = total_rays / dt
     return {
-        "value": tests / dt, "unit": "tests/s", "rays_per_s": total_rays / dt, "cores": threads, "kind": "port",
-        "sample": f"compat FP32 oracle, all {n} emitters of one {n}-primitive sweep geometry, "
-                  f"{4 * n_wi} rays/emitter ({tests:.3e} tests, {dt:.1f} s) on {threads} threads of "
+        "value": cpu_rays, "unit": "rays/s", "tests_per_s": total_rays * n / dt, "cores": threads, "kind": "port",
+        "sample": f"compat FP32 oracle (brute force), all {n} emitters of one {n}-primitive sweep geometry, "
+                  f"{4 * n_wi} rays/emitter ({total_rays * n:.3e} tests, {dt:.1f} s) on {threads} threads of "
                   f"{os.cpu_count()} x {cpu_model()}",
+        "gpu_same_algorithm_rays_per_s": gpu["per_test"],
+        "gpu_same_algorithm_over_cpu": gpu["per_test"] / cpu_rays,
+        "gpu_compat_searched_rays_per_s": gpu["searched"],
+        "gpu_compat_searched_over_cpu": gpu["searched"] / cpu_rays,
     }
 
 
@@ -111,8 +165,17 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
+def timed(rf, ctx, scenes, opts):
+    """One warm solve, then one timed (wall clock around the call)."""
+    rf.batch_solve(ctx, scenes, opts, readback=False)
+    s0 = time.perf_counter()
+    _, t = rf.batch_solve(ctx, scenes, opts, readback=False)
+    ctx.sync()
+    return t, time.perf_counter() - s0
+
+
 def main() -> int:
-    ap = argparse.ArgumentParser(description=__doc__)
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
@@ -121,6 +184,8 @@ def main() -> int:
     ap.add_argument("--rays", type=float, default=1e6, help="rays per emitter")
     ap.add_argument("--seed", type=int, default=0x52465331)
     ap.add_argument("--plain-geoms", type=int, default=16, help="geometries in the plain-form comparison")
+    ap.add_argument("--budgets", default="1e4,1e5,1e6",
+                    help="ray budgets per emitter of the batched vs one-launch-per-geometry comparison")
     ap.add_argument("--precision-geoms", type=int, default=4,
                     help="geometries of the fast-vs-FP64 view-factor check (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -130,6 +195,8 @@ def main() -> int:
     ap.add_argument("--rays-per-thread", type=int, default=0,
                     help="fixed launch shape (with --block-size) instead of the autotuner's")
     ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--band-tile", type=int, default=0,
+                    help="band records in LDS for the band search: 0 = autotuned / default, -1 = off, n = at most n")
     ap.add_argument("--no-refine", action="store_true")
     ap.add_argument("--cylinder-tests", default="search", choices=list(CYL_FORMS),
                     help="search: band stacks, sorted ones searched; walk: band stacks, every band's z range "
@@ -158,7 +225,6 @@ def main() -> int:
     dev = [rf.DeviceScene(ctx, g.prims, None) for g in geoms]
     rays = int(args.rays)
     refine = not args.no_refine
-    rpt, blk = 4, 256          # library defaults (VALU transform)
     if args.rays_per_thread:
         rpt, blk = args.rays_per_thread, args.block_size or 256
     elif not args.no_tune:
@@ -167,19 +233,17 @@ def main() -> int:
                                                       refine=refine, per_test=per_test))
         log(f"autotune: {'MFMA transform' if rpt == 0 else f'{rpt} rays/thread'}, {blk} threads/group "
             f"({time.perf_counter() - t0:.1f} s)")
-    kernel = (f"rf::k_queue_mfma<{str(refine).lower()}, false>" if rpt == 0 else
-              f"rf::k_queue<2, false, false, {rpt}, {str(refine).lower()}>")
-    opts = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", refine=refine,
-                        seed=args.seed, per_test=per_test,
-                        rays_per_thread=rpt if args.rays_per_thread else 0,
-                        block_size=blk if args.rays_per_thread else 0)
+    base = dict(math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", refine=refine, seed=args.seed,
+                per_test=per_test, rays_per_thread=args.rays_per_thread, block_size=args.block_size if
+                args.rays_per_thread else 0, band_tile=args.band_tile)
+    opts = rf.make_opts(count_tests=False, **base)
     for i in range(args.warmup):
         _, t = rf.batch_solve(ctx, dev, opts, readback=False)
         log(f"warmup {i}: {t['kernel_ms']:.1f} ms kernel")
     ctx.sync()
     t_all = []
     kernel_ms = 0.0
-    tests = flops = nrays = refined = launches = 0
+    nrays = refined = launches = brute = 0
     wall0 = time.perf_counter()
     for i in range(args.steps):
         s0 = time.perf_counter()
@@ -187,73 +251,87 @@ def main() -> int:
         ctx.sync()
         t_all.append(time.perf_counter() - s0)
         kernel_ms += t["kernel_ms"]
-        tests += t["tests"]; flops += t["flops"]; nrays += t["rays"]; refined += t["refined"]
-        launches += t["launches"]
+        nrays += t["rays"]; refined += t["refined"]; launches += t["launches"]; brute += t["tests_brute_force"]
         log(f"step {i}: {t_all[-1] * 1e3:.1f} ms wall, {t['kernel_ms']:.1f} ms kernel, "
-            f"{t['tests'] / (t['kernel_ms'] * 1e-3):.3e} tests/s")
+            f"{t['rays'] / (t['kernel_ms'] * 1e-3):.3e} rays/s")
     wall = time.perf_counter() - wall0
-    value = tests / wall
-    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    shape = t
+    kernel = kernel_name(shape, refine)
+    # one counted launch of the same solve: executed evaluations and flops per launch
+    _, tc = rf.batch_solve(ctx, dev, rf.make_opts(count_tests=True, **base), readback=False)
+    evals_per_launch = tc["tests"] / max(tc["launches"], 1)
+    flops_per_launch = tc["flops"] / max(tc["launches"], 1)
+    avg_launch_ms = kernel_ms / max(launches, 1)
+    value = nrays / wall
+    achieved = flops_per_launch / (avg_launch_ms * 1e-3) / 1e12
+    log(f"{value:.3e} rays/s; {evals_per_launch * launches / wall:.3e} executed evaluations/s; "
+        f"{brute / wall:.3e} brute-force-equivalent tests/s; {achieved:.1f} TFLOP/s "
+        f"({achieved / PEAK_FP32_TFLOPS:.1%} of FP32 peak) [{kernel}; counted launch {tc['kernel_ms']:.1f} ms]")
 
-    # plain one-geometry-per-launch form on a sample of the geometries (same math and streams)
+    # the same math in other dispatch and scan forms, on a sample of the geometries
     plain = None
     if args.plain_geoms > 0:
         idx = np.linspace(0, len(dev) - 1, min(args.plain_geoms, len(dev))).astype(int)
         sub = [dev[i] for i in idx]
-        ob = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", refine=refine,
-                          seed=args.seed, per_test=per_test)
-        op = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="per_emitter", refine=refine,
-                          seed=args.seed, per_test=per_test)
-        orf = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="reference", refine=refine,
-                           seed=args.seed, per_test=per_test)
+        sub_g = [geoms[i] for i in idx]
+        kw = dict(base, count_tests=False)
         res = {}
-        for name, o in [("batched", ob), ("per_emitter", op), ("reference_form", orf)]:
-            rf.batch_solve(ctx, sub, o, readback=False)
-            s0 = time.perf_counter()
-            _, t = rf.batch_solve(ctx, sub, o, readback=False)
-            dt = time.perf_counter() - s0
-            res[name] = {"tests_per_s": t["tests"] / dt, "kernel_ms": t["kernel_ms"], "wall_ms": dt * 1e3,
+        for name, form in [("batched", "persistent"), ("per_emitter", "per_emitter"), ("reference_form", "reference")]:
+            t, dt = timed(rf, ctx, sub, rf.make_opts(**{**kw, "form": form}))
+            res[name] = {"rays_per_s": t["rays"] / dt, "kernel_ms": t["kernel_ms"], "wall_ms": dt * 1e3,
                          "launches": t["launches"]}
-            log(f"sample {name}: {res[name]['tests_per_s']:.3e} tests/s, {t['launches']} launches")
-        # one persistent solve per geometry, one after the other
-        tests = launches_g = 0
-        s0 = time.perf_counter()
-        for i, d in enumerate(sub):
-            og = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", refine=refine,
-                              seed=args.seed, geometry=i, per_test=per_test)
-            _, t = rf.batch_solve(ctx, [d], og, readback=False)
-            tests += t["tests"]; launches_g += t["launches"]
-        dt = time.perf_counter() - s0
-        res["per_geometry"] = {"tests_per_s": tests / dt, "wall_ms": dt * 1e3, "launches": launches_g}
-        log(f"sample per_geometry: {res['per_geometry']['tests_per_s']:.3e} tests/s, {launches_g} launches")
-        # the same batched solve with the band stacks walked band by band, and
-        # with every cylinder test in full (no band stacks), each in its best
-        # shape (4 rays per thread, 256-thread groups)
-        for name, pt in [("walk", 2), ("per_test", 1)]:
-            opt = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", refine=refine,
-                               seed=args.seed, per_test=pt, rays_per_thread=4, block_size=256, transform="valu")
-            rf.batch_solve(ctx, sub, opt, readback=False)
+            log(f"sample {name}: {res[name]['rays_per_s']:.3e} rays/s, {t['launches']} launches")
+
+        def per_geometry(r):
+            """One persistent solve per geometry, one after the other."""
+            n_r = n_l = 0
             s0 = time.perf_counter()
-            _, t = rf.batch_solve(ctx, sub, opt, readback=False)
-            dt = time.perf_counter() - s0
-            res[name] = {"tests_per_s": t["tests"] / dt, "kernel_ms": t["kernel_ms"],
+            for i, d in enumerate(sub):
+                o = rf.make_opts(**{**kw, "rays_per_emitter": r, "geometry": i})
+                _, t = rf.batch_solve(ctx, [d], o, readback=False)
+                n_r += t["rays"]; n_l += t["launches"]
+            ctx.sync()
+            return n_r / (time.perf_counter() - s0), n_l
+
+        # batched vs one persistent launch per geometry, at several budgets
+        budgets = []
+        for r in [int(float(x)) for x in args.budgets.split(",") if x]:
+            per_geometry(r)                                           # warm (graphs are per solve)
+            pg, nl = per_geometry(r)
+            tb, dtb = timed(rf, ctx, sub, rf.make_opts(**{**kw, "rays_per_emitter": r}))
+            bt = tb["rays"] / dtb
+            budgets.append({"rays_per_emitter": r, "batched_rays_per_s": bt, "per_geometry_rays_per_s": pg,
+                            "batched_over_one_geometry_per_launch": bt / pg, "per_geometry_launches": nl})
+            log(f"budget {r:.0e}: batched {bt:.3e} rays/s, one launch per geometry {pg:.3e} ({bt / pg:.2f}x)")
+        res["per_geometry"] = next(b for b in budgets if b["rays_per_emitter"] == rays) if any(
+            b["rays_per_emitter"] == rays for b in budgets) else None
+        # the band stacks walked band by band, and every cylinder test in full
+        for name, pt in [("walk", 2), ("per_test", 1)]:
+            t, dt = timed(rf, ctx, sub, rf.make_opts(**{**kw, "per_test": pt, "rays_per_thread": 4, "block_size": 256,
+                                                        "transform": "valu", "count_tests": True}))
+            res[name] = {"rays_per_s": t["rays"] / dt, "kernel_ms": t["kernel_ms"],
                          "tflops": t["flops"] / (t["kernel_ms"] * 1e-3) / 1e12}
-            log(f"sample {name}: {res[name]['tests_per_s']:.3e} tests/s, {res[name]['tflops']:.1f} TFLOP/s")
+            log(f"sample {name}: {res[name]['rays_per_s']:.3e} rays/s, {res[name]['tflops']:.1f} TFLOP/s")
+        b0 = res["batched"]["rays_per_s"]
         plain = {
-            "geometries": len(sub),
-            "batched_tests_per_s": res["batched"]["tests_per_s"],
-            "plain_per_emitter_tests_per_s": res["per_emitter"]["tests_per_s"],
-            "plain_reference_form_tests_per_s": res["reference_form"]["tests_per_s"],
-            "one_geometry_per_launch_tests_per_s": res["per_geometry"]["tests_per_s"],
-            "batched_over_per_emitter": res["batched"]["tests_per_s"] / res["per_emitter"]["tests_per_s"],
-            "batched_over_reference_form": res["batched"]["tests_per_s"] / res["reference_form"]["tests_per_s"],
-            "batched_over_one_geometry_per_launch": res["batched"]["tests_per_s"] / res["per_geometry"]["tests_per_s"],
-            "walk_form_tests_per_s": res["walk"]["tests_per_s"],
+            "geometries": len(sub), "primitives": [g.n for g in sub_g],
+            "batched_rays_per_s": b0,
+            "plain_per_emitter_rays_per_s": res["per_emitter"]["rays_per_s"],
+            "plain_reference_form_rays_per_s": res["reference_form"]["rays_per_s"],
+            "batched_over_per_emitter": b0 / res["per_emitter"]["rays_per_s"],
+            "batched_over_reference_form": b0 / res["reference_form"]["rays_per_s"],
+            "batched_over_one_geometry_per_launch": (res["per_geometry"] or {}).get(
+                "batched_over_one_geometry_per_launch"),
+            "batched_vs_one_geometry_per_launch_by_budget": budgets,
+            "walk_form_rays_per_s": res["walk"]["rays_per_s"],
             "walk_form_frac": res["walk"]["tflops"] / PEAK_FP32_TFLOPS,
-            "batched_over_walk_form": res["batched"]["tests_per_s"] / res["walk"]["tests_per_s"],
-            "per_test_form_tests_per_s": res["per_test"]["tests_per_s"],
+            "batched_over_walk_form": b0 / res["walk"]["rays_per_s"],
+            "per_test_form_rays_per_s": res["per_test"]["rays_per_s"],
             "per_test_form_frac": res["per_test"]["tflops"] / PEAK_FP32_TFLOPS,
-            "batched_over_per_test_form": res["batched"]["tests_per_s"] / res["per_test"]["tests_per_s"],
+            "batched_over_per_test_form": b0 / res["per_test"]["rays_per_s"],
+            "note": ("plain forms: one launch per emitter (our queue kernel; the reference's shape, one work item "
+                     "per thread and one global atomic per ray); one persistent launch per geometry is the "
+                     "strongest plain form"),
         }
 
     # the metric's precision condition: the timed form against all-FP64 on the
@@ -289,13 +367,16 @@ def main() -> int:
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: {threads} threads, ~{args.cpu_seconds:.0f} s")
-        cpu = cpu_baseline(geoms, args.cpu_seconds, threads)
-        log(f"cpu baseline: {cpu['value']:.3e} tests/s")
+        cpu = cpu_baseline(ctx, geoms, args.cpu_seconds, threads, args.seed)
+        log(f"cpu baseline: {cpu['value']:.3e} rays/s; the GPU on the same algorithm "
+            f"{cpu['gpu_same_algorithm_over_cpu']:.0f}x")
 
+    frac = achieved / PEAK_FP32_TFLOPS
     out = {
-        "metric": "ray-primitive tests/s (batched FP32+FP64-refined form), 1 MI355X",
+        "metric": "rays/s (with executed and brute-force-equivalent ray-primitive tests/s), batched FP32+FP64-refined "
+                  "form, 1 MI355X, at <=1e-4 view-factor deviation from FP64",
         "value": value,
-        "unit": "tests/s",
+        "unit": "rays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -307,15 +388,25 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": desc, "geometries": len(geoms), "rays_per_emitter": rays,
                    "primitives_total": int(sum(g.n for g in geoms)), "form": "persistent",
-                   "cylinder_tests": CYL_FORMS[args.cylinder_tests][1],
+                   "cylinder_tests": cylinder_form(shape),
+                   "launch_shape": {"rays_per_thread": shape["shape_rays_per_thread"], "block": shape["shape_block"]},
                    "math": "fp32 fast + fp64 refine" if refine else "fp32 fast", "rng": "philox4x32-10 seeded",
                    "parallelism": "single GPU"},
-        "rays_per_s": nrays / wall,
+        "rays_per_s": value,
+        "executed_tests_per_s": evals_per_launch * launches / wall,
+        "executed_tests_per_launch": evals_per_launch,
+        "brute_force_equivalent_tests_per_s": brute / wall,
         "refined_pairs": refined,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "kernel": kernel, "avg_launch_ms": kernel_ms / max(launches, 1),
-                     "flops_per_launch": flops / max(launches, 1)},
+                     "frac": frac, "traffic": traffic,
+                     "kernel": kernel, "avg_launch_ms": avg_launch_ms, "flops_per_launch": flops_per_launch,
+                     "flops_source": "device-counted in one extra launch of the same solve (the timed launches "
+                                     "run without the counters)"},
+        "targets": {"fp32_fraction": frac, "fp32_fraction_target": 0.40,
+                    "batched_over_one_geometry_per_launch": (plain or {}).get("batched_over_one_geometry_per_launch"),
+                    "batched_over_plain_target": 3.0,
+                    "note": "both north-star targets are unmet by this default form; the per-test form's fraction "
+                            "and the budgets where batching pays are in plain_comparison"},
         "plain_comparison": plain,
         "precision": precision,
         "cpu_baseline": cpu,

@@ -131,6 +131,15 @@ typedef struct rf_solve_opts {
                                  bands a root can lie on, any other has each band's z range tested;
                                  1 = every cylinder test in full; 2 = band stacks, each band's z range
                                  tested.  Same tallies in all three, in every arithmetic form. */
+    int32_t count_tests;      /* 1 (the default): the fast form's band search counts its evaluations
+                                 and flops on the device; 0: it does not (about 3% faster), and
+                                 rf_timing.tests and .flops are -1 for a solve that searched stacks.
+                                 A launch counted once gives the counts of every identical launch. */
+    int32_t band_tile;        /* RF_MATH_FAST band search (1 or 2 rays per thread): each work unit's
+                                 band records (its geometry's cylinder segment) are copied to LDS and
+                                 searched there when every scene's segment has at most this many
+                                 records; 0 = the autotuner's choice (off until tuned; up to 8192
+                                 records when on), -1 = off.  Same tallies either way. */
 } rf_solve_opts;
 
 typedef enum rf_transform {
@@ -141,7 +150,7 @@ typedef enum rf_transform {
 
 /* Fills *opts with defaults: COMPAT math, reference RNG and budget,
  * n_samples = 1000000, persistent form (same tallies as one launch per
- * emitter, one launch for all emitters), refine = 1. */
+ * emitter, one launch for all emitters), refine = 1, count_tests = 1, band_tile = 0. */
 void rf_solve_opts_default(rf_solve_opts *opts);
 
 /* Measurements of the last solve on a context. */
@@ -149,7 +158,10 @@ typedef struct rf_timing {
     double kernel_ms;         /* HIP events around the device work, on the library stream */
     double wall_ms;           /* host steady clock around the whole call */
     int64_t rays;             /* rays traced */
-    int64_t tests;            /* ray-primitive tests executed (counted on the device for BV) */
+    int64_t tests;            /* ray-primitive evaluations executed: one per primitive tested in full;
+                                 in a band stack one per (ray, stack) for the shared quadric and one per
+                                 band whose z range is checked (counted on the device), as in the
+                                 bounding-surface forms' child scans; FP64 re-solves are not counted */
     int64_t flops;            /* algorithmic FP32 flops: sum over tests of F_type; a stacked band
                                  test counts F_band, plus F_stack per (ray, band stack) */
     int64_t refined;          /* (ray, primitive) pairs re-solved in FP64 */
@@ -157,7 +169,24 @@ typedef struct rf_timing {
     int64_t rescans;          /* lane re-scans after a full FP64 re-solve list (no flagged pair is left unsolved) */
     int64_t graph_launches;   /* launches issued as a HIP graph (persistent solves repeated with the
                                  same scenes and options: captured on the second call, replayed after) */
+    int64_t tests_brute_force; /* rays x primitives: the tests of runGPU's brute-force scan
+                                  (RayFactorGPU.cl:445-472) for the same rays */
+    int32_t shape_rays_per_thread; /* launch shape of the last launch: rays per thread (0: MFMA transform) */
+    int32_t shape_block;      /* threads per workgroup */
+    int32_t shape_flags;      /* RF_SHAPE_* bits */
+    int32_t shape_pad;
 } rf_timing;
+
+/* rf_timing.shape_flags bits. */
+enum {
+    RF_SHAPE_MFMA = 1,        /* ray transform on the matrix cores */
+    RF_SHAPE_STACKS = 2,      /* band stacks share their quadric */
+    RF_SHAPE_SEARCHED = 4,    /* sorted band stacks searched (fast form, 1-2 rays per thread) */
+    RF_SHAPE_BAND_TILE = 8,   /* band records read from an LDS copy */
+    RF_SHAPE_COUNTED = 16,    /* evaluations and flops counted (always, except an uncounted band search) */
+    RF_SHAPE_BV_BINNED = 32,  /* fast bounding-surface form with rays binned by surface */
+    RF_SHAPE_REFERENCE = 64   /* the reference dispatch shape (one work item per thread) */
+};
 
 /* ------------------------------------------------------------------ */
 /* Library / context                                                   */
@@ -247,11 +276,12 @@ int32_t rf_batch_solve(rf_ctx *ctx, rf_scene *const *scenes, int32_t n_geoms,
                        const rf_solve_opts *opts, int64_t *const *hits, int64_t *const *rays,
                        rf_timing *timing);
 
-/* Autotuner: times the FAST kernel over {VALU transform x rays per thread,
- * MFMA transform} x {block size} on the given scenes (a bounded sample of
- * their work) and stores the fastest choice in the context, where later
- * solves with rays_per_thread == 0, block_size == 0 and transform == AUTO use
- * it.  best_rpt (0 for the MFMA transform) / best_block may be NULL. */
+/* Autotuner: times the FAST kernel over {VALU transform x rays per thread
+ * (2 with and without the LDS band tile, 4, 8), MFMA transform} x {block size}
+ * on the given scenes (a bounded sample of their work) and stores the fastest
+ * choice in the context, where later solves with rays_per_thread == 0,
+ * block_size == 0, transform == AUTO and band_tile == 0 use it.  best_rpt
+ * (0 for the MFMA transform) / best_block may be NULL. */
 int32_t rf_autotune(rf_ctx *ctx, rf_scene *const *scenes, int32_t n_geoms,
                     const rf_solve_opts *opts, int32_t *best_rpt, int32_t *best_block);
 

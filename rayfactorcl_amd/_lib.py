@@ -38,6 +38,15 @@ RF_ERR_PARSE = -7
 RF_ERR_CAPACITY = -8
 RF_ERR_NO_DEVICE = -9
 
+# rf_timing.shape_flags
+RF_SHAPE_MFMA = 1
+RF_SHAPE_STACKS = 2
+RF_SHAPE_SEARCHED = 4
+RF_SHAPE_BAND_TILE = 8
+RF_SHAPE_COUNTED = 16
+RF_SHAPE_BV_BINNED = 32
+RF_SHAPE_REFERENCE = 64
+
 MATH = {"compat": 0, "fp64": 1, "fast": 2}
 RNG = {"reference": 0, "seeded": 1}
 FORM = {"per_emitter": 0, "persistent": 1, "reference": 2}
@@ -50,7 +59,7 @@ class SolveOpts(ctypes.Structure):
         ("n_samples", c_int32), ("form", c_int32), ("rays_per_emitter", c_int64),
         ("seed", c_uint32), ("geometry", c_uint32), ("refine", c_int32),
         ("rays_per_thread", c_int32), ("block_size", c_int32), ("transform", c_int32),
-        ("refine_margin", c_float), ("per_test", c_int32),
+        ("refine_margin", c_float), ("per_test", c_int32), ("count_tests", c_int32), ("band_tile", c_int32),
     ]
 
 
@@ -58,7 +67,8 @@ class Timing(ctypes.Structure):
     _fields_ = [
         ("kernel_ms", c_double), ("wall_ms", c_double), ("rays", c_int64), ("tests", c_int64),
         ("flops", c_int64), ("refined", c_int64), ("launches", c_int64), ("rescans", c_int64),
-        ("graph_launches", c_int64),
+        ("graph_launches", c_int64), ("tests_brute_force", c_int64), ("shape_rays_per_thread", c_int32),
+        ("shape_block", c_int32), ("shape_flags", c_int32), ("shape_pad", c_int32),
     ]
 
     def as_dict(self) -> dict:

@@ -79,9 +79,10 @@ struct GeomDesc {
                                         // bands[-1] is a pad record
     // binned BV form with band stacks: runs of the children's cylinder
     // segment of surface b are bvc_runs[bvc_run_off[b] .. bvc_run_off[b+1]),
-    // over bvc_fast positions; bvc_bands as bands; bv_cflops[2b + s] the
+    // over bvc_fast positions; bvc_bands as bands; bv_cflops[4b + s] the
     // children's algorithmic flops per ray, per test (s = 0) or, with band
-    // stacks (s = 1), those of the children outside the stacks
+    // stacks (s = 1), those of the children outside the stacks;
+    // bv_cflops[4b + 2 + s] the number of those children
     const CylRun *bvc_runs;
     const int32_t *bvc_run_off;
     const BandRec *bvc_bands;
@@ -112,7 +113,8 @@ struct KParams {
     unsigned int *queue_head;           // atomic unit counter (zeroed before launch)
     unsigned long long *stats;          // [0] tests executed (BV kernels), [1] FP64 re-solves,
                                         // [2] re-scans, [3] flops counted on the device (band
-                                        // stacks; the binned BV kernel's child scans)
+                                        // stacks; the binned BV kernel's child scans), [4] band-
+                                        // stack evaluations of the plain scans (kNStats entries)
     int32_t *trace_out;                 // per-ray hit index: ray 4*(wi - trace_wi0) + v
     int64_t trace_wi0;
     int32_t n_units;
@@ -135,6 +137,8 @@ struct KParams {
 
 enum KMath { KM_COMPAT = 0, KM_FP64 = 1, KM_FAST = 2 };
 
+constexpr int kNStats = 5;              // KParams::stats entries
+
 struct LaunchCfg {
     int math;        // KMath
     int use_bv;
@@ -146,6 +150,8 @@ struct LaunchCfg {
     int direct;      // 1: reference form (one work item per thread, global atomics, no queue)
     int mfma;        // 1: fast plain form with the transform on the matrix cores
     int bvbin;       // 1: fast BV form with rays binned by bounding surface
+    int count;       // fast band search: 1 = count its evaluations on the device
+    int bt;          // fast band search: 1 = band records copied to LDS per unit
     size_t lds_bytes;
 };
 

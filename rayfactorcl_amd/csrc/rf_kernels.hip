@@ -165,7 +165,7 @@ __device__ __forceinline__ int find_d(CPTR(rf_primitive) P, int start, int end, 
 // holding z(hi) (bands walked last to first, as stack_loop).  A root that is
 // absent or not past EPS becomes a NaN product, which fails every range test.
 __device__ __forceinline__ uint32_t stack_d(CPTR(rf_primitive) P, CPTR(BandRec) BR, int b, int e, int mode,
-                                            float inv, const Ray64 &w, double &tbest, int &obj)
+                                            float inv, const Ray64 &w, double &tbest, int &obj, uint64_t &ev)
 {
     const rf_primitive hs = ldc(P + b);
     const float *iv = hs.invm;
@@ -233,6 +233,7 @@ __device__ __forceinline__ uint32_t stack_d(CPTR(rf_primitive) P, CPTR(BandRec) 
         const bool vhi = root_search(phi, ph);
         const double c = vlo ? lo : (vhi ? hi : dinf());
         if (c < tbest) { tbest = c; obj = BR[min(vlo ? pl : ph, e - 1)].orig; }
+        ev += 1u + probes;
         return kFlopsStackD + probes * (kFlopsBandD - 1u);
     }
     int ilo = -1, ihi = -1;
@@ -253,18 +254,23 @@ __device__ __forceinline__ uint32_t stack_d(CPTR(rf_primitive) P, CPTR(BandRec) 
     const double c = ilo >= 0 ? lo : (ihi >= 0 ? hi : dinf());
     const int ci = ilo >= 0 ? ilo : ihi;
     if (c < tbest) { tbest = c; obj = ci; }
+    ev += (uint64_t)(1 + e - b);
     return kFlopsStackD + (uint32_t)(e - b) * kFlopsBandD;
 }
 
 // findObject in the FP64 form over the caller's range [s, e) with the compat
 // runs when `stacks`: a stack cut by the range keeps the bands inside it (two
 // or more stay a stack, and a sorted stack stays sorted); adds the stacks'
-// algorithmic flops to `sflops`.
+// algorithmic flops to `sflops` and the evaluations of the stacks -- of the
+// whole range when `count_all` -- to `ev`.
 __device__ __forceinline__ int find_d_runs(CPTR(GeomDesc) G, int s, int e, const Ray64 &ray, int stacks,
-                                           uint64_t &sflops)
+                                           uint64_t &sflops, uint64_t &ev, bool count_all = false)
 {
     CPTR(rf_primitive) P = cst(G->prims);
-    if (!stacks || s >= e) return find_d(P, s, e, ray);
+    if (!stacks || s >= e) {
+        if (count_all) ev += (uint64_t)(e > s ? e - s : 0);
+        return find_d(P, s, e, ray);
+    }
     int obj = -1;
     double tbest = dinf();
     const int n_runs = G->n_c_runs;
@@ -273,8 +279,9 @@ __device__ __forceinline__ int find_d_runs(CPTR(GeomDesc) G, int s, int e, const
         if (run.begin >= e) break;
         const int rb = max(run.begin, s), re = min(run.end, e);
         if (run.stack && re - rb >= 2) {
-            sflops += stack_d(P, cst(G->c_bands), rb, re, stacks >= 2 ? run.stack : 1, run.inv, ray, tbest, obj);
+            sflops += stack_d(P, cst(G->c_bands), rb, re, stacks >= 2 ? run.stack : 1, run.inv, ray, tbest, obj, ev);
         } else {
+            if (count_all) ev += (uint64_t)(re - rb);
             for (int w = rb; w < re; ++w) {
                 const rf_primitive p = ldc(P + w);
                 const double t = hit_d(p.type, p.invm, p.util1, p.util2, ray);
@@ -499,9 +506,12 @@ __device__ __forceinline__ void stack_c_search(const float (&m)[12], CPTR(BandRe
 // mode 1 walks every band; 2 and 3 (band offsets sorted, non-decreasing or
 // non-increasing) search them (stack_c_search).
 // Returns the stack's algorithmic flops summed over the NR rays.
+// ev gains the stack's evaluations over the NR rays: a quadric per ray and a
+// z-range check per band looked at.
 template <int NR>
 __device__ __forceinline__ uint32_t stack_c(CPTR(rf_primitive) P, CPTR(BandRec) BR, int b, int e, int mode,
-                                            float inv, const Ray32 (&ray)[NR], float (&tbest)[NR], int (&obj)[NR])
+                                            float inv, const Ray32 (&ray)[NR], float (&tbest)[NR], int (&obj)[NR],
+                                            uint32_t &ev)
 {
     const rf_primitive hs = ldc(P + b);
     float m[12];
@@ -548,6 +558,7 @@ __device__ __forceinline__ uint32_t stack_c(CPTR(rf_primitive) P, CPTR(BandRec) 
             const float c = h1 ? T1[r] : (I2[r] >= 0 ? T2[r] : finf());
             if (c < tbest[r]) { tbest[r] = c; obj[r] = BR[min(h1 ? I1[r] : I2[r], e - 1)].orig; }
         }
+        ev += NR + probes;
         return NR * kFlopsStackC + probes * ((axial ? kFlopsBandZ0Axial : kFlopsBandZ0) + 1u);
     }
     if (axial) stack_c_bands<NR, true>(m, BR, b, e, ray, P1, P2, I1, I2);
@@ -559,14 +570,17 @@ __device__ __forceinline__ uint32_t stack_c(CPTR(rf_primitive) P, CPTR(BandRec) 
         const int ci = h1 ? I1[r] : I2[r];
         if (c < tbest[r]) { tbest[r] = c; obj[r] = ci; }
     }
+    ev += NR * (uint32_t)(1 + e - b);
     return NR * (kFlopsStackC + (uint32_t)(e - b) * ((axial ? kFlopsBandZ0Axial : kFlopsBandZ0) + 2u));
 }
 
 // The compat scan of one work item's NR rays over the whole scene: band stacks
 // (G->c_runs) when `stacks` (2: sorted stacks searched), else one per-test
-// range.  Returns the stacks' algorithmic flops summed over the NR rays.
+// range.  Returns the stacks' algorithmic flops summed over the NR rays; ev
+// gains their evaluations.
 template <int NR>
-__device__ __forceinline__ uint32_t find_c_multi(CPTR(GeomDesc) G, const Ray32 (&ray)[NR], int (&obj)[NR], int stacks)
+__device__ __forceinline__ uint32_t find_c_multi(CPTR(GeomDesc) G, const Ray32 (&ray)[NR], int (&obj)[NR], int stacks,
+                                                 uint32_t &ev)
 {
     CPTR(rf_primitive) P = cst(G->prims);
     float tbest[NR];
@@ -579,7 +593,7 @@ __device__ __forceinline__ uint32_t find_c_multi(CPTR(GeomDesc) G, const Ray32 (
         if (stacks) run = ldc(cst(G->c_runs) + j);
         if (run.stack)
             sf += stack_c<NR>(P, cst(G->c_bands), run.begin, run.end, stacks >= 2 ? run.stack : 1, run.inv, ray, tbest,
-                              obj);
+                              obj, ev);
         else
             scan_c_range<NR>(P, run.begin, run.end, ray, tbest, obj);
     }
@@ -588,10 +602,14 @@ __device__ __forceinline__ uint32_t find_c_multi(CPTR(GeomDesc) G, const Ray32 (
 
 // The compat child scan of one ray over the caller's range [s, e): the
 // compat runs cut to the range when `stacks` (as find_d_runs), else per test.
-__device__ __forceinline__ int find_c_runs(CPTR(GeomDesc) G, int s, int e, const Ray32 &ray, int stacks)
+// ev gains the evaluations (one per primitive tested, the stacks' own count).
+__device__ __forceinline__ int find_c_runs(CPTR(GeomDesc) G, int s, int e, const Ray32 &ray, int stacks, uint64_t &ev)
 {
     CPTR(rf_primitive) P = cst(G->prims);
-    if (!stacks || s >= e) return find_c(P, s, e, ray);
+    if (!stacks || s >= e) {
+        ev += (uint64_t)(e > s ? e - s : 0);
+        return find_c(P, s, e, ray);
+    }
     const Ray32 R1[1] = {ray};
     float tbest[1] = {finf()};
     int obj[1] = {-1};
@@ -600,10 +618,14 @@ __device__ __forceinline__ int find_c_runs(CPTR(GeomDesc) G, int s, int e, const
         const CylRun run = ldc(cst(G->c_runs) + j);
         if (run.begin >= e) break;
         const int rb = max(run.begin, s), re = min(run.end, e);
-        if (run.stack && re - rb >= 2)
-            stack_c<1>(P, cst(G->c_bands), rb, re, stacks >= 2 ? run.stack : 1, run.inv, R1, tbest, obj);
-        else
+        if (run.stack && re - rb >= 2) {
+            uint32_t e1 = 0;
+            stack_c<1>(P, cst(G->c_bands), rb, re, stacks >= 2 ? run.stack : 1, run.inv, R1, tbest, obj, e1);
+            ev += e1;
+        } else {
             scan_c_range<1>(P, rb, re, R1, tbest, obj);
+            ev += (uint64_t)(re - rb);
+        }
     }
     return obj[0];
 }
@@ -620,9 +642,10 @@ __device__ __forceinline__ int find_bv_cd(CPTR(GeomDesc) G, const RayT &ray, uin
     CPTR(rf_bounding_volume) B = cst(G->bvs);
     const int nbv = G->n_bvs;
     uint64_t sf = 0;
+    // tests gain the child scans' evaluations
     const auto child = [&](int cs, int ce) {
-        if constexpr (MATH == KM_COMPAT) return find_c_runs(G, cs, ce, ray, stacks);
-        else return find_d_runs(G, cs, ce, ray, stacks, sf);
+        if constexpr (MATH == KM_COMPAT) return find_c_runs(G, cs, ce, ray, stacks, tests);
+        else return find_d_runs(G, cs, ce, ray, stacks, sf, tests, true);
     };
     int bvIndex = -1, objectIndex = -1;
     double tbest = dinf();
@@ -635,7 +658,6 @@ __device__ __forceinline__ int find_bv_cd(CPTR(GeomDesc) G, const RayT &ray, uin
         if (t < tbest) {
             if (b.settings == 1) {
                 objectIndex = child(b.objStartIndex, b.objEndIndex);
-                tests += (uint64_t)(b.objEndIndex - b.objStartIndex);
                 if (objectIndex != -1) { tbest = t; bvIndex = w; }
             } else {
                 tbest = t; bvIndex = w;
@@ -651,7 +673,6 @@ __device__ __forceinline__ int find_bv_cd(CPTR(GeomDesc) G, const RayT &ray, uin
         if (pending && bvIndex == bsel) {
             const int s = B[bsel].objStartIndex, e = B[bsel].objEndIndex;
             objectIndex = child(s, e);
-            tests += (uint64_t)(e - s);
             pending = false;
         }
     }
@@ -1002,6 +1023,29 @@ __device__ __forceinline__ float band_sz(const FastPrim &h, float r11, const Ray
                  : __builtin_fmaf(h.r[8], R.sx, __builtin_fmaf(h.r[9], R.sy, __builtin_fmaf(h.r[10], R.sz, r11)));
 }
 
+// Band records of a searched stack, read either through the constant address
+// space (global memory; the probes' addresses differ per lane, so these are
+// vector loads served by the caches) or from the unit's copy in LDS, whose
+// lower latency shortens the dependent guess -> check -> walk chain.
+struct BandsGlobal {
+    CPTR(BandRec) p;
+    __device__ __forceinline__ BandRec operator[](int i) const { return ldc(p + i); }
+};
+
+#define LDS_PTR(T) const __attribute__((address_space(3))) T *
+
+// LDS copy of the records of cylinder positions [base, base + n): one 64-bit
+// read per record.
+struct BandsLds {
+    LDS_PTR(uint64_t) p;
+    int base;
+    __device__ __forceinline__ BandRec operator[](int i) const
+    {
+        const uint64_t v = p[i - base];
+        return BandRec{__uint_as_float((uint32_t)v), (int32_t)(v >> 32)};
+    }
+};
+
 // Band stack whose band offsets r[11] are sorted over the positions
 // (ASC: non-decreasing, else non-increasing; all finite).  Each z the band
 // loop computes, fma(cz, t, sz(w)) with sz(w) = fma(..., r[11](w)), is then a
@@ -1012,17 +1056,17 @@ __device__ __forceinline__ float band_sz(const FastPrim &h, float r11, const Ray
 // done on the positions of the runs of root 2, root 1 and the tangent point
 // (each position once), so the first band holding each root, the flagged
 // pairs and the list overflow are the band loop's.  I1 / I2 return positions
-// (e: none).  fl gains the algorithmic flops of the probes and bands of the
-// ray slots whose bit is set in vmask.
-template <int NR, bool REFINE, bool AXIAL, bool ASC>
-__device__ __forceinline__ void stack_search(const FastPrim &h, CPTR(BandRec) BR, int b, int e, float r0,
+// (e: none).  With COUNT, ev gains the z evaluations (probes and walked
+// positions) of the ray slots whose bit is set in vmask.  BR: BandsGlobal or
+// BandsLds.
+template <int NR, bool REFINE, bool AXIAL, bool ASC, bool COUNT, class Bands>
+__device__ __forceinline__ void stack_search(const FastPrim &h, const Bands &BR, int b, int e, float r0,
                                              float inv, const Ray32 (&R)[NR], const float (&T1)[NR], const float (&T2)[NR],
                                              const float (&TM)[NR], const float (&CZ)[NR], int (&I1)[NR],
                                              int (&I2)[NR], float lo, float hi, bool wroot1, bool wtan, Pending &pd,
-                                             uint32_t &fl, uint32_t vmask)
+                                             uint32_t &ev, uint32_t vmask)
 {
     const int cnt0 = pd.stack0 >= 0 ? pd.stack0 : pd.count;
-    constexpr uint32_t zf = (AXIAL ? kFlopsBandZ0Axial : kFlopsBandZ0) + kFlopsBandRoot;
     // the band loop's per-band work on position w of ray slot i
     const auto band = [&](int i, int w, int orig, float sz) {
         const float a1 = __builtin_fabsf(__builtin_fmaf(CZ[i], T1[i], sz));
@@ -1041,7 +1085,6 @@ __device__ __forceinline__ void stack_search(const FastPrim &h, CPTR(BandRec) BR
         }
         I1[i] = (in1 && w < I1[i]) ? w : I1[i];
         I2[i] = (in2 && w < I2[i]) ? w : I2[i];
-        fl += (vmask >> i) & 1u ? 2u * kFlopsBandRoot : 0u;
         if (REFINE && f) {
             if (pd.count < PEND_K && pd.resume == NO_KEY) {
                 pd.buf[pd.count * blockDim.x + threadIdx.x] = ((uint32_t)orig << 3) | (uint32_t)(pd.slot0 + i);
@@ -1074,10 +1117,10 @@ __device__ __forceinline__ void stack_search(const FastPrim &h, CPTR(BandRec) BR
         for (int i = 0; i < NR; ++i) {
             const float k0 = __builtin_fmaf(CZ[i], root(i), band_sz<AXIAL>(h, 0.0f, R[i]));
             const int gi = band_guess(b, e, r0, inv, ASC ? -hi - k0 : hi - k0);
-            const BandRec rm = ldc(BR + max(gi - 1, b));
-            rg[i] = ldc(BR + min(gi, e - 1));
-            rn[i] = ldc(BR + min(gi + 1, e - 1));
-            fl += (vmask >> i) & 1u ? 2u * zf : 0u;
+            const BandRec rm = BR[max(gi - 1, b)];
+            rg[i] = BR[min(gi, e - 1)];
+            rn[i] = BR[min(gi + 1, e - 1)];
+            if constexpr (COUNT) ev += (vmask >> i) & 1u ? 2u : 0u;
             ok[i] = (gi >= e || pass(zof(i, rg[i].r11))) && (gi <= b || !pass(zof(i, rm.r11)));
             g[i] = gi;
             off |= !ok[i];
@@ -1085,15 +1128,15 @@ __device__ __forceinline__ void stack_search(const FastPrim &h, CPTR(BandRec) BR
         if (__builtin_amdgcn_ballot_w64(off)) {
             int bs[NR];
             band_lower_bound<NR>(b, e, [&](int w, int i) {
-                fl += (vmask >> i) & 1u ? zf : 0u;
-                return pass(zof(i, ldc(BR + w).r11));
+                if constexpr (COUNT) ev += (vmask >> i) & 1u;
+                return pass(zof(i, BR[w].r11));
             }, bs);
 #pragma unroll
             for (int i = 0; i < NR; ++i) {
                 if (!ok[i]) {
                     g[i] = bs[i];
-                    rg[i] = ldc(BR + min(bs[i], e - 1));
-                    rn[i] = ldc(BR + min(bs[i] + 1, e - 1));
+                    rg[i] = BR[min(bs[i], e - 1)];
+                    rn[i] = BR[min(bs[i] + 1, e - 1)];
                 }
             }
         }
@@ -1104,7 +1147,7 @@ __device__ __forceinline__ void stack_search(const FastPrim &h, CPTR(BandRec) BR
             bool go = w < e;
             const auto step = [&](const BandRec &br) {
                 const float sz = band_sz<AXIAL>(h, br.r11, R[i]);
-                fl += (vmask >> i) & 1u ? zf : 0u;
+                if constexpr (COUNT) ev += (vmask >> i) & 1u;
                 if (!(__builtin_fabsf(__builtin_fmaf(CZ[i], root(i), sz)) <= hi)) {
                     go = false;
                     return;
@@ -1118,7 +1161,7 @@ __device__ __forceinline__ void stack_search(const FastPrim &h, CPTR(BandRec) BR
             };
             if (go) step(rg[i]);
             if (go) step(rn[i]);
-            while (go) step(ldc(BR + w));
+            while (go) step(BR[w]);
         }
     };
     run(2);
@@ -1126,20 +1169,20 @@ __device__ __forceinline__ void stack_search(const FastPrim &h, CPTR(BandRec) BR
     if (REFINE && wtan) run(0);
 }
 
-template <int NR, bool REFINE, bool AXIAL>
-__device__ __forceinline__ void stack_search_sel(bool asc, const FastPrim &h, CPTR(BandRec) BR, int b, int e,
+template <int NR, bool REFINE, bool AXIAL, bool COUNT, class Bands>
+__device__ __forceinline__ void stack_search_sel(bool asc, const FastPrim &h, const Bands &BR, int b, int e,
                                                  float r0, float inv, const Ray32 (&R)[NR], const float (&T1)[NR],
                                                  const float (&T2)[NR], const float (&TM)[NR],
                                                  const float (&CZ)[NR], int (&I1)[NR], int (&I2)[NR], float lo,
-                                                 float hi, bool wroot1, bool wtan, Pending &pd, uint32_t &fl,
+                                                 float hi, bool wroot1, bool wtan, Pending &pd, uint32_t &ev,
                                                  uint32_t vmask)
 {
     if (asc)
-        stack_search<NR, REFINE, AXIAL, true>(h, BR, b, e, r0, inv, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1, wtan, pd, fl,
-                                              vmask);
+        stack_search<NR, REFINE, AXIAL, true, COUNT>(h, BR, b, e, r0, inv, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1, wtan,
+                                                     pd, ev, vmask);
     else
-        stack_search<NR, REFINE, AXIAL, false>(h, BR, b, e, r0, inv, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1, wtan, pd, fl,
-                                               vmask);
+        stack_search<NR, REFINE, AXIAL, false, COUNT>(h, BR, b, e, r0, inv, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1, wtan,
+                                                      pd, ev, vmask);
 }
 
 // Band stack [b, e) of the cylinder segment (CylRun.stack >= 1): the records
@@ -1169,11 +1212,16 @@ __device__ __forceinline__ void stack_search_sel(bool asc, const FastPrim &h, CP
 // fill the 192 (8) a wave has at that shape's occupancy, so those shapes walk
 // every band; the autotuner weighs the two.
 // fl gains the algorithmic flops of the band work the wave ran for the ray
-// slots whose bit is set in vmask.
-template <int NR, bool REFINE>
+// slots whose bit is set in vmask, ev the evaluations: one per (ray, stack)
+// for the shared quadric and one per band whose z range is checked (the
+// search's probes and walked positions, or every band of a walked stack).
+// Without COUNT the search's evaluations are not counted (fl and ev then
+// miss them; the caller knows).  BL: the search reads the band records from
+// this LDS copy when it is set (positions [BL.base, BL.base + n)).
+template <int NR, bool REFINE, bool COUNT = true, bool BT = false>
 __device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, int b, int e, int mode, float inv,
                                            const Ray32 (&R)[NR], float (&tb)[NR], int (&ib)[NR], Pending &pd,
-                                           uint32_t &fl, uint32_t vmask)
+                                           uint32_t &fl, uint32_t &ev, uint32_t vmask, BandsLds BL = BandsLds{})
 {
     const FastPrim hs = ldc(F + b);
     const FastPrim h = vcopy(hs);
@@ -1216,19 +1264,28 @@ __device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, i
         // positions, e = none
 #pragma unroll
         for (int i = 0; i < NR; ++i) { I1[i] = e; I2[i] = e; }
-        fl += (uint32_t)__builtin_popcount(vmask) * kFlopsStack;
+        // z evaluations of this stack: the only counter inside the search
+        uint32_t zs = 0;
         const float r0 = ldc(BR + b).r11;
-        if (axial)
-            stack_search_sel<NR, REFINE, true>(mode == 2, h, BR, b, e, r0, inv, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1,
-                                               wtan, pd, fl, vmask);
-        else
-            stack_search_sel<NR, REFINE, false>(mode == 2, h, BR, b, e, r0, inv, R, T1, T2, TM, CZ, I1, I2, lo, hi, wroot1,
-                                                wtan, pd, fl, vmask);
+        const auto search = [&](const auto &src) {
+            if (axial)
+                stack_search_sel<NR, REFINE, true, COUNT>(mode == 2, h, src, b, e, r0, inv, R, T1, T2, TM, CZ, I1, I2, lo,
+                                                          hi, wroot1, wtan, pd, zs, vmask);
+            else
+                stack_search_sel<NR, REFINE, false, COUNT>(mode == 2, h, src, b, e, r0, inv, R, T1, T2, TM, CZ, I1, I2, lo,
+                                                           hi, wroot1, wtan, pd, zs, vmask);
+        };
+        if constexpr (BT) search(BL);
+        else search(BandsGlobal{BR});
+        const uint32_t q = (uint32_t)__builtin_popcount(vmask);
+        fl += q * kFlopsStack + zs * ((axial ? kFlopsBandZ0Axial : kFlopsBandZ0) + kFlopsBandRoot);
+        ev += q + zs;
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
             const bool h1 = I1[i] < e;
             const float c = h1 ? T1[i] : (I2[i] < e ? T2[i] : finf());
-            if (c < tb[i]) { tb[i] = c; ib[i] = BR[min(h1 ? I1[i] : I2[i], e - 1)].orig; }
+            const int k = min(h1 ? I1[i] : I2[i], e - 1);
+            if (c < tb[i]) { tb[i] = c; ib[i] = BT ? BL[k].orig : BR[k].orig; }
         }
         return;
     }
@@ -1244,6 +1301,7 @@ __device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, i
     const uint32_t sf = kFlopsStack + (uint32_t)(e - b) * ((axial ? kFlopsBandZ0Axial : kFlopsBandZ0) +
                                                            (wroot1 ? 2u : 1u) * kFlopsBandRoot);
     fl += (uint32_t)__builtin_popcount(vmask) * sf;
+    ev += (uint32_t)__builtin_popcount(vmask) * (uint32_t)(1 + e - b);
 }
 
 // FP64 stage of the deferred re-solves (see Pending).  The flagged ray is
@@ -1324,11 +1382,12 @@ __device__ __forceinline__ void finish_dropped(const KParams &kp, CPTR(GeomDesc)
 
 // stacks: the cylinder segment runs as its CylRun list (band stacks share the
 // quadric; 2: sorted stacks are searched); otherwise as one per-test run.
-// fl gains the band-stack flops of the ray slots set in vmask (the rest is
-// counted on the host).
-template <int NR, bool REFINE>
+// fl and ev gain the band stacks' flops and evaluations of the ray slots set
+// in vmask (the per-test primitives are counted on the host).
+template <int NR, bool REFINE, bool COUNT = true, bool BT = false>
 __device__ __forceinline__ void scan_fast(CPTR(GeomDesc) G, const Ray32 (&R)[NR], float (&tb)[NR], int (&ib)[NR],
-                                          Pending &pd, int stacks, uint32_t &fl, uint32_t vmask)
+                                          Pending &pd, int stacks, uint32_t &fl, uint32_t &ev, uint32_t vmask,
+                                          BandsLds BL = BandsLds{})
 {
     CPTR(FastPrim) F = cst(G->fast);
     const int n_runs = stacks ? G->n_cyl_runs : 1;
@@ -1336,8 +1395,8 @@ __device__ __forceinline__ void scan_fast(CPTR(GeomDesc) G, const Ray32 (&R)[NR]
         CylRun run{G->seg_begin[SEG_CYL], G->seg_end[SEG_CYL], 0, 0};
         if (stacks) run = ldc(cst(G->cyl_runs) + j);
         if (run.stack)
-            stack_loop<NR, REFINE>(F, cst(G->bands), run.begin, run.end, stacks >= 2 ? run.stack : 1, run.inv, R,
-                                   tb, ib, pd, fl, vmask);
+            stack_loop<NR, REFINE, COUNT, BT>(F, cst(G->bands), run.begin, run.end, stacks >= 2 ? run.stack : 1,
+                                              run.inv, R, tb, ib, pd, fl, ev, vmask, BL);
         else
             seg_loop<SEG_CYL, NR, REFINE, false>(F, run.begin, run.end, R, tb, ib, pd);
     }
@@ -1457,17 +1516,20 @@ __device__ __forceinline__ void tally(const KParams &kp, CPTR(GeomDesc) G, int e
 // Unit body: the rays of work items [lo, hi) (unit-local) handled by this
 // thread, stepping by `stride` work items.
 // ---------------------------------------------------------------------------
-template <int MATH, bool BV, bool TRACE, int NR, bool REFINE, bool DIRECT>
+template <int MATH, bool BV, bool TRACE, int NR, bool REFINE, bool DIRECT, bool COUNT = true, bool BT = false>
 __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, uint32_t gid, int emitter,
                                          int64_t wi0, int64_t n_wi, int64_t first, int64_t stride,
                                          uint32_t *bins, uint32_t *pend, uint64_t &tests, uint64_t &sflops,
-                                         uint32_t &nref, uint32_t &nover)
+                                         uint64_t &sevals, uint32_t &nref, uint32_t &nover, BandsLds BL = BandsLds{})
 {
     const rf_primitive E = ldc(cst(G->prims) + emitter);
     const int64_t ray0 = (wi0 - kp.trace_wi0) * 4;   // absolute ray index base (trace launches)
     if constexpr (MATH == KM_FAST && !BV && !DIRECT) {
         // NR rays per thread: ray groups first, first+stride, ... of NR consecutive rays
         const int64_t n_rays = n_wi * 4;
+        // the unit's band-stack flops and evaluations, in 32 bits (folded into
+        // the 64-bit totals before they could wrap)
+        uint32_t fl = 0, ev = 0;
         for (int64_t base = first * NR; base < n_rays; base += stride * NR) {
             Ray32 R[NR];
             float tb[NR];
@@ -1476,10 +1538,14 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
 #pragma unroll
             for (int i = 0; i < NR; ++i) { tb[i] = finf(); ib[i] = -1; }
             Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
-            uint32_t fl = 0;
             const int64_t left = n_rays - base;
-            scan_fast<NR, REFINE>(G, R, tb, ib, pd, kp.stacks, fl, left < NR ? (1u << left) - 1u : (1u << NR) - 1u);
-            sflops += fl;
+            scan_fast<NR, REFINE, COUNT, BT>(G, R, tb, ib, pd, kp.stacks, fl, ev,
+                                             left < NR ? (1u << left) - 1u : (1u << NR) - 1u, BL);
+            if ((fl | ev) >> 30) {
+                sflops += fl;
+                sevals += ev;
+                fl = ev = 0;
+            }
             if constexpr (REFINE) {
                 const auto ray_of = [base](int k) { return base + k; };
                 if (__builtin_amdgcn_ballot_w64(pd.count > 0))
@@ -1492,6 +1558,8 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
             for (int i = 0; i < NR; ++i)
                 if (base + i < n_rays) tally<TRACE, DIRECT>(kp, G, emitter, ray0 + base + i, ib[i], bins);
         }
+        sflops += fl;
+        sevals += ev;
     } else {
         // one work item (4 rays) per thread and step
         for (int64_t wl = first; wl < n_wi; wl += stride) {
@@ -1507,7 +1575,9 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                     rotate_rands(f, v, r4);
                     R4[v] = emit_c(&E, r4);
                 }
-                sflops += find_c_multi<4>(G, R4, idx4, kp.stacks);
+                uint32_t ev = 0;
+                sflops += find_c_multi<4>(G, R4, idx4, kp.stacks, ev);
+                sevals += ev;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) tally<TRACE, DIRECT>(kp, G, emitter, ray0 + wl * 4 + v, idx4[v], bins);
                 continue;
@@ -1524,7 +1594,7 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                     const double d4[4] = {r4[0], r4[1], r4[2], r4[3]};
                     const Ray64 ray = emit_d(&E, d4);
                     idx = BV ? find_bv_cd<KM_FP64>(G, ray, tests, kp.stacks)
-                             : find_d_runs(G, 0, G->n, ray, kp.stacks, sflops);
+                             : find_d_runs(G, 0, G->n, ray, kp.stacks, sflops, sevals);
                 } else {
                     const Ray32 ray = emit_fast(&E, r4);
                     if constexpr (BV) {
@@ -1534,9 +1604,10 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                         float tb1[1] = {finf()};
                         int ib1[1] = {-1};
                         Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
-                        uint32_t fl1 = 0;
-                        scan_fast<1, REFINE>(G, R1, tb1, ib1, pd, kp.stacks, fl1, 1u);
+                        uint32_t fl1 = 0, ev1 = 0;
+                        scan_fast<1, REFINE>(G, R1, tb1, ib1, pd, kp.stacks, fl1, ev1, 1u);
                         sflops += fl1;
+                        sevals += ev1;
                         if constexpr (REFINE) {
                             const int64_t r0 = wl * 4 + v;
                             const auto ray_of = [r0](int k) { return r0 + k; };
@@ -1555,22 +1626,26 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
     }
 }
 
-// stats[0]: tests counted on the device (BV forms); stats[1]: pairs re-solved
-// in FP64; stats[2]: lane re-scans after a full re-solve list; stats[3]:
-// algorithmic flops counted on the device (band stacks of the plain scan; all
-// child scans of the binned BV kernel).
+// stats[0]: tests counted on the device (BV forms: surface tests and child
+// evaluations); stats[1]: pairs re-solved in FP64; stats[2]: lane re-scans
+// after a full re-solve list; stats[3]: algorithmic flops counted on the
+// device (band stacks of the plain scan; all child scans of the binned BV
+// kernel); stats[4]: band-stack evaluations of the plain scans (a quadric per
+// ray and stack, a z-range check per band looked at).
 __device__ __forceinline__ void flush_stats(const KParams &kp, uint64_t tests, uint32_t nref, uint32_t nover,
-                                            uint64_t flops = 0)
+                                            uint64_t flops = 0, uint64_t evals = 0)
 {
-    __shared__ unsigned long long s_stats[4];
-    if (threadIdx.x < 4) s_stats[threadIdx.x] = 0;
+    __shared__ unsigned long long s_stats[kNStats];
+    if (threadIdx.x < kNStats) s_stats[threadIdx.x] = 0;
     __syncthreads();
     if (tests) atomicAdd(&s_stats[0], (unsigned long long)tests);
     if (nref) atomicAdd(&s_stats[1], (unsigned long long)nref);
     if (nover) atomicAdd(&s_stats[2], (unsigned long long)nover);
     if (flops) atomicAdd(&s_stats[3], (unsigned long long)flops);
+    if (evals) atomicAdd(&s_stats[4], (unsigned long long)evals);
     __syncthreads();
-    if (threadIdx.x < 4 && kp.stats && s_stats[threadIdx.x]) atomicAdd(&kp.stats[threadIdx.x], s_stats[threadIdx.x]);
+    if (threadIdx.x < kNStats && kp.stats && s_stats[threadIdx.x])
+        atomicAdd(&kp.stats[threadIdx.x], s_stats[threadIdx.x]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1848,7 +1923,7 @@ __device__ __forceinline__ void run_unit_mfma(const KParams &kp, CPTR(GeomDesc) 
 template <bool REFINE, bool TRACE>
 __global__ void __launch_bounds__(256) k_queue_mfma(KParams kp)
 {
-    extern __shared__ uint32_t smem[];
+    extern __shared__ __attribute__((aligned(16))) uint32_t smem[];
     uint32_t *bins = smem;
     uint32_t *pend = smem + (TRACE ? 0 : kp.bins);
     __shared__ int s_unit;
@@ -2049,6 +2124,7 @@ __device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc)
             CPTR(int32_t) sg = cst(G->bvc_seg) + b * (N_SEG + 1);
             Pending pd{pend, 0, NO_KEY, 0, tau, eps};
             uint32_t csf = 0;     // the children's band stacks' flops (valid slots)
+            uint32_t cev = 0;     // and their evaluations
             uint32_t vmask = 0;
 #pragma unroll
             for (int i = 0; i < BNR; ++i) vmask |= valid[i] ? 1u << i : 0u;
@@ -2071,15 +2147,15 @@ __device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc)
                         int (&ib1)[2] = *reinterpret_cast<int (*)[2]>(&ib[2]);
                         pd.stack0 = pd.count;
                         stack_loop<2, REFINE>(FC, cst(G->bvc_bands), run.begin, run.end, run.stack, run.inv, R0,
-                                              tb0, ib0, pd, csf, vmask & 3u);
+                                              tb0, ib0, pd, csf, cev, vmask & 3u);
                         pd.slot0 = 2;
                         stack_loop<2, REFINE>(FC, cst(G->bvc_bands), run.begin, run.end, run.stack, run.inv, R1,
-                                              tb1, ib1, pd, csf, vmask >> 2);
+                                              tb1, ib1, pd, csf, cev, vmask >> 2);
                         pd.slot0 = 0;
                         pd.stack0 = -1;
                     } else if (run.stack)
                         stack_loop<BNR, REFINE>(FC, cst(G->bvc_bands), run.begin, run.end, 1, run.inv, R, tb, ib, pd,
-                                                csf, vmask);
+                                                csf, cev, vmask);
                     else
                         seg_loop<SEG_CYL, BNR, REFINE, false>(FC, run.begin, run.end, R, tb, ib, pd);
                 }
@@ -2096,13 +2172,14 @@ __device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc)
                 if (__builtin_amdgcn_ballot_w64(pd.resume != NO_KEY))
                     finish_dropped<BNR>(kp, G, E, gid, emitter, wi0, ray_of, FC, sg, sg + 1, tb, ib, pd, nref, nover);
             }
-            const BvMeta meta = ldc(M + b);
-            const uint32_t cfl = cst(G->bv_cflops)[2 * b + (kp.stacks ? 1 : 0)];
+            const uint32_t cfl = cst(G->bv_cflops)[4 * b + (kp.stacks ? 1 : 0)];
+            const uint32_t cnt1 = cst(G->bv_cflops)[4 * b + (kp.stacks ? 3 : 2)];
             cflops += csf;
+            tests += cev;
 #pragma unroll
             for (int i = 0; i < BNR; ++i) {
                 if (valid[i]) {
-                    tests += (uint64_t)(meta.end - meta.start);
+                    tests += cnt1;
                     cflops += cfl;
                     tally<TRACE, false>(kp, G, emitter, ray0 + pass0 + lid[i], ib[i], bins);
                 }
@@ -2120,7 +2197,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REFINE
 {
     // dynamic LDS: [bins][re-solve lists: PEND_K x blockDim][rays by bucket: blockDim x BNR x BSUB]
     //              [bucket counts, offsets, cursors, wave-padded offsets: 4 x bv_slots]
-    extern __shared__ uint32_t smem[];
+    extern __shared__ __attribute__((aligned(16))) uint32_t smem[];
     uint32_t *bins = smem;
     uint32_t *pend = smem + (TRACE ? 0 : kp.bins);
     uint32_t *ids = pend + PEND_K * blockDim.x;
@@ -2179,16 +2256,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REFINE
 // Four rays per thread with the FP64 stage fits 96 VGPRs (five waves per
 // SIMD) when asked to; left alone it takes a few more and drops to four.
 // The compat scan (four rays per visit) is held to the same five waves.
-template <int MATH, bool BV, bool TRACE, int NR, bool REFINE>
+// COUNT: the fast band search counts its evaluations (off for timed launches:
+// the counters cost ~3%, and a launch counted once gives the same counts).
+// BT: each unit's band records (its geometry's cylinder segment) are copied
+// to LDS, where the searched stacks read them.
+template <int MATH, bool BV, bool TRACE, int NR, bool REFINE, bool COUNT = true, bool BT = false>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu((NR == 4 || NR == 2 || (MATH == KM_COMPAT && !BV)) ? 5 : (MATH == KM_FP64 ? 4 : 1)))) k_queue(KParams kp)
 {
     // dynamic LDS: [bins: kp.bins x u32][pending re-solves: PEND_K x blockDim x u32]
-    extern __shared__ uint32_t smem[];
+    //              [BT: band records, 8 B each, from the next 8-byte boundary]
+    extern __shared__ __attribute__((aligned(16))) uint32_t smem[];
     uint32_t *bins = smem;
     uint32_t *pend = smem + (TRACE ? 0 : kp.bins);
+    uint64_t *sband = (uint64_t *)(smem + ((kp.bins + PEND_K * blockDim.x + 1u) & ~1u));
     __shared__ int s_unit;
-    uint64_t tests = 0, sflops = 0;
+    uint64_t tests = 0, sflops = 0, sevals = 0;
     uint32_t nref = 0, nover = 0;
     const bool lds_tally = !TRACE && kp.bins > 0;
     for (;;) {
@@ -2214,12 +2297,21 @@ __attribute__((amdgpu_waves_per_eu((NR == 4 || NR == 2 || (MATH == KM_COMPAT && 
             gid = kp.imp_geom_id;
         }
         const int n = G->n;
-        if (lds_tally) {
-            for (int i = threadIdx.x; i < n; i += blockDim.x) bins[i] = 0u;
-            __syncthreads();
+        BandsLds bl{};
+        if constexpr (BT) {
+            // the geometry's cylinder segment fits: the host sized the copy
+            // for the largest segment of the launch
+            const int cb = G->seg_begin[SEG_CYL], nc = G->seg_end[SEG_CYL] - cb;
+            CPTR(uint64_t) src = (CPTR(uint64_t))(G->bands + cb);
+            for (int i = threadIdx.x; i < nc; i += blockDim.x) sband[i] = src[i];
+            bl = BandsLds{(LDS_PTR(uint64_t))sband, cb};
         }
-        run_unit<MATH, BV, TRACE, NR, REFINE, false>(kp, G, gid, emitter, wi0, n_wi, threadIdx.x,
-                                                     blockDim.x, bins, pend, tests, sflops, nref, nover);
+        if (lds_tally)
+            for (int i = threadIdx.x; i < n; i += blockDim.x) bins[i] = 0u;
+        if (BT || lds_tally) __syncthreads();
+        run_unit<MATH, BV, TRACE, NR, REFINE, false, COUNT, BT>(kp, G, gid, emitter, wi0, n_wi, threadIdx.x,
+                                                                blockDim.x, bins, pend, tests, sflops, sevals, nref,
+                                                                nover, bl);
         if (lds_tally) {
             __syncthreads();
             unsigned long long *row = G->hits + (int64_t)emitter * n;
@@ -2230,7 +2322,7 @@ __attribute__((amdgpu_waves_per_eu((NR == 4 || NR == 2 || (MATH == KM_COMPAT && 
         }
         __syncthreads();
     }
-    flush_stats(kp, tests, nref, nover, sflops);
+    flush_stats(kp, tests, nref, nover, sflops, sevals);
 }
 
 // Reference dispatch shape (RayFactorGPU.cl:481 launched as main.cpp:172-176):
@@ -2239,15 +2331,15 @@ __attribute__((amdgpu_waves_per_eu((NR == 4 || NR == 2 || (MATH == KM_COMPAT && 
 template <int MATH, bool BV>
 __global__ void __launch_bounds__(256) k_direct(KParams kp)
 {
-    extern __shared__ uint32_t smem[];   // pending re-solves: PEND_K x blockDim x u32
+    extern __shared__ __attribute__((aligned(16))) uint32_t smem[];   // pending re-solves: PEND_K x blockDim x u32
     CPTR(GeomDesc) G = cst(kp.geoms);
     uint64_t tests = 0;
     uint32_t nref = 0, nover = 0;
     const int64_t wl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t sflops = 0;
+    uint64_t sflops = 0, sevals = 0;
     run_unit<MATH, BV, false, 1, true, true>(kp, G, kp.imp_geom_id, kp.imp_emitter, kp.imp_wi0, kp.imp_n_wi,
-                                             wl, kp.imp_n_wi + 1, nullptr, smem, tests, sflops, nref, nover);
-    flush_stats(kp, tests, nref, nover, sflops);
+                                             wl, kp.imp_n_wi + 1, nullptr, smem, tests, sflops, sevals, nref, nover);
+    flush_stats(kp, tests, nref, nover, sflops, sevals);
 }
 
 __global__ void k_fill_u32(unsigned int *p, unsigned int v, size_t n)
@@ -2330,7 +2422,23 @@ hipError_t launch_fill_u64(unsigned long long *p, size_t count, hipStream_t stre
 template <int MATH, bool BV, bool TRACE, int NR, bool REFINE>
 static hipError_t go_queue(const LaunchCfg &c, const KParams &kp, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_queue<MATH, BV, TRACE, NR, REFINE>), dim3(c.grid), dim3(c.block), c.lds_bytes, s, kp);
+    // the counter switch and the LDS band copy matter for the fast band search
+    // only (1 and 2 rays per thread)
+    if constexpr (MATH == KM_FAST && !BV && !TRACE && NR <= 2) {
+        if (c.count) {
+            if (c.bt) hipLaunchKernelGGL((k_queue<MATH, BV, TRACE, NR, REFINE, true, true>), dim3(c.grid), dim3(c.block),
+                                         c.lds_bytes, s, kp);
+            else hipLaunchKernelGGL((k_queue<MATH, BV, TRACE, NR, REFINE, true, false>), dim3(c.grid), dim3(c.block),
+                                    c.lds_bytes, s, kp);
+        } else {
+            if (c.bt) hipLaunchKernelGGL((k_queue<MATH, BV, TRACE, NR, REFINE, false, true>), dim3(c.grid), dim3(c.block),
+                                         c.lds_bytes, s, kp);
+            else hipLaunchKernelGGL((k_queue<MATH, BV, TRACE, NR, REFINE, false, false>), dim3(c.grid), dim3(c.block),
+                                    c.lds_bytes, s, kp);
+        }
+    } else {
+        hipLaunchKernelGGL((k_queue<MATH, BV, TRACE, NR, REFINE>), dim3(c.grid), dim3(c.block), c.lds_bytes, s, kp);
+    }
     return hipGetLastError();
 }
 

@@ -37,6 +37,7 @@ struct rf_ctx {
     int n_cu = 0;
     int clock_mhz = 0;
     int tuned_rpt = 0, tuned_block = 0, tuned_transform = 0;
+    int tuned_tile = 0;                  // band records in LDS: 0 untuned, 1 off, 2 on
     // scratch device buffers
     GeomDesc *d_geoms = nullptr; size_t geoms_cap = 0;
     WorkUnit *d_units = nullptr; size_t units_cap = 0;
@@ -63,6 +64,7 @@ struct rf_ctx {
     std::vector<uint64_t> last_key;      // last persistent solve run eagerly
     int64_t graph_replays = 0;
     int fail_in = 0;                     // test hook: the fail_in-th next solve fails (0 = off)
+    LaunchCfg last_cfg{};                // shape of the last solve launch (reported in rf_timing)
     // live objects owned by the context (released by rf_ctx_destroy)
     std::set<rf_scene *> scenes;
     std::set<rf_stream *> streams;
@@ -108,6 +110,8 @@ struct rf_scene {
     bool has_search = false;             // a fast-form band stack with sorted offsets
     std::vector<StackInfo> stack_info;   // the fast scan's band stacks
     int32_t stack_prims = 0;             // primitives inside them
+    int32_t n_nonstack = 0;              // primitives outside the fast scan's band stacks
+    int32_t n_cnonstack = 0;             // the same for the compat / FP64 scans (caller order)
 };
 
 // Binned BV form: at most this many bounding surfaces per scene (LDS buckets;
@@ -162,6 +166,7 @@ extern "C" void rf_solve_opts_default(rf_solve_opts *o)
     o->n_samples = 1000000;
     o->form = RF_FORM_PERSISTENT;
     o->refine = 1;
+    o->count_tests = 1;
 }
 
 extern "C" const char *rf_last_error(const rf_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
@@ -187,7 +192,7 @@ extern "C" int32_t rf_ctx_create(int32_t dev, rf_ctx **out)
     }
     hipDeviceProp_t prop;
     if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
-    if (e == hipSuccess) e = hipMalloc((void **)&c->d_stats, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->d_stats, kNStats * sizeof(unsigned long long));
     if (e != hipSuccess) {
         g_err = std::string("context creation failed: ") + hipGetErrorString(e);
         delete c;
@@ -559,14 +564,15 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     std::vector<BandRec> c_bands(n + 1, BandRec{0.f, 0});
     for (int32_t k = 0; k < n; ++k) c_bands[k + 1] = BandRec{prims[k].invm[11], k};
     // the same for the children of each bounding surface (binned BV form), and
-    // the children's flops per ray (per test; with stacks, those outside them)
+    // per surface: the children's flops per ray (per test; with stacks, those
+    // outside them), the children, and the children outside the stacks
     std::vector<CylRun> bvc_runs;
     std::vector<int32_t> bvc_run_off;
     std::vector<BandRec> bvc_bands;
     std::vector<uint32_t> bv_cflops;
     if (binned) {
         bvc_run_off.resize(nbv + 1);
-        bv_cflops.resize(2 * (size_t)nbv);
+        bv_cflops.resize(4 * (size_t)nbv);
         bvc_bands.assign(bvc.size() + 1, BandRec{0.f, 0});
         for (int32_t b = 0; b < nbv; ++b) {
             const int32_t *sg = &bvc_seg[(size_t)b * (N_SEG + 1)];
@@ -574,8 +580,10 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
             const int64_t d = build_cyl_runs(bvc, sg[SEG_CYL], sg[SEG_CYL + 1], bvc_runs);
             int64_t f = 0;
             for (int32_t k = sg[0]; k < sg[N_SEG]; ++k) f += flops_per_test(bvc[k].type);
-            bv_cflops[2 * b] = (uint32_t)f;
-            bv_cflops[2 * b + 1] = (uint32_t)(f + d);
+            bv_cflops[4 * b] = (uint32_t)f;
+            bv_cflops[4 * b + 1] = (uint32_t)(f + d);
+            bv_cflops[4 * b + 2] = (uint32_t)(sg[N_SEG] - sg[0]);
+            bv_cflops[4 * b + 3] = (uint32_t)(sg[N_SEG] - sg[0] + d / flops_per_test(RF_CYLINDER));
             for (int32_t k = sg[SEG_CYL]; k < sg[SEG_CYL + 1]; ++k) bvc_bands[k + 1] = BandRec{bvc[k].r[11], bvc[k].orig};
         }
         bvc_run_off[nbv] = (int32_t)bvc_runs.size();
@@ -583,6 +591,8 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     s->flops_per_ray = fsum;
     s->flops_per_ray_nonstack = fnonstack;
     s->flops_per_ray_cnonstack = fcnonstack;
+    s->n_nonstack = n - (int32_t)((fsum - fnonstack) / flops_per_test(RF_CYLINDER));
+    s->n_cnonstack = n - (int32_t)((fsum - fcnonstack) / flops_per_test(RF_CYLINDER));
     s->flops_per_test_avg = (double)(fsum + fbv) / (double)(n + nbv);
     s->flops_bvs = fbv;
 
@@ -704,6 +714,8 @@ static int check_opts(rf_ctx *ctx, const rf_solve_opts *o)
     if (o->per_test < 0 || o->per_test > 2) return set_err(ctx, RF_ERR_INVALID_ARG, "per_test must be 0, 1 or 2");
     if (o->block_size && (o->block_size < 64 || o->block_size > 256 || o->block_size % 64))
         return set_err(ctx, RF_ERR_INVALID_ARG, "block_size must be 0 or a multiple of 64 in [64, 256]");
+    if (o->count_tests != 0 && o->count_tests != 1) return set_err(ctx, RF_ERR_INVALID_ARG, "count_tests must be 0 or 1");
+    if (o->band_tile < -1) return set_err(ctx, RF_ERR_INVALID_ARG, "band_tile must be -1, 0 or positive");
     return RF_OK;
 }
 
@@ -778,6 +790,36 @@ static bool uses_stacks(const rf_ctx *ctx, const rf_solve_opts *o)
     return (o->math == RF_MATH_FAST && !mfma) || o->math == RF_MATH_COMPAT || o->math == RF_MATH_FP64;
 }
 
+// Records of a scene's fast-form cylinder segment (the band records the
+// searched stacks read).
+static int32_t cyl_records(const rf_scene *s) { return s->desc.seg_end[SEG_CYL] - s->desc.seg_begin[SEG_CYL]; }
+
+// Launch switches of the fast band search (1 and 2 rays per thread, VALU
+// transform, sorted stacks searched): the device counters (count_tests), and
+// the LDS copy of each unit's band records when every scene's cylinder
+// segment fits the tile (band_tile: -1 off, 0 the tuned choice -- off when
+// untuned: on the 256-geometry sweep the copy measured 1.2% slower than the
+// cached global reads -- else the most records staged) and the group's LDS
+// stays within kMaxLds.  Other launches keep the counters and read the
+// records from global memory.
+static void band_switches(const rf_ctx *ctx, const rf_solve_opts *o, LaunchCfg &c, const KParams &kp,
+                          int32_t max_cyl, bool searched)
+{
+    c.count = 1;
+    c.bt = 0;
+    const bool search = c.math == KM_FAST && !c.use_bv && !c.trace && !c.direct && !c.mfma && c.nr <= 2 &&
+                        kp.stacks == 2 && searched;
+    if (!search) return;
+    c.count = o->count_tests;
+    const bool on = o->band_tile > 0 || (o->band_tile == 0 && ctx->tuned_tile == 2);
+    const int32_t cap = o->band_tile > 0 ? o->band_tile : (1 << 13);
+    const size_t bytes = ((c.lds_bytes + 7) & ~(size_t)7) + (size_t)max_cyl * 8;
+    if (on && max_cyl <= cap && bytes <= kMaxLds) {
+        c.bt = 1;
+        c.lds_bytes = bytes;
+    }
+}
+
 static int ensure_queue(rf_ctx *ctx)
 {
     if (ctx->d_queue) return RF_OK;
@@ -846,7 +888,18 @@ static int max_groups(const rf_ctx *ctx, int block)
 // lane: fan lane whose stream takes the launch (0 = the context stream).
 static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int64_t n_wi,
                         const rf_solve_opts *o, uint32_t geom_id, int32_t *trace_out, int64_t trace_wi0,
-                        int64_t *launches, int lane = 0)
+                        int64_t *launches, int lane = 0);
+
+static int launch_cfg_solve(rf_ctx *ctx, const LaunchCfg &c, const KParams &kp, hipStream_t st)
+{
+    if (!c.trace) ctx->last_cfg = c;
+    HIPCHK(ctx, launch_solve(c, kp, st));
+    return RF_OK;
+}
+
+static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int64_t n_wi,
+                        const rf_solve_opts *o, uint32_t geom_id, int32_t *trace_out, int64_t trace_wi0,
+                        int64_t *launches, int lane)
 {
     const hipStream_t st = lane_stream(ctx, lane);
     if (n_wi <= 0) return RF_OK;
@@ -883,7 +936,8 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
         if (grid > INT32_MAX) return set_err(ctx, RF_ERR_CAPACITY, "grid too large");
         c.grid = (int)grid;
         c.lds_bytes = (size_t)kPendK * c.block * sizeof(uint32_t);
-        HIPCHK(ctx, launch_solve(c, kp, st));
+        const int rcd = launch_cfg_solve(ctx, c, kp, st);
+        if (rcd) return rcd;
         ++*launches;
         return RF_OK;
     }
@@ -906,13 +960,15 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
     c.grid = (int)std::min<int64_t>(n_units, groups);
     c.lds_bytes = ((c.trace || kp.bins == 0) ? 0 : (size_t)kp.bins * sizeof(uint32_t)) +
                   (size_t)kPendK * block * sizeof(uint32_t);
+    band_switches(ctx, o, c, kp, cyl_records(s), s->has_search);
     if (c.math == KM_FAST && c.use_bv && s->desc.bv_binned) {
         c.bvbin = 1;
         kp.bv_slots = s->n_bvs;
         if (bvbin_lds(block, kp.bv_slots, kp.bins) > kMaxLds) kp.bins = 0;
         c.lds_bytes = bvbin_lds(block, kp.bv_slots, c.trace ? 0 : kp.bins);
     }
-    HIPCHK(ctx, launch_solve(c, kp, st));
+    rc = launch_cfg_solve(ctx, c, kp, st);
+    if (rc) return rc;
     ++*launches;
     return RF_OK;
 }
@@ -983,9 +1039,9 @@ static double now_ms()
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-static int read_stats(rf_ctx *ctx, unsigned long long st[4])
+static int read_stats(rf_ctx *ctx, unsigned long long st[kNStats])
 {
-    HIPCHK(ctx, hipMemcpy(st, ctx->d_stats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(st, ctx->d_stats, kNStats * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     return RF_OK;
 }
 
@@ -996,10 +1052,13 @@ static void account(const rf_ctx *ctx, const rf_scene *s, const rf_solve_opts *o
     int64_t rays = 0;
     for (int64_t w : n_wi) rays += 4 * w;
     t.rays += rays;
+    t.tests_brute_force += rays * s->n;
     if (!(o->use_bvs && s->n_bvs > 0)) {
-        t.tests += rays * s->n;
-        // band stacks add their flops from the device counter (stats[3])
-        const int64_t f = !uses_stacks(ctx, o) ? s->flops_per_ray
+        // band stacks add their evaluations and flops from the device
+        // counters (stats[4], stats[3]); every other primitive is one test
+        const bool st = uses_stacks(ctx, o);
+        t.tests += rays * (!st ? s->n : (o->math == RF_MATH_FAST ? s->n_nonstack : s->n_cnonstack));
+        const int64_t f = !st ? s->flops_per_ray
                           : (o->math == RF_MATH_FAST ? s->flops_per_ray_nonstack : s->flops_per_ray_cnonstack);
         t.flops += rays * f;
     }
@@ -1092,6 +1151,9 @@ static int prepare_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms,
     c.block = block;
     c.grid = (int)std::min<int64_t>((int64_t)units.size(), groups);
     c.lds_bytes = (kp.bins ? (size_t)kp.bins * sizeof(uint32_t) : 0) + (size_t)kPendK * block * sizeof(uint32_t);
+    int32_t max_cyl = 0;
+    for (int g = 0; g < n_geoms; ++g) max_cyl = std::max(max_cyl, cyl_records(scenes[g]));
+    band_switches(ctx, o, c, kp, max_cyl, searched);
     if (c.math == KM_FAST && use_bv) {
         bool binned = true;
         int slots = 0;
@@ -1117,7 +1179,7 @@ static hipError_t enqueue_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n
     hipError_t e = hipSuccess;
     for (int g = 0; g < n_geoms && e == hipSuccess; ++g)
         e = launch_fill_u64(scenes[g]->d_hits, (size_t)scenes[g]->n * scenes[g]->n, ctx->stream);
-    if (e == hipSuccess) e = launch_fill_u64(ctx->d_stats, 4, ctx->stream);
+    if (e == hipSuccess) e = launch_fill_u64(ctx->d_stats, kNStats, ctx->stream);
     if (e == hipSuccess && kp.queue_head) e = launch_fill_u32(kp.queue_head, 0u, 1, ctx->stream);
     if (e == hipSuccess && kp.n_units > 0) e = launch_solve(c, kp, ctx->stream);
     return e;
@@ -1138,7 +1200,8 @@ static std::vector<uint64_t> persistent_key(const rf_ctx *ctx, rf_scene *const *
         std::memcpy(&w, ob + i, std::min<size_t>(8, sizeof ob - i));
         k.push_back(w);
     }
-    k.push_back(((uint64_t)ctx->tuned_rpt << 40) ^ ((uint64_t)ctx->tuned_block << 20) ^ (uint64_t)ctx->tuned_transform);
+    k.push_back(((uint64_t)ctx->tuned_rpt << 40) ^ ((uint64_t)ctx->tuned_block << 20) ^
+                ((uint64_t)ctx->tuned_tile << 8) ^ (uint64_t)ctx->tuned_transform);
     for (int g = 0; g < n_geoms; ++g) {
         k.push_back(scenes[g]->serial);
         k.push_back(ids[g]);
@@ -1182,6 +1245,7 @@ static int solve_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, c
     bool empty = false;
     int rc = prepare_persistent(ctx, scenes, n_geoms, o, budgets, ids, c, kp, empty);
     if (rc) return rc;
+    ctx->last_cfg = c;
     if (!empty && key == ctx->last_key) {
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
@@ -1244,7 +1308,7 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
         // hit matrices and statistics start at zero
         for (int g = 0; g < n_geoms; ++g)
             HIPCHK(ctx, launch_fill_u64(scenes[g]->d_hits, (size_t)scenes[g]->n * scenes[g]->n, ctx->stream));
-        HIPCHK(ctx, launch_fill_u64(ctx->d_stats, 4, ctx->stream));
+        HIPCHK(ctx, launch_fill_u64(ctx->d_stats, kNStats, ctx->stream));
         HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         // one launch per emitter; queue-kernel launches of different emitters
         // are independent and fan out over the lanes, the reference shape
@@ -1263,7 +1327,7 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
     HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
     float ms = 0.f;
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    unsigned long long st[4];
+    unsigned long long st[kNStats];
     rc = read_stats(ctx, st);
     if (rc) return rc;
     rf_timing t{};
@@ -1281,7 +1345,17 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
         if (rays && rays[g])
             for (int e = 0; e < scenes[g]->n; ++e) rays[g][e] = 4 * budgets[g][e];
     }
-    if (!o->use_bvs && uses_stacks(ctx, o)) t.flops += (int64_t)st[3];
+    if (!o->use_bvs && uses_stacks(ctx, o)) {
+        t.flops += (int64_t)st[3];
+        t.tests += (int64_t)st[4];
+        // the fast band search ran without its counters: not measured
+        int nr, block, mfma;
+        bool searched = false;
+        for (int g = 0; g < n_geoms; ++g) searched = searched || scenes[g]->has_search;
+        pick_shape(ctx, o, nr, block, mfma, searched);
+        if (o->math == RF_MATH_FAST && !o->count_tests && o->per_test == 0 && searched && nr <= 2 && !mfma)
+            t.tests = t.flops = -1;
+    }
     if (o->use_bvs) {
         t.tests += (int64_t)st[0];
         // the binned kernel counts its children's flops on the device (stats[3],
@@ -1301,6 +1375,16 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
             t.flops += (int64_t)((double)st[0] * favg);
         }
     }
+    const LaunchCfg &lc = ctx->last_cfg;
+    t.shape_rays_per_thread = lc.mfma ? 0 : (lc.direct || o->math != RF_MATH_FAST || o->use_bvs ? 4 : lc.nr);
+    t.shape_block = lc.block;
+    t.shape_flags = (lc.mfma ? RF_SHAPE_MFMA : 0) | (lc.bvbin ? RF_SHAPE_BV_BINNED : 0) |
+                    (lc.direct ? RF_SHAPE_REFERENCE : 0) | (lc.bt ? RF_SHAPE_BAND_TILE : 0) |
+                    (uses_stacks(ctx, o) ? RF_SHAPE_STACKS : 0) |
+                    (uses_stacks(ctx, o) && o->per_test == 0 && lc.math == KM_FAST && lc.nr <= 2 && !lc.mfma &&
+                             !lc.use_bv
+                         ? RF_SHAPE_SEARCHED : 0) |
+                    (lc.count || !(lc.math == KM_FAST && lc.nr <= 2) ? RF_SHAPE_COUNTED : 0);
     t.wall_ms = now_ms() - w0;
     if (timing) *timing = t;
     return RF_OK;
@@ -1405,28 +1489,32 @@ extern "C" int32_t rf_autotune(rf_ctx *ctx, rf_scene *const *scenes, int32_t n_g
     rays = std::max<int64_t>(rays, 4096);
     if (o->budget == RF_BUDGET_EXPLICIT && o->rays_per_emitter > 0) rays = std::min(rays, o->rays_per_emitter);
     t.rays_per_emitter = rays;
-    // candidates: VALU transform with 2/4/8 rays per thread, MFMA transform
-    const int cands[][2] = {{RF_TRANSFORM_VALU, 2}, {RF_TRANSFORM_VALU, 4}, {RF_TRANSFORM_VALU, 8},
-                            {RF_TRANSFORM_MFMA, 4}};
+    t.count_tests = 0;                  // timed as the solves it tunes
+    // candidates: VALU transform with 2 rays per thread (band records in LDS
+    // or not: the scene tile), 4 and 8 rays per thread, MFMA transform
+    const int cands[][3] = {{RF_TRANSFORM_VALU, 2, 1}, {RF_TRANSFORM_VALU, 2, -1}, {RF_TRANSFORM_VALU, 4, -1},
+                            {RF_TRANSFORM_VALU, 8, -1}, {RF_TRANSFORM_MFMA, 4, -1}};
     const int blocks[] = {128, 256};
     double best = 1e300;
-    int br = 4, bb = 256, bt = RF_TRANSFORM_VALU;
+    int br = 4, bb = 256, bt = RF_TRANSFORM_VALU, btile = -1;
     for (const auto &cd : cands)
         for (int bl : blocks) {
             t.transform = cd[0];
             t.rays_per_thread = cd[1];
+            t.band_tile = cd[2] > 0 ? (1 << 13) : -1;
             t.block_size = bl;
             rf_timing tm{};
             rc = solve_impl(ctx, scenes, n_geoms, &t, nullptr, nullptr, &tm);   // warm
             if (rc) return rc;
             rc = solve_impl(ctx, scenes, n_geoms, &t, nullptr, nullptr, &tm);
             if (rc) return rc;
-            if (tm.kernel_ms < best) { best = tm.kernel_ms; br = cd[1]; bb = bl; bt = cd[0]; }
+            if (tm.kernel_ms < best) { best = tm.kernel_ms; br = cd[1]; bb = bl; bt = cd[0]; btile = cd[2]; }
         }
     ctx->tuned_transform = bt;
     if (bt == RF_TRANSFORM_MFMA) br = 0;
     ctx->tuned_rpt = br ? br : 4;
     ctx->tuned_block = bb;
+    ctx->tuned_tile = btile > 0 ? 2 : 1;
     if (best_rpt) *best_rpt = br;
     if (best_block) *best_block = bb;
     return RF_OK;

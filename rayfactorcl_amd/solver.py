@@ -24,12 +24,17 @@ def make_opts(math: str = "compat", rng: str = "reference", use_bvs: bool = Fals
               n_samples: int | None = None, rays_per_emitter: int | None = None,
               form: str = "persistent", refine: bool = True, seed: int = 0, geometry: int = 0,
               rays_per_thread: int = 0, block_size: int = 0, transform: str = "auto",
-              refine_margin: float = 0.0, per_test: bool | int = False) -> SolveOpts:
+              refine_margin: float = 0.0, per_test: bool | int = False, count_tests: bool = True,
+              band_tile: int = 0) -> SolveOpts:
     """Solve options.  The ray budget is the reference's launch geometry
     (``n_samples``, main.cpp:172-173) unless ``rays_per_emitter`` is given.
     ``per_test``: False / 0 = band stacks, sorted ones searched; True / 1 =
     every cylinder test in full; 2 = band stacks with every band's z tested.
-    The tallies are the same in all three."""
+    The tallies are the same in all three.  ``count_tests``: the fast band
+    search counts its evaluations (timing ``tests`` / ``flops`` are -1 without).
+    ``band_tile``: LDS copy of the band records for the fast band search
+    (0 = the autotuner's choice, off until tuned; -1 = off; n = at most n
+    records)."""
     o = SolveOpts()
     lib().rf_solve_opts_default(byref(o))
     o.math = MATH[math]
@@ -44,6 +49,8 @@ def make_opts(math: str = "compat", rng: str = "reference", use_bvs: bool = Fals
     o.transform = TRANSFORM[transform]
     o.refine_margin = float(refine_margin)
     o.per_test = int(per_test)
+    o.count_tests = int(bool(count_tests))
+    o.band_tile = int(band_tile)
     if rays_per_emitter is not None:
         o.budget = 1
         o.rays_per_emitter = int(rays_per_emitter)

@@ -41,8 +41,9 @@ def test_stacks_equal_per_test(ctx, form, rpt, stacked):
     sc = scenes.parse_xml(scenes.furnace_xml(wall_bands=40, preform_bands=12))
     a, b = _pair(ctx, sc, stacked, rays_per_emitter=20000, form=form, rays_per_thread=rpt)
     _same(a, b)
-    # the stacked form counts fewer algorithmic flops for the same tests
-    assert b.timing["tests"] == a.timing["tests"] and b.timing["flops"] < a.timing["flops"]
+    # the same rays; the stacked form runs fewer algorithmic flops
+    assert b.timing["tests_brute_force"] == a.timing["tests_brute_force"] == a.timing["tests"]
+    assert b.timing["flops"] < a.timing["flops"]
 
 
 @pytest.mark.parametrize("stacked", STACKED)
@@ -74,7 +75,11 @@ def test_stacks_sweep_batched(ctx):
         b, tb = rf.batch_solve(ctx, dev, rf.make_opts(per_test=stacked, **kw))
         for x, y in zip(a, b):
             assert np.array_equal(x.hits, y.hits)
-        assert ta["refined"] == tb["refined"] and ta["tests"] == tb["tests"]
+        assert ta["refined"] == tb["refined"] and ta["tests_brute_force"] == tb["tests_brute_force"]
+        # the search looks at a few bands per root; the walk at every band,
+        # plus one quadric per stack (more evaluations than brute force)
+        assert ta["tests"] == ta["tests_brute_force"]
+        assert (tb["tests"] < ta["tests"]) if stacked == 0 else (tb["tests"] > ta["tests"])
         print(f"[stacks] sweep sample: flops per launch {ta['flops']:.3e} per test, {tb['flops']:.3e} "
               f"stacked ({stacked})", flush=True)
     for d in dev:
@@ -192,7 +197,9 @@ def test_stacks_bv_binned(ctx, margin, form, stacked):
     a = rf.solve(ctx, dev, per_test=True, **kw)
     b = rf.solve(ctx, dev, per_test=stacked, **kw)
     _same(a, b)
-    assert a.timing["tests"] == b.timing["tests"]
+    assert a.timing["tests_brute_force"] == b.timing["tests_brute_force"]
+    # the search looks at a few bands; the walk at every band plus a quadric
+    assert (0 < b.timing["tests"] < a.timing["tests"]) if stacked == 0 else (b.timing["tests"] > a.timing["tests"])
     assert 0 < b.timing["flops"] < a.timing["flops"]
     dev.close()
 
@@ -277,7 +284,9 @@ def test_compat_stacks_bv_match_oracle(ctx, oracle, math, stacked):
     diff = np.argwhere(got.hits != want)
     assert diff.size == 0, f"{len(diff)} cells differ, first {diff[:5].tolist()}"
     a = rf.solve(ctx, dev, math=math, use_bvs=True, rays_per_emitter=4 * n_wi, per_test=True)
-    assert np.array_equal(a.hits, got.hits) and a.timing["tests"] == got.timing["tests"]
+    assert np.array_equal(a.hits, got.hits) and a.timing["tests_brute_force"] == got.timing["tests_brute_force"]
+    # the search looks at a few bands; the walk at every band plus a quadric
+    assert (0 < got.timing["tests"] <= a.timing["tests"]) if stacked == 0 else (got.timing["tests"] >= a.timing["tests"])
     dev.close()
 
 
@@ -316,3 +325,59 @@ def test_compat_stacks_uneven_spacing(ctx, oracle, math):
     diff = np.argwhere(got.hits != want)
     assert diff.size == 0, f"{len(diff)} cells differ, first {diff[:5].tolist()}"
     dev.close()
+
+
+@pytest.mark.parametrize("math", ["fast", "compat", "fp64"])
+def test_executed_evaluation_counts(ctx, math):
+    """rf_timing.tests counts what runs: every test in the per-test form
+    (rays x n); in the walk form one quadric per (ray, stack) plus one z-range
+    check per band (exactly rays x (n outside stacks + sum of (1 + bands)));
+    in the search form fewer.  tests_brute_force is rays x n in all three."""
+    wall, pre = 40, 12
+    sc = scenes.parse_xml(scenes.furnace_xml(wall_bands=wall, preform_bands=pre))
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    rays = 1 << 14
+    kw = dict(math=math, rng="seeded", rays_per_emitter=rays, form="persistent", rays_per_thread=4 if math == "fast"
+              else 0)
+    full = rf.solve(ctx, dev, per_test=1, **kw).timing
+    walk = rf.solve(ctx, dev, per_test=2, **kw).timing
+    search = rf.solve(ctx, dev, per_test=0, **{**kw, "rays_per_thread": 2 if math == "fast" else 0}).timing
+    n_rays = rays * sc.n                       # every emitter
+    assert full["tests"] == full["tests_brute_force"] == n_rays * sc.n
+    per_ray = (sc.n - wall - pre) + (1 + wall) + (1 + pre)
+    assert walk["tests"] == n_rays * per_ray and walk["tests_brute_force"] == n_rays * sc.n
+    assert search["tests_brute_force"] == n_rays * sc.n
+    assert 0 < search["tests"] < walk["tests"]
+    print(f"[evals] {math}: evaluations per ray: per test {sc.n}, walk {per_ray}, "
+          f"search {search['tests'] / (rays * sc.n):.2f}", flush=True)
+    dev.close()
+
+
+@pytest.mark.parametrize("rpt", [1, 2])
+def test_band_tile_and_counters_keep_tallies(ctx, rpt):
+    """The band search reading its records from an LDS copy, and without its
+    evaluation counters, gives the same tallies and re-solves; the reported
+    shape says which ran; uncounted solves report tests / flops as -1."""
+    from rayfactorcl_amd import _lib as L
+    geoms = scenes.sweep(5, seed=0x52465331) + [_uneven_scene()]
+    dev = [rf.DeviceScene(ctx, g.prims, None) for g in geoms]
+    kw = dict(math="fast", rng="seeded", rays_per_emitter=1 << 14, form="persistent", rays_per_thread=rpt)
+    ref, tr = rf.batch_solve(ctx, dev, rf.make_opts(band_tile=-1, **kw))
+    assert tr["shape_flags"] & L.RF_SHAPE_SEARCHED and not tr["shape_flags"] & L.RF_SHAPE_BAND_TILE
+    # explicit tiles: the session's context may have been tuned (tile 0 = its choice)
+    for tile, count in [(8192, True), (8192, False), (-1, False), (4096, True)]:
+        got, t = rf.batch_solve(ctx, dev, rf.make_opts(band_tile=tile, count_tests=count, **kw))
+        for a, b in zip(ref, got):
+            assert np.array_equal(a.hits, b.hits), (tile, count)
+        assert t["refined"] == tr["refined"]
+        assert bool(t["shape_flags"] & L.RF_SHAPE_BAND_TILE) == (tile != -1)
+        assert bool(t["shape_flags"] & L.RF_SHAPE_COUNTED) == count
+        if count:
+            assert t["tests"] == tr["tests"] and t["flops"] == tr["flops"]
+        else:
+            assert t["tests"] == -1 and t["flops"] == -1
+    # a tile smaller than a scene's cylinder segment keeps the records in global memory
+    _, t = rf.batch_solve(ctx, dev, rf.make_opts(band_tile=8, **kw))
+    assert not t["shape_flags"] & L.RF_SHAPE_BAND_TILE
+    for d in dev:
+        d.close()
