@@ -247,6 +247,20 @@ int32_t rf_hits_reset(rf_ctx *ctx, rf_scene *scene);
 int32_t rf_solve(rf_ctx *ctx, rf_scene *scene, const rf_solve_opts *opts,
                  int64_t *hits, int64_t *rays, rf_timing *timing);
 
+/* The persistent form of rf_solve for callers that cannot allocate: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation.  No counterpart in the reference.  All emitters of the scene,
+ * in one launch on the context stream, tallied into the caller's device
+ * matrix d_hits (n*n uint64, row = emitter, cleared by the call); the same
+ * tallies as rf_solve with the same options (opts->form must be
+ * RF_FORM_PERSISTENT).  Asynchronous: rf_sync, then read d_hits.  It allocates
+ * nothing: the queue counter and statistics are a fixed 256-byte device array
+ * per GPU, held until the process ends and shared by every call on that GPU,
+ * so calls that overlap on one GPU (from any context or host thread) race.
+ * Limit: 2^31-1 work units (RF_ERR_CAPACITY beyond, before any launch); no
+ * timing is reported. */
+int32_t rf_solve_noalloc(rf_ctx *ctx, rf_scene *scene, const rf_solve_opts *opts, uint64_t *d_hits);
+
 /* Per-ray hit index (-1 = miss) of work items [wi0, wi0+n_wi) of one
  * emitter: idx_out holds 4*n_wi int32.  No reference counterpart: the
  * diagnostic behind the differing-ray counts between arithmetic modes. */

@@ -118,6 +118,7 @@ SIGNATURES = {
     "rf_hits_read": (c_int32, [_P, _P, _P]),
     "rf_hits_reset": (c_int32, [_P, _P]),
     "rf_solve": (c_int32, [_P, _P, POINTER(SolveOpts), _P, _P, POINTER(Timing)]),
+    "rf_solve_noalloc": (c_int32, [_P, _P, POINTER(SolveOpts), _P]),
     "rf_trace_rays": (c_int32, [_P, _P, c_int32, c_int64, c_int64, POINTER(SolveOpts), _P]),
     "rf_run_cl_gpurng": (c_int32, [_P, _P, c_int32, _P, c_int32, c_int32, c_int32, _P, _P, _P]),
     "rf_batch_solve": (c_int32, [_P, _P, c_int32, POINTER(SolveOpts), _P, _P, POINTER(Timing)]),

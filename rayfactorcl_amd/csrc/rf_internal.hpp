@@ -124,12 +124,17 @@ struct KParams {
     int32_t bins;                       // LDS tally bins (0 = global atomics per ray)
     // Implicit unit list (units == nullptr): unit u is work items
     // [imp_wi0 + u*imp_unit, min(imp_wi0 + (u+1)*imp_unit, imp_wi0 + imp_n_wi))
-    // of emitter imp_emitter of geoms[0], seeded with geometry word imp_geom_id.
+    // of emitter imp_emitter of geoms[0], seeded with geometry word imp_geom_id;
+    // with imp_emitter < 0, unit u is chunk u mod imp_upe of emitter u / imp_upe
+    // (every emitter with the budget imp_n_wi).
     int32_t imp_emitter;
     uint32_t imp_geom_id;
     int64_t imp_wi0;
     int64_t imp_n_wi;
     int64_t imp_unit;
+    int32_t imp_upe;
+    int32_t pad2;
+    unsigned long long *hits;           // tally into this n x n matrix instead of geoms[0]'s (nullptr: the geometry's)
     float tau;                          // fast form: relative margin of the FP64 re-solve flags
     int32_t bv_slots;                   // binned BV kernel: LDS buckets (>= n_bvs of every geometry)
     int32_t stacks;                     // plain (non-BV) forms: 1 = band stacks share the quadric (CylRun)

@@ -1495,8 +1495,51 @@ __device__ __forceinline__ int find_bv_fast(CPTR(GeomDesc) G, const Ray32 &R, ui
 }
 
 // ---------------------------------------------------------------------------
-// Tally
+// Work units and the tally
 // ---------------------------------------------------------------------------
+// The hit matrix a launch tallies into: the caller's (kp.hits) when set, else
+// the geometry's own.
+__device__ __forceinline__ unsigned long long *hits_of(const KParams &kp, CPTR(GeomDesc) G)
+{
+    return kp.hits ? kp.hits : G->hits;
+}
+
+struct UnitRef {
+    CPTR(GeomDesc) G;
+    int emitter;
+    int64_t wi0, n_wi;
+    uint32_t gid;
+};
+
+// Unit u of a persistent launch: from the unit array, or implicit -- work
+// items [imp_wi0 + c imp_unit, ...) of emitter imp_emitter, or, when
+// imp_emitter < 0, of emitter u / imp_upe (chunk c = u mod imp_upe; every
+// emitter of geoms[0] has the same budget imp_n_wi).
+__device__ __forceinline__ UnitRef unit_of(const KParams &kp, int u)
+{
+    UnitRef r;
+    if (kp.units) {
+        const WorkUnit U = ldc(cst(kp.units) + u);
+        r.G = cst(kp.geoms) + U.geom;
+        r.emitter = U.emitter;
+        r.wi0 = U.wi0;
+        r.n_wi = U.n_wi;
+        r.gid = r.G->geometry_id;
+        return r;
+    }
+    int c = u;
+    r.emitter = kp.imp_emitter;
+    if (kp.imp_emitter < 0) {
+        r.emitter = u / kp.imp_upe;
+        c = u - r.emitter * kp.imp_upe;
+    }
+    r.G = cst(kp.geoms);
+    r.wi0 = kp.imp_wi0 + (int64_t)c * kp.imp_unit;
+    const int64_t left = kp.imp_wi0 + kp.imp_n_wi - r.wi0;
+    r.n_wi = left < kp.imp_unit ? left : kp.imp_unit;
+    r.gid = kp.imp_geom_id;
+    return r;
+}
 template <bool TRACE, bool DIRECT>
 __device__ __forceinline__ void tally(const KParams &kp, CPTR(GeomDesc) G, int emitter, int64_t ray_abs,
                                       int idx, uint32_t *bins)
@@ -1506,7 +1549,7 @@ __device__ __forceinline__ void tally(const KParams &kp, CPTR(GeomDesc) G, int e
     } else {
         if (idx < 0) return;
         if (DIRECT || kp.bins == 0)
-            atomicAdd(&G->hits[(int64_t)emitter * G->n + idx], 1ull);
+            atomicAdd(&hits_of(kp, G)[(int64_t)emitter * G->n + idx], 1ull);
         else
             atomicAdd(&bins[idx], 1u);
     }
@@ -1934,23 +1977,11 @@ __global__ void __launch_bounds__(256) k_queue_mfma(KParams kp)
         __syncthreads();
         const int u = __builtin_amdgcn_readfirstlane(s_unit);
         if (u >= kp.n_units) break;
-        CPTR(GeomDesc) G;
-        int emitter;
-        int64_t wi0, n_wi;
-        uint32_t gid;
-        if (kp.units) {
-            const WorkUnit U = ldc(cst(kp.units) + u);
-            G = cst(kp.geoms) + U.geom;
-            emitter = U.emitter; wi0 = U.wi0; n_wi = U.n_wi;
-            gid = G->geometry_id;
-        } else {
-            G = cst(kp.geoms);
-            emitter = kp.imp_emitter;
-            wi0 = kp.imp_wi0 + (int64_t)u * kp.imp_unit;
-            const int64_t left = kp.imp_wi0 + kp.imp_n_wi - wi0;
-            n_wi = left < kp.imp_unit ? left : kp.imp_unit;
-            gid = kp.imp_geom_id;
-        }
+        const UnitRef ur = unit_of(kp, u);
+        CPTR(GeomDesc) G = ur.G;
+        const int emitter = ur.emitter;
+        const int64_t wi0 = ur.wi0, n_wi = ur.n_wi;
+        const uint32_t gid = ur.gid;
         const int n = G->n;
         if (lds_tally) {
             for (int i = threadIdx.x; i < n; i += blockDim.x) bins[i] = 0u;
@@ -1959,7 +1990,7 @@ __global__ void __launch_bounds__(256) k_queue_mfma(KParams kp)
         run_unit_mfma<REFINE, TRACE>(kp, G, gid, emitter, wi0, n_wi, bins, pend, nref, nover);
         if (lds_tally) {
             __syncthreads();
-            unsigned long long *row = G->hits + (int64_t)emitter * n;
+            unsigned long long *row = hits_of(kp, G) + (int64_t)emitter * n;
             for (int i = threadIdx.x; i < n; i += blockDim.x) {
                 const uint32_t c = bins[i];
                 if (c) atomicAdd(&row[i], (unsigned long long)c);
@@ -2214,23 +2245,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REFINE
         __syncthreads();
         const int u = __builtin_amdgcn_readfirstlane(s_unit);
         if (u >= kp.n_units) break;
-        CPTR(GeomDesc) G;
-        int emitter;
-        int64_t wi0, n_wi;
-        uint32_t gid;
-        if (kp.units) {
-            const WorkUnit U = ldc(cst(kp.units) + u);
-            G = cst(kp.geoms) + U.geom;
-            emitter = U.emitter; wi0 = U.wi0; n_wi = U.n_wi;
-            gid = G->geometry_id;
-        } else {
-            G = cst(kp.geoms);
-            emitter = kp.imp_emitter;
-            wi0 = kp.imp_wi0 + (int64_t)u * kp.imp_unit;
-            const int64_t left = kp.imp_wi0 + kp.imp_n_wi - wi0;
-            n_wi = left < kp.imp_unit ? left : kp.imp_unit;
-            gid = kp.imp_geom_id;
-        }
+        const UnitRef ur = unit_of(kp, u);
+        CPTR(GeomDesc) G = ur.G;
+        const int emitter = ur.emitter;
+        const int64_t wi0 = ur.wi0, n_wi = ur.n_wi;
+        const uint32_t gid = ur.gid;
         const int n = G->n;
         if (lds_tally) {
             for (int i = threadIdx.x; i < n; i += blockDim.x) bins[i] = 0u;
@@ -2240,7 +2259,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REFINE
                                       tests, cflops, nref, nover);
         if (lds_tally) {
             __syncthreads();
-            unsigned long long *row = G->hits + (int64_t)emitter * n;
+            unsigned long long *row = hits_of(kp, G) + (int64_t)emitter * n;
             for (int i = threadIdx.x; i < n; i += blockDim.x) {
                 const uint32_t c = bins[i];
                 if (c) atomicAdd(&row[i], (unsigned long long)c);
@@ -2279,23 +2298,11 @@ __attribute__((amdgpu_waves_per_eu((NR == 4 || NR == 2 || (MATH == KM_COMPAT && 
         __syncthreads();
         const int u = __builtin_amdgcn_readfirstlane(s_unit);
         if (u >= kp.n_units) break;
-        CPTR(GeomDesc) G;
-        int emitter;
-        int64_t wi0, n_wi;
-        uint32_t gid;
-        if (kp.units) {
-            const WorkUnit U = ldc(cst(kp.units) + u);
-            G = cst(kp.geoms) + U.geom;
-            emitter = U.emitter; wi0 = U.wi0; n_wi = U.n_wi;
-            gid = G->geometry_id;
-        } else {
-            G = cst(kp.geoms);
-            emitter = kp.imp_emitter;
-            wi0 = kp.imp_wi0 + (int64_t)u * kp.imp_unit;
-            const int64_t left = kp.imp_wi0 + kp.imp_n_wi - wi0;
-            n_wi = left < kp.imp_unit ? left : kp.imp_unit;
-            gid = kp.imp_geom_id;
-        }
+        const UnitRef ur = unit_of(kp, u);
+        CPTR(GeomDesc) G = ur.G;
+        const int emitter = ur.emitter;
+        const int64_t wi0 = ur.wi0, n_wi = ur.n_wi;
+        const uint32_t gid = ur.gid;
         const int n = G->n;
         BandsLds bl{};
         if constexpr (BT) {
@@ -2314,7 +2321,7 @@ __attribute__((amdgpu_waves_per_eu((NR == 4 || NR == 2 || (MATH == KM_COMPAT && 
                                                                 nover, bl);
         if (lds_tally) {
             __syncthreads();
-            unsigned long long *row = G->hits + (int64_t)emitter * n;
+            unsigned long long *row = hits_of(kp, G) + (int64_t)emitter * n;
             for (int i = threadIdx.x; i < n; i += blockDim.x) {
                 const uint32_t c = bins[i];
                 if (c) atomicAdd(&row[i], (unsigned long long)c);

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <set>
 #include <string>
@@ -65,6 +66,7 @@ struct rf_ctx {
     int64_t graph_replays = 0;
     int fail_in = 0;                     // test hook: the fail_in-th next solve fails (0 = off)
     LaunchCfg last_cfg{};                // shape of the last solve launch (reported in rf_timing)
+    void *scratch_ptr = nullptr;         // this device's allocation-free scratch (see scratch_setup)
     // live objects owned by the context (released by rf_ctx_destroy)
     std::set<rf_scene *> scenes;
     std::set<rf_stream *> streams;
@@ -171,6 +173,38 @@ extern "C" void rf_solve_opts_default(rf_solve_opts *o)
 
 extern "C" const char *rf_last_error(const rf_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free solve (rf_solve_noalloc): its queue counter
+// and statistics live in this fixed device array, one copy per GPU, held until
+// the process ends.  Host code uses the address hipGetSymbolAddress returns
+// for the current device (scratch_ptr), never the array's name.
+// Layout: [0, 4) queue counter; [8, 8 + 8 kNStats) statistics.
+// ---------------------------------------------------------------------------
+static constexpr size_t kScratchBytes = 256;
+static constexpr size_t kScratchStatsOff = 8;
+static_assert(kScratchStatsOff + kNStats * sizeof(unsigned long long) <= kScratchBytes, "scratch layout");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this GPU's scratch, looked up once per device with the
+// device current (called by rf_ctx_create).
+static hipError_t scratch_setup(int dev, void **ptr)
+{
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
 extern "C" int32_t rf_ctx_create(int32_t dev, rf_ctx **out)
 {
     if (!out) return set_err(nullptr, RF_ERR_INVALID_ARG, "out is NULL");
@@ -193,6 +227,7 @@ extern "C" int32_t rf_ctx_create(int32_t dev, rf_ctx **out)
     hipDeviceProp_t prop;
     if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
     if (e == hipSuccess) e = hipMalloc((void **)&c->d_stats, kNStats * sizeof(unsigned long long));
+    if (e == hipSuccess) e = scratch_setup(dev, &c->scratch_ptr);
     if (e != hipSuccess) {
         g_err = std::string("context creation failed: ") + hipGetErrorString(e);
         delete c;
@@ -1398,6 +1433,96 @@ extern "C" int32_t rf_solve(rf_ctx *ctx, rf_scene *s, const rf_solve_opts *o, in
     int64_t *const h[1] = {hits};
     int64_t *const r[1] = {rays};
     return solve_impl(ctx, sc, 1, o, h, r, timing);
+}
+
+// rf_solve's persistent form without any allocation: one launch whose work
+// units are implicit over every emitter (all emitters share one budget),
+// tallying into the caller's device matrix; the queue counter and statistics
+// are the device's scratch, cleared by fill kernels on the context stream
+// before the launch.  Asynchronous; no host or device allocation, no
+// hipMemsetAsync, nothing synchronised.
+extern "C" int32_t rf_solve_noalloc(rf_ctx *ctx, rf_scene *s, const rf_solve_opts *o, uint64_t *d_hits)
+{
+    if (!ctx || !s || !d_hits) return set_err(ctx, RF_ERR_INVALID_ARG, "NULL context, scene or tally");
+    if (s->ctx != ctx) return set_err(ctx, RF_ERR_INVALID_ARG, "scene of another context");
+    int rc = check_opts(ctx, o);
+    if (rc) return rc;
+    if (o->form != RF_FORM_PERSISTENT)
+        return set_err(ctx, RF_ERR_INVALID_ARG, "the allocation-free solve runs the persistent form only");
+    if (!ctx->scratch_ptr) return set_err(ctx, RF_ERR_HIP, "scratch address not set up");
+    // the budget every emitter shares (emitter_budget without its vector)
+    int64_t n_wi;
+    if (o->budget == RF_BUDGET_REFERENCE) {
+        rc = rf_nthreads(s->prims[0].rayDensity, o->n_samples, 4, 128, &n_wi);
+        if (rc) return set_err(ctx, rc, "rayDensity * n_samples overflows int32 (main.cpp:172)");
+        if (n_wi > INT32_MAX) return set_err(ctx, RF_ERR_OVERFLOW, "work items exceed the reference's int gid");
+    } else {
+        n_wi = (o->rays_per_emitter + 3) / 4;
+        if (o->rng == RF_RNG_REFERENCE && n_wi > ((int64_t)1 << 32))
+            return set_err(ctx, RF_ERR_OVERFLOW, "reference RNG keys work items with 32 bits");
+    }
+    int nr, block, mfma;
+    pick_shape(ctx, o, nr, block, mfma, s->has_search);
+    // units: about 8 per resident group over all emitters, whole passes of the
+    // group, never spanning two emitters
+    const int groups = max_groups(ctx, block);
+    const int64_t pass_wi = std::max<int64_t>(1, (int64_t)block * (nr >= 4 ? nr / 4 : 1));
+    const int64_t upe0 = std::max<int64_t>(1, (8 * (int64_t)groups + s->n - 1) / s->n);
+    int64_t unit = std::max<int64_t>(1, (n_wi + upe0 - 1) / upe0);
+    unit = std::min<int64_t>(unit, (int64_t)1 << 28);   // 32-bit LDS bins: < 2^30 rays per unit
+    unit = std::max(pass_wi, (unit + pass_wi - 1) / pass_wi * pass_wi);
+    const int64_t upe = n_wi > 0 ? (n_wi + unit - 1) / unit : 0;
+    const int64_t n_units = upe * s->n;
+    if (n_units > INT32_MAX || upe > INT32_MAX)
+        return set_err(ctx, RF_ERR_CAPACITY,
+                       "allocation-free solve: more than 2^31-1 work units for the scratch's 32-bit queue counter");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    rc = fan_join(ctx);
+    if (rc) return rc;
+    unsigned *queue = (unsigned *)ctx->scratch_ptr;
+    unsigned long long *stats = (unsigned long long *)((unsigned char *)ctx->scratch_ptr + kScratchStatsOff);
+    KParams kp{};
+    kp.geoms = s->d_desc;
+    kp.units = nullptr;
+    kp.queue_head = queue;
+    kp.stats = stats;
+    kp.n_units = (int32_t)n_units;
+    kp.rng = o->rng;
+    kp.seed = o->seed;
+    kp.refine = o->refine;
+    kp.tau = o->refine_margin > 0.0f ? o->refine_margin : rfd::TAU_DEFAULT;
+    kp.bins = s->n <= kMaxLdsBins ? s->n : 0;
+    kp.stacks = !uses_stacks(ctx, o) ? 0 : (o->per_test == 2 ? 1 : 2);
+    kp.imp_emitter = -1;
+    kp.imp_upe = (int32_t)upe;
+    kp.imp_geom_id = o->geometry;
+    kp.imp_wi0 = 0;
+    kp.imp_n_wi = n_wi;
+    kp.imp_unit = unit;
+    kp.hits = (unsigned long long *)d_hits;
+    LaunchCfg c{};
+    c.math = o->math == RF_MATH_COMPAT ? KM_COMPAT : (o->math == RF_MATH_FP64 ? KM_FP64 : KM_FAST);
+    c.use_bv = o->use_bvs && s->n_bvs > 0;
+    c.nr = nr;
+    c.refine = o->refine;
+    c.mfma = mfma;
+    c.block = block;
+    c.grid = (int)std::min<int64_t>(std::max<int64_t>(n_units, 1), groups);
+    c.lds_bytes = (kp.bins ? (size_t)kp.bins * sizeof(uint32_t) : 0) + (size_t)kPendK * block * sizeof(uint32_t);
+    band_switches(ctx, o, c, kp, cyl_records(s), s->has_search);
+    if (c.math == KM_FAST && c.use_bv && s->desc.bv_binned) {
+        c.bvbin = 1;
+        kp.bv_slots = s->n_bvs;
+        if (bvbin_lds(block, kp.bv_slots, kp.bins) > kMaxLds) kp.bins = 0;
+        c.lds_bytes = bvbin_lds(block, kp.bv_slots, kp.bins);
+    }
+    // the tally, the queue counter and the statistics start at zero (fill
+    // kernels: the sequence stays capturable in a graph)
+    HIPCHK(ctx, launch_fill_u64((unsigned long long *)d_hits, (size_t)s->n * s->n, ctx->stream));
+    HIPCHK(ctx, launch_fill_u32(queue, 0u, 1, ctx->stream));
+    HIPCHK(ctx, launch_fill_u64(stats, kNStats, ctx->stream));
+    if (n_units > 0) HIPCHK(ctx, launch_solve(c, kp, ctx->stream));
+    return RF_OK;
 }
 
 extern "C" int32_t rf_batch_solve(rf_ctx *ctx, rf_scene *const *scenes, int32_t n_geoms, const rf_solve_opts *o,
