@@ -237,6 +237,12 @@ int32_t rf_launch_emitter(rf_ctx *ctx, rf_scene *scene, int32_t emitter, int64_t
 int32_t rf_sync(rf_ctx *ctx);
 /* Replaces: the blocking clEnqueueReadBuffer of hits (main.cpp:199-205); int64 cells. */
 int32_t rf_hits_read(rf_ctx *ctx, rf_scene *scene, int64_t *hits);
+/* Replaces: the normalisation of main.cpp:216-246 (float division of the
+ * summed hits by samplesForObj), done on the device in FP64: vf[i][j] =
+ * hits[i][j] / rays_per_emitter for the scene's device tally (every emitter of
+ * a solve has the same budget; 0 where it is 0).  Bit-identical to
+ * rf_normalise on the same counts. */
+int32_t rf_vf_read(rf_ctx *ctx, rf_scene *scene, int64_t rays_per_emitter, double *vf);
 /* Replaces: the blocking zero write of hits (main.cpp:130-134). */
 int32_t rf_hits_reset(rf_ctx *ctx, rf_scene *scene);
 

@@ -116,6 +116,7 @@ SIGNATURES = {
     "rf_launch_emitter": (c_int32, [_P, _P, c_int32, c_int64, c_int64, POINTER(SolveOpts)]),
     "rf_sync": (c_int32, [_P]),
     "rf_hits_read": (c_int32, [_P, _P, _P]),
+    "rf_vf_read": (c_int32, [_P, _P, c_int64, _P]),
     "rf_hits_reset": (c_int32, [_P, _P]),
     "rf_solve": (c_int32, [_P, _P, POINTER(SolveOpts), _P, _P, POINTER(Timing)]),
     "rf_solve_noalloc": (c_int32, [_P, _P, POINTER(SolveOpts), _P]),

@@ -163,6 +163,7 @@ struct LaunchCfg {
 hipError_t launch_solve(const LaunchCfg &cfg, const KParams &kp, hipStream_t stream);
 hipError_t launch_fill_u32(unsigned int *p, unsigned int value, size_t count, hipStream_t stream);
 hipError_t launch_fill_u64(unsigned long long *p, size_t count, hipStream_t stream);
+hipError_t launch_normalise(const unsigned long long *hits, double *vf, size_t n2, double rays, hipStream_t stream);
 hipError_t launch_selftest_math(unsigned long long *counts, unsigned int *first, hipStream_t stream);
 
 // Algorithmic FP32 flops of one band stack, same counting as flops_per_test,

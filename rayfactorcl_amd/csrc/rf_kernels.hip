@@ -1215,8 +1215,8 @@ __device__ __forceinline__ void stack_search_sel(bool asc, const FastPrim &h, co
 // slots whose bit is set in vmask, ev the evaluations: one per (ray, stack)
 // for the shared quadric and one per band whose z range is checked (the
 // search's probes and walked positions, or every band of a walked stack).
-// Without COUNT the search's evaluations are not counted (fl and ev then
-// miss them; the caller knows).  BL: the search reads the band records from
+// Without COUNT nothing is counted (fl and ev are left alone: the launch
+// reports no counts).  BL: the search reads the band records from
 // this LDS copy when it is set (positions [BL.base, BL.base + n)).
 template <int NR, bool REFINE, bool COUNT = true, bool BT = false>
 __device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, int b, int e, int mode, float inv,
@@ -1277,9 +1277,11 @@ __device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, i
         };
         if constexpr (BT) search(BL);
         else search(BandsGlobal{BR});
-        const uint32_t q = (uint32_t)__builtin_popcount(vmask);
-        fl += q * kFlopsStack + zs * ((axial ? kFlopsBandZ0Axial : kFlopsBandZ0) + kFlopsBandRoot);
-        ev += q + zs;
+        if constexpr (COUNT) {
+            const uint32_t q = (uint32_t)__builtin_popcount(vmask);
+            fl += q * kFlopsStack + zs * ((axial ? kFlopsBandZ0Axial : kFlopsBandZ0) + kFlopsBandRoot);
+            ev += q + zs;
+        }
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
             const bool h1 = I1[i] < e;
@@ -1298,10 +1300,12 @@ __device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, i
         const int ci = h1 ? I1[i] : I2[i];
         if (c < tb[i]) { tb[i] = c; ib[i] = ci; }
     }
-    const uint32_t sf = kFlopsStack + (uint32_t)(e - b) * ((axial ? kFlopsBandZ0Axial : kFlopsBandZ0) +
-                                                           (wroot1 ? 2u : 1u) * kFlopsBandRoot);
-    fl += (uint32_t)__builtin_popcount(vmask) * sf;
-    ev += (uint32_t)__builtin_popcount(vmask) * (uint32_t)(1 + e - b);
+    if constexpr (COUNT) {
+        const uint32_t sf = kFlopsStack + (uint32_t)(e - b) * ((axial ? kFlopsBandZ0Axial : kFlopsBandZ0) +
+                                                               (wroot1 ? 2u : 1u) * kFlopsBandRoot);
+        fl += (uint32_t)__builtin_popcount(vmask) * sf;
+        ev += (uint32_t)__builtin_popcount(vmask) * (uint32_t)(1 + e - b);
+    }
 }
 
 // FP64 stage of the deferred re-solves (see Pending).  The flagged ray is
@@ -1540,6 +1544,19 @@ __device__ __forceinline__ UnitRef unit_of(const KParams &kp, int u)
     r.gid = kp.imp_geom_id;
     return r;
 }
+
+// Flush of a unit's LDS bins into its emitter's row: one 64-bit atomic add
+// per non-zero bin (a row may have several units; plain stores of whole rows
+// measured no less traffic and no faster).
+__device__ __forceinline__ void flush_row(const KParams &kp, const UnitRef &ur, const uint32_t *bins)
+{
+    const int n = ur.G->n;
+    unsigned long long *row = hits_of(kp, ur.G) + (int64_t)ur.emitter * n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint32_t c = bins[i];
+        if (c) atomicAdd(&row[i], (unsigned long long)c);
+    }
+}
 template <bool TRACE, bool DIRECT>
 __device__ __forceinline__ void tally(const KParams &kp, CPTR(GeomDesc) G, int emitter, int64_t ray_abs,
                                       int idx, uint32_t *bins)
@@ -1584,7 +1601,7 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
             const int64_t left = n_rays - base;
             scan_fast<NR, REFINE, COUNT, BT>(G, R, tb, ib, pd, kp.stacks, fl, ev,
                                              left < NR ? (1u << left) - 1u : (1u << NR) - 1u, BL);
-            if ((fl | ev) >> 30) {
+            if (COUNT && ((fl | ev) >> 30)) {
                 sflops += fl;
                 sevals += ev;
                 fl = ev = 0;
@@ -1601,8 +1618,10 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
             for (int i = 0; i < NR; ++i)
                 if (base + i < n_rays) tally<TRACE, DIRECT>(kp, G, emitter, ray0 + base + i, ib[i], bins);
         }
-        sflops += fl;
-        sevals += ev;
+        if constexpr (COUNT) {
+            sflops += fl;
+            sevals += ev;
+        }
     } else {
         // one work item (4 rays) per thread and step
         for (int64_t wl = first; wl < n_wi; wl += stride) {
@@ -1990,11 +2009,7 @@ __global__ void __launch_bounds__(256) k_queue_mfma(KParams kp)
         run_unit_mfma<REFINE, TRACE>(kp, G, gid, emitter, wi0, n_wi, bins, pend, nref, nover);
         if (lds_tally) {
             __syncthreads();
-            unsigned long long *row = hits_of(kp, G) + (int64_t)emitter * n;
-            for (int i = threadIdx.x; i < n; i += blockDim.x) {
-                const uint32_t c = bins[i];
-                if (c) atomicAdd(&row[i], (unsigned long long)c);
-            }
+            flush_row(kp, ur, bins);
         }
         __syncthreads();
     }
@@ -2259,11 +2274,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REFINE
                                       tests, cflops, nref, nover);
         if (lds_tally) {
             __syncthreads();
-            unsigned long long *row = hits_of(kp, G) + (int64_t)emitter * n;
-            for (int i = threadIdx.x; i < n; i += blockDim.x) {
-                const uint32_t c = bins[i];
-                if (c) atomicAdd(&row[i], (unsigned long long)c);
-            }
+            flush_row(kp, ur, bins);
         }
         __syncthreads();
     }
@@ -2321,11 +2332,7 @@ __attribute__((amdgpu_waves_per_eu((NR == 4 || NR == 2 || (MATH == KM_COMPAT && 
                                                                 nover, bl);
         if (lds_tally) {
             __syncthreads();
-            unsigned long long *row = hits_of(kp, G) + (int64_t)emitter * n;
-            for (int i = threadIdx.x; i < n; i += blockDim.x) {
-                const uint32_t c = bins[i];
-                if (c) atomicAdd(&row[i], (unsigned long long)c);
-            }
+            flush_row(kp, ur, bins);
         }
         __syncthreads();
     }
@@ -2353,6 +2360,14 @@ __global__ void k_fill_u32(unsigned int *p, unsigned int v, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// FP64 normalisation of a hit matrix: vf = hits / rays (the int64 count is
+// exact in double below 2^53; the division is IEEE double, as on the host).
+__global__ void k_normalise(const unsigned long long *hits, double *vf, size_t n2, double rays)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n2) vf[i] = rays > 0.0 ? (double)hits[i] / rays : 0.0;
 }
 
 __global__ void k_fill_u64(unsigned long long *p, size_t n)
@@ -2415,6 +2430,14 @@ hipError_t launch_fill_u32(unsigned int *p, unsigned int value, size_t count, hi
     if (!count) return hipSuccess;
     const unsigned blocks = (unsigned)((count + 255) / 256);
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, stream, p, value, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_normalise(const unsigned long long *hits, double *vf, size_t n2, double rays, hipStream_t stream)
+{
+    if (!n2) return hipSuccess;
+    const unsigned blocks = (unsigned)((n2 + 255) / 256);
+    hipLaunchKernelGGL(k_normalise, dim3(blocks), dim3(256), 0, stream, hits, vf, n2, rays);
     return hipGetLastError();
 }
 

@@ -1051,6 +1051,25 @@ static int read_hits(rf_ctx *ctx, rf_scene *s, int64_t *hits)
     return RF_OK;
 }
 
+extern "C" int32_t rf_vf_read(rf_ctx *ctx, rf_scene *s, int64_t rays_per_emitter, double *vf)
+{
+    if (!ctx || !s || !vf) return set_err(ctx, RF_ERR_INVALID_ARG, "NULL argument");
+    if (rays_per_emitter < 0) return set_err(ctx, RF_ERR_INVALID_ARG, "negative ray count");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int rc = fan_join(ctx);
+    if (rc) return rc;
+    const size_t n2 = (size_t)s->n * s->n;
+    double *d_vf = nullptr;
+    HIPCHK(ctx, hipMalloc((void **)&d_vf, n2 * sizeof(double)));
+    hipError_t e = launch_normalise(s->d_hits, d_vf, n2, (double)rays_per_emitter, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(vf, d_vf, n2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_vf);
+    if (e != hipSuccess) return set_err(ctx, RF_ERR_HIP, std::string("view-factor normalisation: ") + hipGetErrorString(e));
+    return RF_OK;
+}
+
 extern "C" int32_t rf_hits_read(rf_ctx *ctx, rf_scene *s, int64_t *hits)
 {
     if (!ctx || !s || !hits) return set_err(ctx, RF_ERR_INVALID_ARG, "NULL argument");

@@ -142,6 +142,13 @@ class DeviceScene:
         check(lib().rf_hits_read(self.ctx.handle, self.handle, ptr(out)), "rf_hits_read", self.ctx.handle)
         return out
 
+    def vf(self, rays_per_emitter: int) -> np.ndarray:
+        """View factors of the device tally, normalised on the device in FP64."""
+        out = np.empty((self.n, self.n), dtype=np.float64)
+        check(lib().rf_vf_read(self.ctx.handle, self.handle, int(rays_per_emitter), ptr(out)), "rf_vf_read",
+              self.ctx.handle)
+        return out
+
     def reset_hits(self) -> None:
         check(lib().rf_hits_reset(self.ctx.handle, self.handle), "rf_hits_reset", self.ctx.handle)
 

@@ -112,3 +112,19 @@ def test_launch_emitter_fan_out_and_implicit_join(ctx, form):
         for e in reversed(range(sc.n)):
             dev.launch_emitter(e, n_wi // 2, n_wi - n_wi // 2, o)
         assert np.array_equal(dev.hits(), want), rep
+
+
+def test_device_fp64_normalisation_matches_host(ctx, golden_scene):
+    """rf_vf_read divides the device tally by the emitters' ray count in FP64 on
+    the GPU: bit-identical to the host FP64 normalisation of the same counts,
+    and the rows of the closed enclosure sum to 1 less the seam losses."""
+    sc = golden_scene("config1")
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    rays = 1 << 20
+    r = rf.solve(ctx, dev, math="fast", rng="seeded", rays_per_emitter=rays)
+    vf = dev.vf(rays)
+    assert vf.dtype == np.float64
+    assert np.array_equal(vf, rf.normalise(r.hits, r.rays))
+    assert np.all(np.abs(vf[:3].sum(axis=1) - 1.0) < 2e-3)
+    assert np.array_equal(dev.vf(0), np.zeros_like(vf))
+    dev.close()
