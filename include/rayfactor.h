@@ -332,11 +332,17 @@ double rf_stream_queued_cost(const rf_stream *st);
 /* Enqueues one solve; the primitives are copied.  *request_id identifies it. */
 int32_t rf_stream_submit(rf_stream *st, const rf_primitive *prims, int32_t n_prims,
                          const rf_solve_opts *opts, int64_t *request_id);
-/* Runs the queued requests: requests share a persistent launch only when
- * rf_opts_same_tallies holds for their options; launches are packed as set by
- * rf_stream_set_cost_limits and run in order, each request completing with
- * its launch.  On an error, the requests not completed stay queued (a later
- * flush runs them) and *n_completed counts those that did. */
+/* Runs one persistent launch of queued work and returns: requests share a
+ * launch only when rf_opts_same_tallies holds for their options; the first
+ * queued request's option group goes first, cheapest requests first, packed up
+ * to the launch limit of rf_stream_set_cost_limits; a request costlier than
+ * the limit runs in chunks of whole emitters of about that cost, one chunk per
+ * step, so requests submitted in between can run before its next chunk.
+ * *n_completed counts the requests this step completed.  On an error nothing
+ * of the launch is kept and its requests stay queued. */
+int32_t rf_stream_step(rf_stream *st, int32_t *n_completed);
+/* rf_stream_step until the queue is empty; *n_completed counts the requests
+ * completed (also when an error stops it; the rest stay queued). */
 int32_t rf_stream_flush(rf_stream *st, int32_t *n_completed);
 /* Copies a completed request's n*n hits (and per-emitter rays) out and
  * releases its device memory.  RF_ERR_INVALID_ARG if not completed. */

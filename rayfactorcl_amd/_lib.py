@@ -128,6 +128,7 @@ SIGNATURES = {
     "rf_stream_destroy": (c_int32, [_P]),
     "rf_stream_submit": (c_int32, [_P, _P, c_int32, POINTER(SolveOpts), POINTER(c_int64)]),
     "rf_stream_flush": (c_int32, [_P, POINTER(c_int32)]),
+    "rf_stream_step": (c_int32, [_P, POINTER(c_int32)]),
     "rf_stream_collect": (c_int32, [_P, c_int64, _P, _P, POINTER(c_double)]),
     "rf_stream_resident_bytes": (c_int64, [_P]),
     "rf_stream_set_cost_limits": (c_int32, [_P, c_double, c_double]),

@@ -303,6 +303,12 @@ class Stream:
         check(lib().rf_stream_flush(self.handle, byref(n)), "rf_stream_flush", self.ctx.handle)
         return n.value
 
+    def step(self) -> int:
+        """One launch of queued work; returns the requests it completed."""
+        n = c_int32()
+        check(lib().rf_stream_step(self.handle, byref(n)), "rf_stream_step", self.ctx.handle)
+        return n.value
+
     def collect(self, rid: int) -> tuple[np.ndarray, np.ndarray, float]:
         n = self._n[rid]
         hits = np.zeros((n, n), dtype=np.int64)

@@ -117,6 +117,42 @@ def test_stream_cost_admission_and_packing(ctx):
     st.close()
 
 
+def test_stream_chunks_a_large_request(ctx):
+    """A request costlier than the launch limit runs in chunks of emitters,
+    one per step; its tallies are rf_solve's, and a small request submitted
+    between two chunks completes before the large one."""
+    big = scenes.parse_xml(scenes.soup_xml(120, seed=31))
+    small = scenes.parse_xml(scenes.soup_xml(15, seed=32))
+    ob = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=20000, seed=7)
+    os_ = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=4000, seed=7)
+    st = rf.Stream(ctx, 1 << 30)
+    cost_big = 20000 * 120 * 120
+    st.set_cost_limits(0.0, cost_big / 6)              # about 20 emitters per chunk
+    rb = st.submit(big.prims, ob)
+    steps, done = 0, 0
+    done += st.step()                                  # first chunk of the large request
+    steps += 1
+    assert done == 0 and 0 < st.queued_cost() < cost_big
+    rs = st.submit(small.prims, os_)                   # cheaper than what is left: runs next
+    done += st.step()
+    steps += 1
+    assert done == 1
+    hs, _, lat_s = st.collect(rs)
+    while done < 2:
+        done += st.step()
+        steps += 1
+    hb, rays_b, lat_b = st.collect(rb)
+    assert steps >= 7 and lat_s < lat_b
+    assert st.queued_cost() == 0 and st.resident_bytes() == 0
+    for sc, o, h in ((big, ob, hb), (small, os_, hs)):
+        dev = rf.DeviceScene(ctx, sc.prims, None)
+        ref = rf.solve(ctx, dev, o)
+        assert np.array_equal(h, ref.hits)
+        dev.close()
+    assert list(rays_b) == [20000] * big.n
+    st.close()
+
+
 def test_autotune_keeps_results(ctx):
     """Tuning picks a launch shape and a transform; the VALU kernel's tallies
     do not depend on the shape, and AUTO afterwards equals the chosen
