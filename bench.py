@@ -11,7 +11,7 @@ Headline: rays/s over the timed steps.  Beside it:
 - executed ray-primitive evaluations per second: what the kernel runs (one per
   primitive tested in full; per band stack one quadric per ray plus one z-range
   check per band it looks at), counted on the device in one extra launch of the
-  same solve -- the timed launches run without the counters, which cost ~3%,
+  same solve -- the timed launches run without the counters, which cost ~2%,
   and every launch of the solve does the same work;
 - brute-force-equivalent tests/s: rays x primitives per second, the tests the
   reference's runGPU scan would run for the same rays;

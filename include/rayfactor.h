@@ -132,7 +132,7 @@ typedef struct rf_solve_opts {
                                  1 = every cylinder test in full; 2 = band stacks, each band's z range
                                  tested.  Same tallies in all three, in every arithmetic form. */
     int32_t count_tests;      /* 1 (the default): the fast form's band search counts its evaluations
-                                 and flops on the device; 0: it does not (about 3% faster), and
+                                 and flops on the device; 0: it does not (about 2% faster), and
                                  rf_timing.tests and .flops are -1 for a solve that searched stacks.
                                  A launch counted once gives the counts of every identical launch. */
     int32_t band_tile;        /* RF_MATH_FAST band search (1 or 2 rays per thread): each work unit's

@@ -2287,7 +2287,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REFINE
 // SIMD) when asked to; left alone it takes a few more and drops to four.
 // The compat scan (four rays per visit) is held to the same five waves.
 // COUNT: the fast band search counts its evaluations (off for timed launches:
-// the counters cost ~3%, and a launch counted once gives the same counts).
+// the counters cost ~2%, and a launch counted once gives the same counts).
 // BT: each unit's band records (its geometry's cylinder segment) are copied
 // to LDS, where the searched stacks read them.
 template <int MATH, bool BV, bool TRACE, int NR, bool REFINE, bool COUNT = true, bool BT = false>
