@@ -32,16 +32,20 @@ __device__ __forceinline__ float finf() { return __builtin_inff(); }
 __device__ __forceinline__ double dinf() { return __builtin_inf(); }
 
 // ---------------------------------------------------------------------------
-// Philox4x32-10 (Random123/philox.h:206-305): 32x32->64 multiplies on
-// v_mul_hi_u32 / v_mul_lo_u32.
+// Philox4x32-10 (Random123/philox.h:206-305).  Each 32x32->64 product is one
+// v_mad_u64_u32 (both halves at once) rather than a v_mul_hi_u32 and a
+// v_mul_lo_u32: the integer multiplies are the costliest VALU operations of
+// ray emission.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k)
 {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         if (r) { k.x += 0x9E3779B9u; k.y += 0xBB67AE85u; }
-        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
         c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
     }
     return c;

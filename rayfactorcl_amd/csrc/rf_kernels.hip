@@ -1411,20 +1411,45 @@ __device__ __forceinline__ void scan_fast(CPTR(GeomDesc) G, const Ray32 (&R)[NR]
 }
 
 // The NR rays of ray group `base` (unit-local ray index) of one emitter.
+// base is a multiple of NR (NR = 1, 2, 4, 8), so a group never straddles two
+// work items unless NR > 4, and each ray's rotation of its work item's four
+// uniforms (ray v of a work item takes f[v+1], f[v+2], f[v+3], f[v], indices
+// mod 4) is known at compile time up to, for NR = 2, which half of the work
+// item the group is: one Philox block per work item and no per-ray
+// selection of the rotation.
 template <int NR>
 __device__ __forceinline__ void emit_group(const KParams &kp, uint32_t gid, int emitter, int64_t wi0, int64_t base,
                                            const rf_primitive &E, Ray32 (&R)[NR])
 {
+    static_assert(NR == 1 || NR == 2 || NR % 4 == 0, "rays per thread");
     float f[4];
-    int64_t cur_wi = -1;
-#pragma unroll
-    for (int i = 0; i < NR; ++i) {
-        const int64_t rl = base + i;
-        const int64_t wl = rl >> 2;
-        if (wl != cur_wi) { workitem_floats(kp, gid, emitter, wi0 + wl, f); cur_wi = wl; }
+    if constexpr (NR == 1) {
+        workitem_floats(kp, gid, emitter, wi0 + (base >> 2), f);
         float r4[4];
-        rotate_rands(f, (int)(rl & 3), r4);
-        R[i] = emit_fast(&E, r4);
+        rotate_rands(f, (int)(base & 3), r4);
+        R[0] = emit_fast(&E, r4);
+    } else if constexpr (NR == 2) {
+        workitem_floats(kp, gid, emitter, wi0 + (base >> 2), f);
+        // rays v = 0, 1 of the work item, or v = 2, 3
+        const bool upper = (base & 2) != 0;
+        float r4[4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) r4[j] = upper ? f[(i + j + 3) & 3] : f[(i + j + 1) & 3];
+            R[i] = emit_fast(&E, r4);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < NR / 4; ++q) {
+            workitem_floats(kp, gid, emitter, wi0 + (base >> 2) + q, f);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                float r4[4];
+                rotate_rands(f, v, r4);
+                R[4 * q + v] = emit_fast(&E, r4);
+            }
+        }
     }
 }
 
