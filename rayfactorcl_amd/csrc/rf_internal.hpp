@@ -58,7 +58,8 @@ struct GeomDesc {
     int32_t seg_begin[N_SEG];
     int32_t seg_end[N_SEG];
     uint32_t geometry_id;               // seeded RNG geometry word
-    int32_t pad;
+    int32_t zring_end;                  // fast: ring positions [seg_begin[SEG_RING], zring_end) are
+                                        // world z-plane rings (zring_loop's record encoding)
     // MFMA form: primitive quads (64 floats each: the 16x4 A operand of
     // v_mfma_f32_16x16x4f32, element A[4p + row][k] at a[16k + 4p + row]),
     // type segments padded to whole quads, and traversal position -> index.
@@ -181,6 +182,12 @@ constexpr uint32_t kFlopsBandRoot = 2;
 // roots and c_z t at each; per (ray, band) z origin and z at both roots.
 constexpr uint32_t kFlopsStackD = 51;
 constexpr uint32_t kFlopsBandD = 3;
+
+// World z-plane ring of the fast scan (zring_loop): t (subtract, multiply),
+// the hit's offsets from the centre (an FMA and a subtract each) and r^2 (a
+// multiply and an FMA); once per ray, the reciprocal of c_z.
+constexpr int64_t kFlopsZRing = 11;
+constexpr int64_t kFlopsZRingRay = 1;
 
 // Algorithmic FP32 flops per ray-primitive test (counted from the reference
 // formulas: FMA = 2, mul/add = 1, rcp/sqrt = 1, compare/select = 0).

@@ -1384,6 +1384,42 @@ __device__ __forceinline__ void finish_dropped(const KParams &kp, CPTR(GeomDesc)
     pd.resume = NO_KEY;
 }
 
+// Rings (discs, annuli) whose plane is z = z0 in world coordinates and whose
+// object frame is a similarity of world x-y (records [begin, end) of the ring
+// segment, re-encoded on the host: r[0] = z0, r[1], r[2] = the centre's world
+// x, y, k0, k1 = the r^2 range in world units).  The hit is tested in world
+// coordinates: t = (z0 - s_z) / c_z with 1/c_z computed once per ray for all
+// of them, and r^2 = (s_x + t c_x - x0)^2 + (s_y + t c_y - y0)^2.  No FP32
+// conditioning flag, as for every plane.
+template <int NR>
+__device__ __forceinline__ void zring_loop(CPTR(FastPrim) F, int begin, int end, const Ray32 (&R)[NR],
+                                           float (&tb)[NR], int (&ib)[NR], float eps)
+{
+    if (begin >= end) return;
+    float rcz[NR];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) rcz[i] = rcp_fast(R[i].cz);
+    FastPrim next = ldc(F + begin);
+    for (int w = begin; w < end; ++w) {
+        // the five fields the test reads, in VGPRs (three moves)
+        float z0, x0, y0, k0, k1;
+        vcopy2(next.r[0], next.r[1], z0, x0);
+        y0 = vcopy(next.r[2]);
+        vcopy2(next.k0, next.k1, k0, k1);
+        const int orig = next.orig;
+        next = ldc(F + min(w + 1, end - 1));
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const float t = (z0 - R[i].sz) * rcz[i];
+            const float dx = __builtin_fmaf(R[i].cx, t, R[i].sx) - x0;
+            const float dy = __builtin_fmaf(R[i].cy, t, R[i].sy) - y0;
+            const float rr = __builtin_fmaf(dx, dx, dy * dy);
+            const float c = ((t > eps) & (rr >= k0) & (rr <= k1)) ? t : finf();
+            if (c < tb[i]) { tb[i] = c; ib[i] = orig; }
+        }
+    }
+}
+
 // stacks: the cylinder segment runs as its CylRun list (band stacks share the
 // quadric; 2: sorted stacks are searched); otherwise as one per-test run.
 // fl and ev gain the band stacks' flops and evaluations of the ray slots set
@@ -1406,7 +1442,8 @@ __device__ __forceinline__ void scan_fast(CPTR(GeomDesc) G, const Ray32 (&R)[NR]
     }
     seg_loop<SEG_FRUSTUM, NR, REFINE, false>(F, G->seg_begin[SEG_FRUSTUM], G->seg_end[SEG_FRUSTUM], R, tb, ib, pd);
     seg_loop<SEG_SPHERE, NR, REFINE, false>(F, G->seg_begin[SEG_SPHERE], G->seg_end[SEG_SPHERE], R, tb, ib, pd);
-    seg_loop<SEG_RING, NR, REFINE, false>(F, G->seg_begin[SEG_RING], G->seg_end[SEG_RING], R, tb, ib, pd);
+    zring_loop<NR>(F, G->seg_begin[SEG_RING], G->zring_end, R, tb, ib, pd.eps);
+    seg_loop<SEG_RING, NR, REFINE, false>(F, G->zring_end, G->seg_end[SEG_RING], R, tb, ib, pd);
     seg_loop<SEG_RECT, NR, REFINE, false>(F, G->seg_begin[SEG_RECT], G->seg_end[SEG_RECT], R, tb, ib, pd);
 }
 

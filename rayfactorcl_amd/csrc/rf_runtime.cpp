@@ -99,7 +99,8 @@ struct rf_scene {
     GeomDesc *d_desc = nullptr;          // single-geometry descriptor
     GeomDesc desc{};                     // host copy
     int64_t flops_per_ray = 0;           // sum over primitives of F_type (plain kernels)
-    int64_t flops_per_ray_nonstack = 0;  // the same without the band stacks' cylinders (counted on the device)
+    int64_t flops_per_ray_fast = 0;      // the fast VALU scan's (world z-plane rings at kFlopsZRing)
+    int64_t flops_per_ray_nonstack = 0;  // the fast scan's without the band stacks' cylinders (counted on the device)
     int64_t flops_per_ray_cnonstack = 0; // the same for the compat scan's stacks (caller order)
     CylRun *d_c_runs = nullptr;          // compat scan: runs over the caller's order
     BandRec *d_c_bands = nullptr;
@@ -328,6 +329,44 @@ static FastPrim make_fast(const float invm[12], int32_t type, float u1, float u2
     return f;
 }
 
+// A disc or annulus whose plane is z = z0 in world coordinates and whose
+// object x-y is a similarity (scale s, rotation or reflection) of world x-y:
+// the fast scan tests it in world coordinates (zring_loop).  The record is
+// re-encoded: r[0] = z0, r[1], r[2] = the centre's world x, y, k0, k1 = the
+// r^2 range in world units (the object range / s^2).  Checked in FP64 on the
+// FP32 matrix: the z row must not depend on world x or y beyond the round-off
+// of a rotation by a multiple of 90 degrees (|m20|, |m21| at most 1e-7 |m22|;
+// float sin(pi) is -8.7e-8), and the x-y block's columns must have equal norms
+// and be orthogonal to 1e-6 relative.  The tilt so ignored moves the plane by
+// at most 1e-7 |x| in object z, the order of the FP32 rounding of the
+// object-frame test.
+static bool zring_record(const float invm[12], int32_t type, float u1, float u2, int32_t orig, FastPrim &f)
+{
+    if (type != RF_DISC && type != RF_ANNULUS) return false;
+    const double a = invm[0], b = invm[1], c = invm[4], d = invm[5];
+    const double m22 = invm[10];
+    if (!(std::fabs(m22) > 0.0) || !std::isfinite(m22)) return false;
+    if (std::fabs((double)invm[8]) > 1e-7 * std::fabs(m22) || std::fabs((double)invm[9]) > 1e-7 * std::fabs(m22))
+        return false;
+    const double s2 = a * a + c * c, s2b = b * b + d * d;
+    if (!(s2 > 0.0) || !std::isfinite(s2)) return false;
+    if (std::fabs(s2b - s2) > 1e-6 * s2 || std::fabs(a * b + c * d) > 1e-6 * s2) return false;
+    const double z0 = -(double)invm[11] / m22;
+    // object x, y on the plane: a X + b Y + e0, c X + d Y + e1 (the world-z
+    // terms are constants there); the centre solves them = 0
+    const double e0 = (double)invm[2] * z0 + (double)invm[3], e1 = (double)invm[6] * z0 + (double)invm[7];
+    const double det = a * d - b * c;
+    if (!(std::fabs(det) > 0.0)) return false;
+    const double x0 = (-e0 * d + b * e1) / det, y0 = (-a * e1 + c * e0) / det;
+    const double k0 = type == RF_DISC ? 0.0 : (double)u1, k1 = type == RF_DISC ? 1.0 : (double)u2;
+    f = FastPrim{};
+    f.r[0] = (float)z0; f.r[1] = (float)x0; f.r[2] = (float)y0;
+    f.k0 = (float)(k0 / s2); f.k1 = (float)(k1 / s2);
+    f.orig = orig; f.type = type;
+    return std::isfinite(f.r[0]) && std::isfinite(f.r[1]) && std::isfinite(f.r[2]) && std::isfinite(f.k0) &&
+           std::isfinite(f.k1);
+}
+
 static int seg_of(int32_t type)
 {
     switch (type) {
@@ -518,12 +557,30 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     // type-segmented traversal order (stable within a type)
     std::vector<FastPrim> fast, fast_orig(n), fast_bvs(nbv);
     std::vector<BvMeta> meta(nbv);
-    int32_t seg_begin[N_SEG], seg_end[N_SEG];
+    int32_t seg_begin[N_SEG], seg_end[N_SEG], zring_end = 0, n_zring = 0;
     for (int k = 0; k < N_SEG; ++k) {
         seg_begin[k] = (int32_t)fast.size();
-        for (int32_t i = 0; i < n; ++i)
-            if (seg_of(prims[i].type) == k)
-                fast.push_back(make_fast(prims[i].invm, prims[i].type, prims[i].util1, prims[i].util2, i));
+        if (k == SEG_RING) {
+            // world z-plane rings first (zring_loop), then the others
+            for (int32_t i = 0; i < n; ++i) {
+                FastPrim z;
+                if (seg_of(prims[i].type) == k &&
+                    zring_record(prims[i].invm, prims[i].type, prims[i].util1, prims[i].util2, i, z))
+                    fast.push_back(z);
+            }
+            zring_end = (int32_t)fast.size();
+            n_zring = zring_end - seg_begin[k];
+            for (int32_t i = 0; i < n; ++i) {
+                FastPrim z;
+                if (seg_of(prims[i].type) == k &&
+                    !zring_record(prims[i].invm, prims[i].type, prims[i].util1, prims[i].util2, i, z))
+                    fast.push_back(make_fast(prims[i].invm, prims[i].type, prims[i].util1, prims[i].util2, i));
+            }
+        } else {
+            for (int32_t i = 0; i < n; ++i)
+                if (seg_of(prims[i].type) == k)
+                    fast.push_back(make_fast(prims[i].invm, prims[i].type, prims[i].util1, prims[i].util2, i));
+        }
         seg_end[k] = (int32_t)fast.size();
     }
     int64_t fsum = 0;
@@ -624,8 +681,12 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
         }
         bvc_run_off[nbv] = (int32_t)bvc_runs.size();
     }
+    // the fast VALU scan tests world z-plane rings with kFlopsZRing flops each
+    // (and one reciprocal per ray) instead of the reference formula's
+    const int64_t fzring = n_zring ? kFlopsZRingRay - (int64_t)n_zring * (flops_per_test(RF_DISC) - kFlopsZRing) : 0;
     s->flops_per_ray = fsum;
-    s->flops_per_ray_nonstack = fnonstack;
+    s->flops_per_ray_fast = fsum + fzring;
+    s->flops_per_ray_nonstack = fnonstack + fzring;
     s->flops_per_ray_cnonstack = fcnonstack;
     s->n_nonstack = n - (int32_t)((fsum - fnonstack) / flops_per_test(RF_CYLINDER));
     s->n_cnonstack = n - (int32_t)((fsum - fcnonstack) / flops_per_test(RF_CYLINDER));
@@ -670,7 +731,7 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     GeomDesc &d = s->desc;
     d.prims = s->d_prims; d.fast = s->d_fast; d.fast_orig = s->d_fast_orig;
     d.bvs = s->d_bvs; d.fast_bvs = s->d_fast_bvs; d.bv_meta = s->d_bv_meta;
-    d.hits = s->d_hits; d.n = n; d.n_bvs = nbv; d.geometry_id = 0;
+    d.hits = s->d_hits; d.n = n; d.n_bvs = nbv; d.geometry_id = 0; d.zring_end = zring_end;
     for (int k = 0; k < N_SEG; ++k) { d.seg_begin[k] = seg_begin[k]; d.seg_end[k] = seg_end[k]; }
     d.quads = s->d_quads; d.pos2orig = s->d_pos2orig; d.orig2pos = s->d_orig2pos;
     for (int k = 0; k < N_SEG; ++k) { d.qseg_begin[k] = qseg_begin[k]; d.qseg_end[k] = qseg_end[k]; }
@@ -1118,8 +1179,11 @@ static void account(const rf_ctx *ctx, const rf_scene *s, const rf_solve_opts *o
         // band stacks add their evaluations and flops from the device
         // counters (stats[4], stats[3]); every other primitive is one test
         const bool st = uses_stacks(ctx, o);
+        int nr, block, mfma;
+        pick_shape(ctx, o, nr, block, mfma);
+        const bool fast_valu = o->math == RF_MATH_FAST && !mfma;   // scan_fast: world z-plane rings
         t.tests += rays * (!st ? s->n : (o->math == RF_MATH_FAST ? s->n_nonstack : s->n_cnonstack));
-        const int64_t f = !st ? s->flops_per_ray
+        const int64_t f = !st ? (fast_valu ? s->flops_per_ray_fast : s->flops_per_ray)
                           : (o->math == RF_MATH_FAST ? s->flops_per_ray_nonstack : s->flops_per_ray_cnonstack);
         t.flops += rays * f;
     }

@@ -66,6 +66,52 @@ def test_fast_forms_and_shapes_identical(ctx, form, rpt):
     assert np.array_equal(got.hits, base.hits)
 
 
+def _ring_zoo_xml() -> str:
+    """Discs and annuli inside a cylinder wall (r 3, z 0-4): four whose plane is
+    z = const and whose x-y is a similarity of world x-y (off-centre, rotated
+    about z, flipped by 180 degrees about x: the fast scan tests these in world
+    coordinates), one tilted disc and one elliptical disc (the object-frame
+    test)."""
+    P = scenes._prim
+    body = "".join([
+        P("cylinderSurface", bounding=True, small=1.0, scale=(3.0, 3.0, 4.0)),
+        P("disc", scale=(0.8, 0.8, 1.0), translate=(0.5, -0.3, 1.0)),
+        P("annulus", small=0.3, large=1.2, rotate=(37.0, (0.0, 0.0, 1.0)), translate=(-0.7, 0.4, 2.5)),
+        P("disc", scale=(1.5, 1.5, 1.0), rotate=scenes.FLIP_X, translate=(0.0, 0.0, 3.5)),
+        P("annulus", small=0.5, large=2.0, rotate=scenes.FLIP_X, translate=(0.2, 0.1, 0.2)),
+        P("disc", scale=(0.6, 0.6, 1.0), rotate=(20.0, (1.0, 0.0, 0.0)), translate=(1.0, 1.0, 2.0)),
+        P("disc", scale=(1.0, 0.5, 1.0), translate=(-1.0, -1.0, 1.5)),
+    ])
+    return scenes._doc("discs and annuli, world z-planes and others", body)
+
+
+def test_world_plane_rings_ray_for_ray(ctx, oracle):
+    """The fast scan's world-coordinate test of z-plane discs and annuli
+    against the FP64 oracle's object-frame test (RayFactorGPU.cl:126-187),
+    ray for ray, from every emitter; and the flops it reports: 11 per ray per
+    world z-plane ring plus one reciprocal per ray, against the reference
+    formula's 42 in the MFMA form, which tests every ring in its object
+    frame."""
+    sc = scenes.parse_xml(_ring_zoo_xml())
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    diff = total = zr = 0
+    for e in range(sc.n):
+        a = rf.trace(ctx, dev, e, 0, 8192, math="fast")
+        b = oracle.trace(sc.prims, None, e, 0, 8192, mode="fp64")
+        assert (a != b).mean() < 1e-3, e
+        zr += int(np.isin(b, [1, 2, 3, 4]).sum())
+        diff += int((a != b).sum())
+        total += a.size
+    print(f"[rings] {diff} of {total} rays differ from the FP64 oracle; {zr} hit a z-plane ring", flush=True)
+    assert zr > 0.05 * total                # the z-plane rings are in the rays' way
+    assert diff <= 1e-4 * total
+    kw = dict(math="fast", rng="seeded", rays_per_emitter=1 << 14, form="persistent", per_test=1)
+    v = rf.solve(ctx, dev, transform="valu", **kw).timing
+    m = rf.solve(ctx, dev, transform="mfma", **kw).timing
+    assert v["rays"] == m["rays"]
+    assert m["flops"] - v["flops"] == v["rays"] * (4 * (42 - 11) - 1)
+
+
 def test_fast_bv_matches_fp64_bv(ctx):
     sc = scenes.parse_xml(scenes.furnace_xml(with_bv=True))
     dev = rf.DeviceScene(ctx, sc.prims, sc.bvs)
