@@ -30,12 +30,31 @@ struct BvMeta {
 // is then (end - begin - 1) / (last offset - first offset), 0 if they are
 // equal (the search's first guess assumes even spacing).  Other runs
 // (stack = 0) are scanned one test at a time.
+//
+// Analytic band positions (fast form; an_margin > 0 enables them): a searched
+// stack with no x or y term in its z row whose offsets lie within an_margin's
+// deviation part of the progression r0 + k*delta (k = position - begin,
+// |delta| >= 1, so the bands do not overlap).  The root at z offset Z from
+// band `begin` then lies at position x = -Z/delta = Z * an_ninv (evaluated in
+// FP64), and a position is taken without reading any band record when x is
+// far enough from a band boundary that the per-band FP32 tests cannot decide
+// otherwise (margin: tau + an_margin + 2^-23 (|c_z t| + 4.1) in z units).
+// an_ainv = |an_ninv| in FP32; the caller index at position p is
+// orig0 + (p - begin) * ostep, or the band record's when ostep = 0.
 struct CylRun {
     int32_t begin;
     int32_t end;
     int32_t stack;
     float inv;
+    float r0;
+    float an_margin;
+    float an_ainv;
+    int32_t orig0;
+    double an_ninv;
+    int32_t ostep;
+    int32_t pad;
 };
+static_assert(sizeof(CylRun) == 48, "CylRun layout");
 
 // Per-position band record of a stack: the z-row offset and the primitive's
 // index in the caller's order.

@@ -1218,11 +1218,29 @@ __device__ __forceinline__ void stack_search_sel(bool asc, const FastPrim &h, co
 // Without COUNT nothing is counted (fl and ev are left alone: the launch
 // reports no counts).  BL: the search reads the band records from
 // this LDS copy when it is set (positions [BL.base, BL.base + n)).
+//
+// Analytic positions (run.an_margin > 0, axial z row; CylRun): before the
+// search, each ray slot's roots are placed in the progression of band offsets
+// in FP64.  With Z* = c_z t + r[10] s_z + r[11] the exact z of a root on a
+// band and zof its FP32 evaluation (two roundings), |zof - Z*| <= E =
+// 2^-24 (|c_z t| + 4.01) on every band where |Z*| <= 2.  Let dc be the distance
+// of x from the nearest position kc in [0, n) and delta the model spacing
+// (|delta| >= 1), dev the offsets' deviation from it, and
+// m = tau + dev + 2E + (FP64 and compare rounding).  Then
+//  - |delta| dc <= 1/2 - m: band kc has |zof| < 1/2 - tau (on it, no rim flag)
+//    and every other band |zof| > 1/2 + tau (off it, no flag);
+//  - |delta| dc >= 1/2 + m: every band has |zof| > 1/2 + tau;
+// so the searched bands would decide exactly this.  A slot whose roots are all
+// so decided (and whose tangent point is not flagged) enters the running
+// minimum at once and leaves the search (its roots become NaN there); the
+// search runs, for the other slots only, when any lane has one.
 template <int NR, bool REFINE, bool COUNT = true, bool BT = false>
-__device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, int b, int e, int mode, float inv,
+__device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, const CylRun &run, int mode,
                                            const Ray32 (&R)[NR], float (&tb)[NR], int (&ib)[NR], Pending &pd,
                                            uint32_t &fl, uint32_t &ev, uint32_t vmask, BandsLds BL = BandsLds{})
 {
+    const int b = run.begin, e = run.end;
+    const float inv = run.inv;
     const FastPrim hs = ldc(F + b);
     const FastPrim h = vcopy(hs);
     // wave-uniform choice of the band loop: integer tests on the scalar copy (+-0)
@@ -1259,21 +1277,84 @@ __device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, i
         I2[i] = -1;
     }
     const bool wtan = REFINE && __builtin_amdgcn_ballot_w64(tan);
-    const bool wroot1 = __builtin_amdgcn_ballot_w64(root1);
+    bool wroot1 = __builtin_amdgcn_ballot_w64(root1);
     if (NR <= 2 && mode >= 2) {
         // positions, e = none
 #pragma unroll
         for (int i = 0; i < NR; ++i) { I1[i] = e; I2[i] = e; }
         // z evaluations of this stack: the only counter inside the search
         uint32_t zs = 0;
-        const float r0 = ldc(BR + b).r11;
+        const float r0 = run.r0;
+        const auto orig_at = [&](int k) {
+            if (run.ostep) return run.orig0 + (k - b) * run.ostep;
+            return BT ? BL[k].orig : ldc(BR + k).orig;
+        };
+        uint32_t smask = 0;       // ray slots placed analytically
+        bool rest = true;         // some lane has a slot left for the search
+        if (axial && run.an_margin > 0.0f) {
+            const float mfix = tau + run.an_margin + 0x1.1p-21f;   // 2^-23 x 4.25
+            const float ha = 0.5f * run.an_ainv;
+            const double nm1 = (double)(e - b - 1);
+            bool left = false;
+#pragma unroll
+            for (int i = 0; i < NR; ++i) {
+                const double P = __builtin_fma((double)h.r[10], (double)R[i].sz, (double)r0);
+                const double czd = (double)CZ[i];
+                // position of root t (e: none); sure = the decision is certain
+                const auto place = [&](float t, bool &sure) {
+                    const double x = __builtin_fma(czd, (double)t, P) * run.an_ninv;
+                    const double k = __builtin_fmin(__builtin_fmax(__builtin_rint(x), 0.0), nm1);
+                    const float dc = (float)__builtin_fabs(x - k);
+                    const float m = __builtin_fmaf(0x1.002p-23f, __builtin_fabsf(CZ[i] * t), mfix);
+                    const bool on = dc <= __builtin_fmaf(-m, run.an_ainv, ha);
+                    sure = on || !(dc < __builtin_fmaf(m, run.an_ainv, ha));
+                    return on ? b + (int)k : e;
+                };
+                bool s2, s1 = true;
+                const int k2 = place(T2[i], s2);
+                int k1 = e;
+                if (wroot1) k1 = place(T1[i], s1);
+                const bool sure = s1 && s2 && !(TM[i] == TM[i]);
+                if (sure) {
+                    const bool h1 = k1 < e;
+                    const float c = h1 ? T1[i] : (k2 < e ? T2[i] : finf());
+                    if (c < tb[i]) { tb[i] = c; ib[i] = orig_at(h1 ? k1 : k2); }
+                    T1[i] = __builtin_nanf("");
+                    T2[i] = __builtin_nanf("");
+                    smask |= 1u << i;
+                }
+                left |= !sure;
+            }
+            if constexpr (COUNT) {
+                // one position per root per slot: the z origin row, z at the root and the scale
+                const uint32_t q = (uint32_t)__builtin_popcount(vmask & smask) * (wroot1 ? 2u : 1u);
+                fl += q * (kFlopsBandZ0Axial + kFlopsBandRoot + 1u);
+                ev += q;
+            }
+            rest = __builtin_amdgcn_ballot_w64(left);
+            if (rest) {
+                bool r1 = false;
+#pragma unroll
+                for (int i = 0; i < NR; ++i) r1 |= T1[i] == T1[i];
+                wroot1 = __builtin_amdgcn_ballot_w64(r1);
+            }
+        }
+        if (!rest) {
+            if constexpr (COUNT) {
+                const uint32_t q = (uint32_t)__builtin_popcount(vmask);
+                fl += q * kFlopsStack;
+                ev += q;
+            }
+            return;
+        }
+        const uint32_t xmask = vmask & ~smask;
         const auto search = [&](const auto &src) {
             if (axial)
                 stack_search_sel<NR, REFINE, true, COUNT>(mode == 2, h, src, b, e, r0, inv, R, T1, T2, TM, CZ, I1, I2, lo,
-                                                          hi, wroot1, wtan, pd, zs, vmask);
+                                                          hi, wroot1, wtan, pd, zs, xmask);
             else
                 stack_search_sel<NR, REFINE, false, COUNT>(mode == 2, h, src, b, e, r0, inv, R, T1, T2, TM, CZ, I1, I2, lo,
-                                                           hi, wroot1, wtan, pd, zs, vmask);
+                                                           hi, wroot1, wtan, pd, zs, xmask);
         };
         if constexpr (BT) search(BL);
         else search(BandsGlobal{BR});
@@ -1435,8 +1516,8 @@ __device__ __forceinline__ void scan_fast(CPTR(GeomDesc) G, const Ray32 (&R)[NR]
         CylRun run{G->seg_begin[SEG_CYL], G->seg_end[SEG_CYL], 0, 0};
         if (stacks) run = ldc(cst(G->cyl_runs) + j);
         if (run.stack)
-            stack_loop<NR, REFINE, COUNT, BT>(F, cst(G->bands), run.begin, run.end, stacks >= 2 ? run.stack : 1,
-                                              run.inv, R, tb, ib, pd, fl, ev, vmask, BL);
+            stack_loop<NR, REFINE, COUNT, BT>(F, cst(G->bands), run, stacks >= 2 ? run.stack : 1, R, tb, ib, pd, fl,
+                                              ev, vmask, BL);
         else
             seg_loop<SEG_CYL, NR, REFINE, false>(F, run.begin, run.end, R, tb, ib, pd);
     }
@@ -2254,16 +2335,15 @@ __device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc)
                         int (&ib0)[2] = *reinterpret_cast<int (*)[2]>(&ib[0]);
                         int (&ib1)[2] = *reinterpret_cast<int (*)[2]>(&ib[2]);
                         pd.stack0 = pd.count;
-                        stack_loop<2, REFINE>(FC, cst(G->bvc_bands), run.begin, run.end, run.stack, run.inv, R0,
-                                              tb0, ib0, pd, csf, cev, vmask & 3u);
+                        stack_loop<2, REFINE>(FC, cst(G->bvc_bands), run, run.stack, R0, tb0, ib0, pd, csf, cev,
+                                              vmask & 3u);
                         pd.slot0 = 2;
-                        stack_loop<2, REFINE>(FC, cst(G->bvc_bands), run.begin, run.end, run.stack, run.inv, R1,
-                                              tb1, ib1, pd, csf, cev, vmask >> 2);
+                        stack_loop<2, REFINE>(FC, cst(G->bvc_bands), run, run.stack, R1, tb1, ib1, pd, csf, cev,
+                                              vmask >> 2);
                         pd.slot0 = 0;
                         pd.stack0 = -1;
                     } else if (run.stack)
-                        stack_loop<BNR, REFINE>(FC, cst(G->bvc_bands), run.begin, run.end, 1, run.inv, R, tb, ib, pd,
-                                                csf, cev, vmask);
+                        stack_loop<BNR, REFINE>(FC, cst(G->bvc_bands), run, 1, R, tb, ib, pd, csf, cev, vmask);
                     else
                         seg_loop<SEG_CYL, BNR, REFINE, false>(FC, run.begin, run.end, R, tb, ib, pd);
                 }

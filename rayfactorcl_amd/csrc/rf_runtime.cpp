@@ -439,7 +439,50 @@ static CylRun stack_run(int32_t p, int32_t n, Off off)
     if (!up && !down) return CylRun{p, p + n, 1, 0.f};
     const double span = (double)off(n - 1) - (double)off(0);
     const float inv = span != 0.0 ? (float)((double)(n - 1) / span) : 0.f;
-    return CylRun{p, p + n, up ? 2 : 3, std::isfinite(inv) ? inv : 0.f};
+    CylRun r{p, p + n, up ? 2 : 3, std::isfinite(inv) ? inv : 0.f};
+    r.r0 = off(0);
+    return r;
+}
+
+static uint32_t float_bits(float x)
+{
+    uint32_t u;
+    std::memcpy(&u, &x, sizeof u);
+    return u;
+}
+
+// Analytic band positions of a searched stack (CylRun, fast form): the model
+// progression runs from the first offset with the end-to-end mean spacing,
+// raised to |delta| = 1 when rounding left it just below (bands that abut);
+// an_margin is the largest deviation of an offset from the model plus the
+// FP64 evaluation's rounding and the FP32 threshold compares' (both far below
+// the FP32 per-band tests' own rounding, which the kernel adds per ray).
+// Stacks whose bands overlap (|delta| < 1), are far apart (|delta| > 64) or
+// deviate by more than 1/32 of a band keep the searched records.
+template <class Off, class Orig>
+static void analytic_positions(CylRun &r, bool axial, Off off, Orig orig)
+{
+    const int32_t n = r.end - r.begin;
+    if (r.stack < 2 || !axial || n < 2) return;
+    double delta = ((double)off(n - 1) - (double)off(0)) / (double)(n - 1);
+    const double ad = std::fabs(delta);
+    if (!(ad >= 1.0 - 1e-4) || ad > 64.0) return;
+    if (ad < 1.0) delta = delta < 0 ? -1.0 : 1.0;
+    double dev = 0;
+    for (int32_t k = 0; k < n; ++k)
+        dev = std::max(dev, std::fabs((double)off(k) - ((double)off(0) + (double)k * delta)));
+    const double margin = dev + 1e-9 * (1.0 + (double)n * std::fabs(delta)) + 4e-7;
+    if (!(margin < 1.0 / 32)) return;
+    const double ninv = -1.0 / delta;
+    r.an_ninv = ninv;
+    r.an_ainv = (float)std::fabs(ninv);
+    r.an_margin = (float)(margin * (1.0 + 1e-6));
+    r.orig0 = orig(0);
+    int32_t step = n >= 2 ? orig(1) - orig(0) : 0;
+    if (step != 1 && step != -1) step = 0;
+    for (int32_t k = 1; k < n && step; ++k)
+        if (orig(k) != r.orig0 + k * step) step = 0;
+    r.ostep = step;
 }
 
 // The same over the caller's order for the compat scan: a stack is two or
@@ -484,7 +527,12 @@ static int64_t build_cyl_runs(const std::vector<FastPrim> &recs, int32_t cb, int
     while (p < ce) {
         int32_t q = stack_end(p);
         if (q - p >= 2) {
-            runs.push_back(stack_run(p, q - p, [&](int32_t k) { return recs[p + k].r[11]; }));
+            const auto off = [&](int32_t k) { return recs[p + k].r[11]; };
+            CylRun r = stack_run(p, q - p, off);
+            // the stack_loop test of an axial z row: integer tests on the bits (+-0)
+            const bool axial = ((float_bits(recs[p].r[8]) | float_bits(recs[p].r[9])) << 1) == 0u;
+            analytic_positions(r, axial, off, [&](int32_t k) { return recs[p + k].orig; });
+            runs.push_back(r);
             delta -= (q - p) * flops_per_test(RF_CYLINDER);
             p = q;
             continue;
