@@ -567,30 +567,74 @@ __device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcp
 __device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float rsq_fast(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// sin(2 pi u) and cos(2 pi u) for u a multiple of 2^-24 in [0, 1) (u01_24's
+// values): the quadrant q = rint(4u) and f = u - q/4 in [-1/8, 1/8] are exact,
+// x = 2 pi f is one rounding, and sincos_pinned's Taylor polynomials on
+// |x| <= pi/4 finish it.
+__device__ __forceinline__ void sincos_rev(float u, float *s, float *c)
+{
+    const float qf = __builtin_rintf(4.0f * u);
+    const float x = __builtin_fmaf(-0.25f, qf, u) * 0x1.921fb6p+2f;
+    const int q = (int)qf;
+    const float z = x * x;
+    float ps = __builtin_fmaf(z, 0x1.71de3ap-19f, -0x1.a01a02p-13f);
+    ps = __builtin_fmaf(z, ps, 0x1.111112p-7f);
+    ps = __builtin_fmaf(z, ps, -0x1.555556p-3f);
+    const float sn = __builtin_fmaf(x * z, ps, x);
+    float pc = __builtin_fmaf(z, -0x1.27e4fcp-22f, 0x1.a01a02p-16f);
+    pc = __builtin_fmaf(z, pc, -0x1.6c16c2p-10f);
+    pc = __builtin_fmaf(z, pc, 0x1.555556p-5f);
+    pc = __builtin_fmaf(z, pc, -0.5f);
+    const float cs = __builtin_fmaf(z, pc, 1.0f);
+    const float a = (q & 1) ? cs : sn;
+    const float b = (q & 1) ? sn : cs;
+    *s = (q & 2) ? -a : a;
+    *c = ((q + 1) & 2) ? -b : b;
+}
+
+// Emission frame of a cylinder emitter, built once on the host in FP64
+// (kind 1).  With A, B the first two rows of invm's linear part and a the
+// axis (m's third column), the world normal at point angle phi is
+// u2 (cos A + sin B) normalised and the tangent is a orthogonalised against
+// it.  When A and B are orthogonal, of equal length and both orthogonal to a
+// (no shear, a uniform cross-section scale), the normal is
+// cos na + sin nb (na, nb = sign(u2) A/|A|, sign(u2) B/|B|), the tangent the
+// unit axis ax, and the binormal (normal x tangent) cos pa + sin pb with
+// pa = na x ax, pb = nb x ax.  Kind 0: the frame is built per ray.
+struct EmitFrame {
+    int32_t kind;
+    float ax[3], na[3], nb[3], pa[3], pb[3];
+};
+
+// An emitter: its primitive and its frame (64 bytes after the 112).
+struct Emitter : rf_primitive {
+    EmitFrame f;
+};
+
 // Emission with the FP64 form's formulas, in FP32.
 __device__ __forceinline__ Ray32 emit_fast(const rf_primitive *__restrict__ p, const float r4[4])
 {
     Ray32 ray;
     float sn, cs, t, v0, v1, v2, n0, n1, n2;
-    sincos_pinned(r4[2] * M_2PI_F, &sn, &cs);
+    sincos_rev(r4[2], &sn, &cs);
     t = sqrt_fast(r4[3]);
     const float o0 = t * cs, o1 = t * sn, o2 = sqrt_fast(1.0f - r4[3]);
     const int type = p->type;
     const float u1 = p->util1, u2 = p->util2;
     if (type == RF_DISC || type == RF_ANNULUS) {
-        sincos_pinned(r4[0] * M_2PI_F, &sn, &cs);
+        sincos_rev(r4[0], &sn, &cs);
         t = (type == RF_DISC) ? sqrt_fast(r4[1]) : sqrt_fast(__builtin_fmaf(u2 - u1, r4[1], u1));
         v0 = t * sn; v1 = t * cs; v2 = 0.f;
         n0 = 0.f; n1 = 0.f; n2 = 1.f;
     } else if (type == RF_CYLINDER) {
-        sincos_pinned(r4[0] * M_2PI_F, &sn, &cs);
+        sincos_rev(r4[0], &sn, &cs);
         v0 = cs; v1 = sn; v2 = r4[1];
         n0 = cs * u2; n1 = sn * u2; n2 = 0.f;
     } else if (type == RF_FRUSTUM) {
         const float a = r4[0] * __builtin_fmaf(u1, u1, -1.0f);
         v2 = (r4[0] * (u1 + 1.0f)) * rcp_fast(sqrt_fast(1.0f + a) + 1.0f);
         const float rho = __builtin_fmaf(v2, u1 - 1.0f, 1.0f);
-        sincos_pinned(r4[1] * M_2PI_F, &sn, &cs);
+        sincos_rev(r4[1], &sn, &cs);
         v0 = rho * cs; v1 = rho * sn;
         n0 = v0 * u2; n1 = v1 * u2; n2 = ((1.0f - u1) * rho) * u2;
     } else if (type == RF_RECTANGLE) {
@@ -599,7 +643,7 @@ __device__ __forceinline__ Ray32 emit_fast(const rf_primitive *__restrict__ p, c
     } else {
         v2 = __builtin_fmaf(2.0f, r4[0], -1.0f);
         const float rho = sqrt_fast((1.0f - v2) * (1.0f + v2));
-        sincos_pinned(r4[1] * M_2PI_F, &sn, &cs);
+        sincos_rev(r4[1], &sn, &cs);
         v0 = rho * cs; v1 = rho * sn;
         n0 = v0 * u2; n1 = v1 * u2; n2 = v2 * u2;
     }
@@ -631,6 +675,36 @@ __device__ __forceinline__ Ray32 emit_fast(const rf_primitive *__restrict__ p, c
     ray.cx = __builtin_fmaf(o0, t0, __builtin_fmaf(o1, b0, o2 * w0));
     ray.cy = __builtin_fmaf(o0, t1, __builtin_fmaf(o1, b1, o2 * w1));
     ray.cz = __builtin_fmaf(o0, t2, __builtin_fmaf(o1, b2, o2 * w2));
+    return ray;
+}
+
+// The same with the emitter's frame: a cylinder with a kind-1 frame takes its
+// normal, tangent and binormal from the frame's constants (the sampling,
+// the point and the local direction are emit_fast's).
+__device__ __forceinline__ Ray32 emit_fast(const Emitter *__restrict__ e, const float r4[4])
+{
+    if (e->f.kind != 1) return emit_fast(static_cast<const rf_primitive *>(e), r4);
+    float s0, c0, s2, c2;
+    sincos_rev(r4[0], &s0, &c0);
+    sincos_rev(r4[2], &s2, &c2);
+    const float t = sqrt_fast(r4[3]);
+    const float o0 = t * c2, o1 = t * s2, o2 = sqrt_fast(1.0f - r4[3]);
+    const float *m = e->m;
+    const EmitFrame &f = e->f;
+    Ray32 ray;
+    ray.sx = __builtin_fmaf(m[0], c0, __builtin_fmaf(m[1], s0, __builtin_fmaf(m[2], r4[1], m[3])));
+    ray.sy = __builtin_fmaf(m[4], c0, __builtin_fmaf(m[5], s0, __builtin_fmaf(m[6], r4[1], m[7])));
+    ray.sz = __builtin_fmaf(m[8], c0, __builtin_fmaf(m[9], s0, __builtin_fmaf(m[10], r4[1], m[11])));
+    float c[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const float u = __builtin_fmaf(o1, f.pa[j], o2 * f.na[j]);
+        const float v = __builtin_fmaf(o1, f.pb[j], o2 * f.nb[j]);
+        c[j] = __builtin_fmaf(o0, f.ax[j], __builtin_fmaf(c0, u, s0 * v));
+    }
+    ray.cx = c[0];
+    ray.cy = c[1];
+    ray.cz = c[2];
     return ray;
 }
 

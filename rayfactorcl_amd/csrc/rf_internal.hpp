@@ -116,6 +116,8 @@ struct GeomDesc {
     const int32_t *c_run_of;            // run index of each caller position
     int32_t n_c_runs;
     int32_t pad3;
+    // the caller's primitives with their emission frames (fast form)
+    const rfd::Emitter *emitters;
 };
 
 // One work unit: work items [wi0, wi0+n_wi) of one emitter of one geometry.

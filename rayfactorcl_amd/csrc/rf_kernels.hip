@@ -1395,7 +1395,7 @@ __device__ __forceinline__ void stack_loop(CPTR(FastPrim) F, CPTR(BandRec) BR, c
 // ONE: every entry belongs to one ray, merged into tb[0] / ib[0].
 // ray_of(k): unit-local index of the ray in slot k.
 template <int NR, bool ONE = false, class RayOf>
-__device__ __forceinline__ void resolve_pending(const KParams &kp, CPTR(GeomDesc) G, const rf_primitive &E,
+__device__ __forceinline__ void resolve_pending(const KParams &kp, CPTR(GeomDesc) G, const Emitter &E,
                                                 uint32_t gid, int emitter, int64_t wi0, RayOf ray_of,
                                                 float (&tb)[NR], int (&ib)[NR], Pending &pd, uint32_t &nref)
 {
@@ -1424,7 +1424,7 @@ __device__ __forceinline__ void resolve_pending(const KParams &kp, CPTR(GeomDesc
 // that slot may have been dropped, re-solving each time the list fills.
 // Segment s of the scanned record array F spans [sb[s], se[s]).
 template <int NR, class RayOf>
-__device__ __forceinline__ void finish_dropped(const KParams &kp, CPTR(GeomDesc) G, const rf_primitive &E,
+__device__ __forceinline__ void finish_dropped(const KParams &kp, CPTR(GeomDesc) G, const Emitter &E,
                                                uint32_t gid, int emitter, int64_t wi0, RayOf ray_of,
                                                CPTR(FastPrim) F, CPTR(int32_t) sb, CPTR(int32_t) se,
                                                float (&tb)[NR], int (&ib)[NR], Pending &pd, uint32_t &nref,
@@ -1537,7 +1537,7 @@ __device__ __forceinline__ void scan_fast(CPTR(GeomDesc) G, const Ray32 (&R)[NR]
 // selection of the rotation.
 template <int NR>
 __device__ __forceinline__ void emit_group(const KParams &kp, uint32_t gid, int emitter, int64_t wi0, int64_t base,
-                                           const rf_primitive &E, Ray32 (&R)[NR])
+                                           const Emitter &E, Ray32 (&R)[NR])
 {
     static_assert(NR == 1 || NR == 2 || NR % 4 == 0, "rays per thread");
     float f[4];
@@ -1725,7 +1725,7 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                                          uint32_t *bins, uint32_t *pend, uint64_t &tests, uint64_t &sflops,
                                          uint64_t &sevals, uint32_t &nref, uint32_t &nover, BandsLds BL = BandsLds{})
 {
-    const rf_primitive E = ldc(cst(G->prims) + emitter);
+    const Emitter E = ldc(cst(G->emitters) + emitter);
     const int64_t ray0 = (wi0 - kp.trace_wi0) * 4;   // absolute ray index base (trace launches)
     if constexpr (MATH == KM_FAST && !BV && !DIRECT) {
         // NR rays per thread: ray groups first, first+stride, ... of NR consecutive rays
@@ -2003,7 +2003,7 @@ __device__ __forceinline__ void quad_loop(const float *__restrict__ Q, int qb, i
 
 // FP64 stage of the MFMA form: entries (pos << 2) | set; the pair's ray is
 // (set, lane % 16), regenerated from its index.
-__device__ __forceinline__ void resolve_pending_mfma(const KParams &kp, CPTR(GeomDesc) G, const rf_primitive &E,
+__device__ __forceinline__ void resolve_pending_mfma(const KParams &kp, CPTR(GeomDesc) G, const Emitter &E,
                                                      uint32_t gid, int emitter, int64_t wi0, int64_t base,
                                                      float (&tb)[4], int (&ib)[4], Pending &pd, uint32_t &nref)
 {
@@ -2035,7 +2035,7 @@ __device__ __forceinline__ void resolve_pending_mfma(const KParams &kp, CPTR(Geo
 // component l / 16 of ray (set, l % 16) for origins ([s; 1]) and directions
 // ([c; 0]).
 __device__ __forceinline__ void mfma_rays(const KParams &kp, uint32_t gid, int emitter, int64_t wi0, int64_t rl,
-                                          const rf_primitive &E, int lane, float (&Bo)[4], float (&Bd)[4])
+                                          const Emitter &E, int lane, float (&Bo)[4], float (&Bd)[4])
 {
     float f[4], r4[4];
     workitem_floats(kp, gid, emitter, wi0 + (rl >> 2), f);
@@ -2057,7 +2057,7 @@ __device__ __forceinline__ void run_unit_mfma(const KParams &kp, CPTR(GeomDesc) 
                                               int64_t wi0, int64_t n_wi, uint32_t *bins, uint32_t *pend,
                                               uint32_t &nref, uint32_t &nover)
 {
-    const rf_primitive E = ldc(cst(G->prims) + emitter);
+    const Emitter E = ldc(cst(G->emitters) + emitter);
     const int64_t ray0 = (wi0 - kp.trace_wi0) * 4;
     const float *Q = G->quads;
     const int32_t *p2o = G->pos2orig;
@@ -2184,7 +2184,7 @@ __device__ __forceinline__ void run_unit_bvbin(const KParams &kp, CPTR(GeomDesc)
                                                uint32_t *pad, int *s_total, uint64_t &tests, uint64_t &cflops,
                                                uint32_t &nref, uint32_t &nover)
 {
-    const rf_primitive E = ldc(cst(G->prims) + emitter);
+    const Emitter E = ldc(cst(G->emitters) + emitter);
     const int64_t ray0 = (wi0 - kp.trace_wi0) * 4;
     const int64_t n_rays = n_wi * 4;
     const int nbv = G->n_bvs;

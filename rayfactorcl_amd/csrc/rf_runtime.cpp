@@ -23,6 +23,8 @@
 
 using namespace rf;
 using rfd::FastPrim;
+using rfd::EmitFrame;
+using rfd::Emitter;
 
 // ---------------------------------------------------------------------------
 // Context and scene objects
@@ -96,6 +98,7 @@ struct rf_scene {
     int32_t *d_bvc_seg = nullptr;
     CylRun *d_cyl_runs = nullptr;        // fast form: cylinder runs (band stacks)
     BandRec *d_bands = nullptr;
+    Emitter *d_emitters = nullptr;       // primitives with their emission frames
     GeomDesc *d_desc = nullptr;          // single-geometry descriptor
     GeomDesc desc{};                     // host copy
     int64_t flops_per_ray = 0;           // sum over primitives of F_type (plain kernels)
@@ -544,10 +547,47 @@ static int64_t build_cyl_runs(const std::vector<FastPrim> &recs, int32_t cb, int
     return delta;
 }
 
+// Emission frame of a primitive (EmitFrame): kind 1 for a cylinder whose
+// invm rows A, B are orthogonal, of equal length and orthogonal to its axis a
+// (m's third column), to a relative 1e-6; kind 0 (the per-ray frame) for every
+// other primitive.  Built in FP64 from the FP32 records.
+static EmitFrame emit_frame(const rf_primitive &p)
+{
+    EmitFrame f{};
+    if (p.type != RF_CYLINDER) return f;
+    const double A[3] = {p.invm[0], p.invm[1], p.invm[2]}, B[3] = {p.invm[4], p.invm[5], p.invm[6]};
+    const double a[3] = {p.m[2], p.m[6], p.m[10]};
+    const auto dot = [](const double *x, const double *y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+    const double la = std::sqrt(dot(A, A)), lb = std::sqrt(dot(B, B)), lx = std::sqrt(dot(a, a));
+    if (!(la > 0 && lb > 0 && lx > 0) || !std::isfinite(la * lb * lx)) return f;
+    const double tol = 1e-6;
+    if (std::fabs(la - lb) > tol * la || std::fabs(dot(A, B)) > tol * la * lb || std::fabs(dot(a, A)) > tol * lx * la ||
+        std::fabs(dot(a, B)) > tol * lx * lb)
+        return f;
+    if (p.util2 == 0.0f || !std::isfinite(p.util2)) return f;
+    const double sg = p.util2 > 0 ? 1.0 : -1.0;
+    double na[3], nb[3], ax[3];
+    for (int j = 0; j < 3; ++j) { na[j] = sg * A[j] / la; nb[j] = sg * B[j] / lb; ax[j] = a[j] / lx; }
+    const auto cross = [](const double *x, const double *y, double *z) {
+        z[0] = x[1] * y[2] - x[2] * y[1];
+        z[1] = x[2] * y[0] - x[0] * y[2];
+        z[2] = x[0] * y[1] - x[1] * y[0];
+    };
+    double pa[3], pb[3];
+    cross(na, ax, pa);
+    cross(nb, ax, pb);
+    f.kind = 1;
+    for (int j = 0; j < 3; ++j) {
+        f.ax[j] = (float)ax[j]; f.na[j] = (float)na[j]; f.nb[j] = (float)nb[j];
+        f.pa[j] = (float)pa[j]; f.pb[j] = (float)pb[j];
+    }
+    return f;
+}
+
 extern "C" int64_t rf_scene_footprint(int32_t n, int32_t nbv)
 {
     if (n < 0 || nbv < 0) return -1;
-    return (int64_t)n * (sizeof(rf_primitive) + 2 * sizeof(FastPrim) + 64 + 8 + 2 * sizeof(BandRec) +
+    return (int64_t)n * (sizeof(rf_primitive) + sizeof(Emitter) + 2 * sizeof(FastPrim) + 64 + 8 + 2 * sizeof(BandRec) +
                          2 * sizeof(CylRun) + 4) + 9 * 256 +
            (int64_t)nbv * (sizeof(rf_bounding_volume) + sizeof(FastPrim) + sizeof(BvMeta) + (N_SEG + 1) * 4 +
                            4 + 8) +
@@ -559,7 +599,7 @@ static void free_scene_mem(rf_scene *s)
 {
     (void)hipFree(s->d_quads); (void)hipFree(s->d_pos2orig); (void)hipFree(s->d_orig2pos);
     (void)hipFree(s->d_bvc_fast); (void)hipFree(s->d_bvc_seg);
-    (void)hipFree(s->d_cyl_runs); (void)hipFree(s->d_bands);
+    (void)hipFree(s->d_cyl_runs); (void)hipFree(s->d_bands); (void)hipFree(s->d_emitters);
     (void)hipFree(s->d_c_runs); (void)hipFree(s->d_c_bands); (void)hipFree(s->d_c_run_of);
     (void)hipFree(s->d_bvc_runs); (void)hipFree(s->d_bvc_run_off); (void)hipFree(s->d_bvc_bands);
     (void)hipFree(s->d_bv_cflops);
@@ -766,6 +806,12 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
         for (int32_t k = c_runs[j].begin; k < c_runs[j].end; ++k) c_run_of[k] = (int32_t)j;
     up((void **)&s->d_c_run_of, c_run_of.data(), c_run_of.size() * sizeof(int32_t));
     up((void **)&s->d_bands, bands.data(), bands.size() * sizeof(BandRec));
+    std::vector<Emitter> emitters(n);
+    for (int32_t i = 0; i < n; ++i) {
+        std::memcpy(static_cast<rf_primitive *>(&emitters[i]), &prims[i], sizeof(rf_primitive));
+        emitters[i].f = emit_frame(prims[i]);
+    }
+    up((void **)&s->d_emitters, emitters.data(), emitters.size() * sizeof(Emitter));
     if (binned) {
         up((void **)&s->d_bvc_runs, bvc_runs.data(), bvc_runs.size() * sizeof(CylRun));
         up((void **)&s->d_bvc_run_off, bvc_run_off.data(), bvc_run_off.size() * sizeof(int32_t));
@@ -792,6 +838,7 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     d.bvc_runs = s->d_bvc_runs; d.bvc_run_off = s->d_bvc_run_off;
     d.bvc_bands = s->d_bvc_bands ? s->d_bvc_bands + 1 : nullptr;
     d.bv_cflops = s->d_bv_cflops;
+    d.emitters = s->d_emitters;
     up((void **)&s->d_desc, &d, sizeof d);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
