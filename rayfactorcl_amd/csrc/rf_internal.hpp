@@ -185,6 +185,8 @@ struct LaunchCfg {
 hipError_t launch_solve(const LaunchCfg &cfg, const KParams &kp, hipStream_t stream);
 hipError_t launch_fill_u32(unsigned int *p, unsigned int value, size_t count, hipStream_t stream);
 hipError_t launch_fill_u64(unsigned long long *p, size_t count, hipStream_t stream);
+hipError_t launch_fill_solve(const GeomDesc *geoms, int n_geoms, size_t cells, unsigned long long *stats,
+                             unsigned int *queue_head, hipStream_t stream);
 hipError_t launch_normalise(const unsigned long long *hits, double *vf, size_t n2, double rays, hipStream_t stream);
 hipError_t launch_selftest_math(unsigned long long *counts, unsigned int *first, hipStream_t stream);
 

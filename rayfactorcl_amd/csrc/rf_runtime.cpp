@@ -1398,10 +1398,18 @@ static hipError_t enqueue_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n
                                      const KParams &kp)
 {
     hipError_t e = hipSuccess;
-    for (int g = 0; g < n_geoms && e == hipSuccess; ++g)
-        e = launch_fill_u64(scenes[g]->d_hits, (size_t)scenes[g]->n * scenes[g]->n, ctx->stream);
-    if (e == hipSuccess) e = launch_fill_u64(ctx->d_stats, kNStats, ctx->stream);
-    if (e == hipSuccess && kp.queue_head) e = launch_fill_u32(kp.queue_head, 0u, 1, ctx->stream);
+    if (kp.geoms) {
+        // one launch: the uploaded descriptors kp.geoms hold each scene's tally (d_hits)
+        size_t cells = 0;
+        for (int g = 0; g < n_geoms; ++g) cells += (size_t)scenes[g]->n * scenes[g]->n;
+        e = launch_fill_solve(kp.geoms, n_geoms, cells, ctx->d_stats, kp.queue_head, ctx->stream);
+    } else {
+        // an empty solve uploads no descriptors (prepare_persistent): fill each scene's tally
+        for (int g = 0; g < n_geoms && e == hipSuccess; ++g)
+            e = launch_fill_u64(scenes[g]->d_hits, (size_t)scenes[g]->n * scenes[g]->n, ctx->stream);
+        if (e == hipSuccess) e = launch_fill_u64(ctx->d_stats, kNStats, ctx->stream);
+        if (e == hipSuccess && kp.queue_head) e = launch_fill_u32(kp.queue_head, 0u, 1, ctx->stream);
+    }
     if (e == hipSuccess && kp.n_units > 0) e = launch_solve(c, kp, ctx->stream);
     return e;
 }

@@ -155,7 +155,7 @@ class DeviceScene:
 
 @dataclass
 class SolveResult:
-    hits: np.ndarray        # (n, n) int64, row = emitter
+    hits: np.ndarray | None  # (n, n) int64, row = emitter (None: batch_solve without readback)
     rays: np.ndarray        # (n,) int64, rays traced per emitter
     timing: dict
 
@@ -182,7 +182,9 @@ def batch_solve(ctx: Context, scenes: list[DeviceScene], opts: SolveOpts | None 
     o = opts if opts is not None else make_opts(**kw)
     n_g = len(scenes)
     sc = (c_void_p * n_g)(*[s.handle.value for s in scenes])
-    hits = [np.zeros((s.n, s.n), dtype=np.int64) for s in scenes]
+    # without readback no host tally is allocated (hits of the results are None):
+    # zeroing n x n int64 arrays for a whole sweep costs tens of ms per call
+    hits = [np.zeros((s.n, s.n), dtype=np.int64) if readback else None for s in scenes]
     rays = [np.zeros(s.n, dtype=np.int64) for s in scenes]
     hp = (c_void_p * n_g)(*[h.ctypes.data if readback else None for h in hits])
     rp = (c_void_p * n_g)(*[r.ctypes.data for r in rays])
