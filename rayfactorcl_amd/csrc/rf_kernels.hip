@@ -2429,14 +2429,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REFINE
 // empty (all waves reach the same exit).
 // Four rays per thread with the FP64 stage fits 96 VGPRs (five waves per
 // SIMD) when asked to; left alone it takes a few more and drops to four.
-// The compat scan (four rays per visit) is held to the same five waves.
+// The compat scan (four rays per visit) is held to the same five waves.  Two
+// rays per thread, uncounted, run at six (80 VGPRs, a few spilled outside the
+// analytic band path): 3% faster on the 256-geometry sweep than five, 2.5%
+// faster than seven; the counted variant stays at five (it spills heavily at
+// six).
 // COUNT: the fast band search counts its evaluations (off for timed launches:
 // the counters cost ~2%, and a launch counted once gives the same counts).
 // BT: each unit's band records (its geometry's cylinder segment) are copied
 // to LDS, where the searched stacks read them.
 template <int MATH, bool BV, bool TRACE, int NR, bool REFINE, bool COUNT = true, bool BT = false>
 __global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu((NR == 4 || NR == 2 || (MATH == KM_COMPAT && !BV)) ? 5 : (MATH == KM_FP64 ? 4 : 1)))) k_queue(KParams kp)
+__attribute__((amdgpu_waves_per_eu((NR == 2 && !COUNT) ? 6 : (NR == 4 || (MATH == KM_COMPAT && !BV)) ? 5 : (MATH == KM_FP64 ? 4 : 1)))) k_queue(KParams kp)
 {
     // dynamic LDS: [bins: kp.bins x u32][pending re-solves: PEND_K x blockDim x u32]
     //              [BT: band records, 8 B each, from the next 8-byte boundary]
