@@ -312,3 +312,47 @@ def test_bv_binned_empty_and_overlapping_children(ctx, oracle):
     c = rf.solve(ctx, dev, math="compat", use_bvs=True, rays_per_emitter=256)
     want, _ = oracle.solve(sc.prims, bvs, n_wi=64, mode="compat", use_bvs=True)
     assert np.array_equal(c.hits, want)
+
+
+def _cylinder_zoo_xml() -> str:
+    """Cylinder emitters inside a closed wall: uniform cross-sections (the
+    emitter frame built once on the host: upright, rotated about z, tilted),
+    an elliptical one and a sheared one (the frame built per ray), and a
+    frustum."""
+    P = scenes._prim
+    body = "".join([
+        P("cylinderSurface", bounding=True, small=1.0, scale=(3.0, 3.0, 4.0)),
+        P("disc", scale=(3.0, 3.0, 1.0)),
+        P("disc", scale=(3.0, 3.0, 1.0), rotate=scenes.FLIP_X, translate=(0.0, 0.0, 4.0)),
+        P("cylinderSurface", small=1.0, scale=(0.4, 0.4, 1.0), translate=(1.0, 0.5, 0.5)),
+        P("cylinderSurface", small=1.0, scale=(0.3, 0.3, 0.8), rotate=(37.0, (0.0, 0.0, 1.0)),
+          translate=(-1.2, 0.3, 1.5)),
+        P("cylinderSurface", small=1.0, scale=(0.3, 0.3, 1.0), rotate=(30.0, (1.0, 0.0, 0.0)),
+          translate=(0.5, -1.2, 1.2)),
+        P("cylinderSurface", small=1.0, scale=(0.5, 0.25, 1.0), translate=(-0.8, -1.0, 2.2)),
+        P("cylinderSurface", small=1.0, scale=(0.3, 0.3, 1.0), rotate=(25.0, (1.0, 1.0, 0.0)),
+          translate=(1.5, 1.5, 2.5)),
+        P("frustum", small=0.5, scale=(0.5, 0.5, 0.8), translate=(0.0, 1.6, 0.3)),
+    ])
+    return scenes._doc("cylinder emitters: host-built and per-ray frames", body)
+
+
+def test_cylinder_emitter_frames_ray_for_ray(ctx, oracle):
+    """Cylinder emitters whose frame is built once on the host (uniform
+    cross-section, any rotation) and per ray (elliptical, the rest) against
+    the FP64 oracle's per-ray Gram-Schmidt frame (RayFactorGPU.cl:347-365 for
+    the sampling), ray for ray: the same rays up to FP32 rounding, so almost
+    every ray lands on the same primitive from every emitter."""
+    sc = scenes.parse_xml(_cylinder_zoo_xml())
+    dev = rf.DeviceScene(ctx, sc.prims, None)
+    diff = total = 0
+    for e in range(sc.n):
+        a = rf.trace(ctx, dev, e, 0, 8192, math="fast")
+        b = oracle.trace(sc.prims, None, e, 0, 8192, mode="fp64")
+        assert (a != b).mean() < 1e-3, e
+        assert (b >= 0).mean() > 0.99, e          # the enclosure is closed
+        diff += int((a != b).sum())
+        total += a.size
+    print(f"[frames] {diff} of {total} rays differ from the FP64 oracle", flush=True)
+    assert diff <= 2e-4 * total
+    dev.close()

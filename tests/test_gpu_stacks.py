@@ -381,3 +381,47 @@ def test_band_tile_and_counters_keep_tallies(ctx, rpt):
     assert not t["shape_flags"] & L.RF_SHAPE_BAND_TILE
     for d in dev:
         d.close()
+
+
+def _shift_z(prims, dz):
+    """The scene moved by dz along world z: m's z translation and invm's
+    offsets, rounded to FP32 like any loaded scene."""
+    p = prims.copy()
+    m = p["m"].astype(np.float64)
+    iv = p["invm"].astype(np.float64)
+    m[:, 11] += dz
+    for r in range(3):
+        iv[:, 4 * r + 3] -= iv[:, 4 * r + 2] * dz
+    p["m"] = m.astype(np.float32)
+    p["invm"] = iv.astype(np.float32)
+    return p
+
+
+@pytest.mark.parametrize("dz, bands", [(0.0, 500), (900.0, 180), (-2.5e4, 60)])
+def test_analytic_positions_equal_per_test(ctx, dz, bands):
+    """Analytic band positions: each root of an evenly spaced axial stack is
+    placed in FP64 and taken without reading a band record only where the
+    margin (tau, the offsets' deviation from their progression, and the FP32
+    rounding of the per-band z) proves every per-band test agrees.  Far from
+    the origin the offsets are large, their rounding makes the progression
+    uneven and the margin wide, so more slots fall back to the probe search;
+    either way the tallies, the re-solve set and the traces are the per-test
+    form's."""
+    sc = scenes.parse_xml(scenes.furnace_xml(wall_bands=bands, preform_bands=max(12, bands // 3)))
+    prims = _shift_z(sc.prims, dz)
+    dev = rf.DeviceScene(ctx, prims, None)
+    kw = dict(math="fast", rng="seeded", rays_per_emitter=1 << 15, form="persistent", rays_per_thread=2,
+              block_size=256)
+    a = rf.solve(ctx, dev, per_test=True, **kw)
+    b = rf.solve(ctx, dev, per_test=0, count_tests=True, **kw)
+    _same(a, b)
+    for e in (0, bands // 2, len(prims) - 1):
+        ta = rf.trace(ctx, dev, e, 0, 4096, math="fast", rng="seeded", per_test=True)
+        tb = rf.trace(ctx, dev, e, 0, 4096, math="fast", rng="seeded", per_test=0)
+        assert np.array_equal(ta, tb), e
+    # fewer evaluations than the per-test form's n per ray (far from the
+    # origin more slots go through the search's probes)
+    assert 0 < b.timing["tests"] < 0.25 * a.timing["tests"]
+    print(f"[analytic] dz={dz} bands={bands}: {b.timing['tests'] / b.rays.sum():.2f} evaluations per ray "
+          f"(per test {a.timing['tests'] / a.rays.sum():.0f})", flush=True)
+    dev.close()
