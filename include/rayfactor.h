@@ -403,6 +403,14 @@ int32_t rf_normalise(int32_t n, const int64_t *hits, const int64_t *rays, double
  * first[k] = lowest mismatching input bits (0xffffffff if none). */
 int32_t rf_selftest_math(rf_ctx *ctx, uint64_t mismatches[4], uint32_t first[4]);
 
+/* Largest absolute error, against FP64, of the fast form's sin and cos of
+ * 2 pi u (v_sin_f32 / v_cos_f32, argument in revolutions) over every
+ * emission uniform u = k 2^-24, k < 2^24: max_err[0] sin, max_err[1] cos.
+ * No reference counterpart (RayFactorGPU.cl:114 calls the vendor's sincos,
+ * whose accuracy is the runtime's): a device check of the builtin the fast
+ * emission uses. */
+int32_t rf_selftest_sincos(rf_ctx *ctx, double max_err[2]);
+
 /* Surface area of each primitive: its object-space shape (unit cylinder
  * z in [0,1], frustum radius 1+(util1-1)z, unit sphere, unit disc, annulus
  * r^2 in [util1, util2], rectangle [-1,1]^2) under the linear part of m,

@@ -141,6 +141,7 @@ SIGNATURES = {
     "rf_write_output": (c_int32, [c_char_p, c_int32, _P, _P]),
     "rf_normalise": (c_int32, [c_int32, _P, _P, _P]),
     "rf_selftest_math": (c_int32, [_P, _P, _P]),
+    "rf_selftest_sincos": (c_int32, [_P, _P]),
     "rf_areas": (c_int32, [_P, c_int32, _P]),
     "rf_vf_check": (c_int32, [c_int32, _P, _P, c_double, _P, POINTER(VfReport)]),
 }

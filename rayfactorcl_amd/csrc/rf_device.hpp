@@ -567,29 +567,15 @@ __device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcp
 __device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float rsq_fast(float x) { return __builtin_amdgcn_rsqf(x); }
 
-// sin(2 pi u) and cos(2 pi u) for u a multiple of 2^-24 in [0, 1) (u01_24's
-// values): the quadrant q = rint(4u) and f = u - q/4 in [-1/8, 1/8] are exact,
-// x = 2 pi f is one rounding, and sincos_pinned's Taylor polynomials on
-// |x| <= pi/4 finish it.
+// sin(2 pi u) and cos(2 pi u) for the fast emission: v_sin_f32 / v_cos_f32
+// take their argument in revolutions, so u (u01_24's values in [0, 1)) needs
+// no reduction.  Largest absolute error over all 2^24 such u, against FP64:
+// 1.25e-7 for both (rf_selftest_sincos); the FP32 polynomial it replaced
+// measured 8.3e-8.
 __device__ __forceinline__ void sincos_rev(float u, float *s, float *c)
 {
-    const float qf = __builtin_rintf(4.0f * u);
-    const float x = __builtin_fmaf(-0.25f, qf, u) * 0x1.921fb6p+2f;
-    const int q = (int)qf;
-    const float z = x * x;
-    float ps = __builtin_fmaf(z, 0x1.71de3ap-19f, -0x1.a01a02p-13f);
-    ps = __builtin_fmaf(z, ps, 0x1.111112p-7f);
-    ps = __builtin_fmaf(z, ps, -0x1.555556p-3f);
-    const float sn = __builtin_fmaf(x * z, ps, x);
-    float pc = __builtin_fmaf(z, -0x1.27e4fcp-22f, 0x1.a01a02p-16f);
-    pc = __builtin_fmaf(z, pc, -0x1.6c16c2p-10f);
-    pc = __builtin_fmaf(z, pc, 0x1.555556p-5f);
-    pc = __builtin_fmaf(z, pc, -0.5f);
-    const float cs = __builtin_fmaf(z, pc, 1.0f);
-    const float a = (q & 1) ? cs : sn;
-    const float b = (q & 1) ? sn : cs;
-    *s = (q & 2) ? -a : a;
-    *c = ((q + 1) & 2) ? -b : b;
+    *s = __builtin_amdgcn_sinf(u);
+    *c = __builtin_amdgcn_cosf(u);
 }
 
 // Emission frame of a cylinder emitter, built once on the host in FP64

@@ -189,6 +189,7 @@ hipError_t launch_fill_solve(const GeomDesc *geoms, int n_geoms, size_t cells, u
                              unsigned int *queue_head, hipStream_t stream);
 hipError_t launch_normalise(const unsigned long long *hits, double *vf, size_t n2, double rays, hipStream_t stream);
 hipError_t launch_selftest_math(unsigned long long *counts, unsigned int *first, hipStream_t stream);
+hipError_t launch_selftest_sincos(unsigned int *err, hipStream_t stream);
 
 // Algorithmic FP32 flops of one band stack, same counting as flops_per_test,
 // counted on the device from the band loop a wave runs: per (ray, stack) the

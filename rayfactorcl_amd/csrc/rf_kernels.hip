@@ -2585,6 +2585,29 @@ __global__ __launch_bounds__(256) void k_selftest_math(unsigned long long *count
         if (n[k]) { atomicAdd(&counts[k], (unsigned long long)n[k]); atomicMin(&first[k], lo[k]); }
 }
 
+// Largest absolute error of the fast emission's sin and cos of 2 pi u over
+// every u = k 2^-24, k < 2^24 (u01_24's values), against FP64 sin and cos:
+// err[0] sin, err[1] cos, as the bits of non-negative floats (atomic max).
+__global__ __launch_bounds__(256) void k_selftest_sincos(unsigned int *err)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const float u = (float)k * 0x1p-24f;
+    const double ph = 6.283185307179586476925 * (double)u;
+    float s, c;
+    sincos_rev(u, &s, &c);
+    const float es = (float)__builtin_fabs((double)s - sin(ph));
+    const float ec = (float)__builtin_fabs((double)c - cos(ph));
+    atomicMax(&err[0], __float_as_uint(es));
+    atomicMax(&err[1], __float_as_uint(ec));
+}
+
+hipError_t launch_selftest_sincos(unsigned int *err, hipStream_t stream)
+{
+    // 2^24 inputs: 2^16 groups x 256 threads
+    hipLaunchKernelGGL(k_selftest_sincos, dim3(1u << 16), dim3(256), 0, stream, err);
+    return hipGetLastError();
+}
+
 hipError_t launch_selftest_math(unsigned long long *counts, unsigned int *first, hipStream_t stream)
 {
     // 2^32 inputs: 2^16 groups x 256 threads x 256 inputs

@@ -303,6 +303,30 @@ extern "C" int32_t rf_selftest_math(rf_ctx *ctx, uint64_t mismatches[4], uint32_
     return RF_OK;
 }
 
+extern "C" int32_t rf_selftest_sincos(rf_ctx *ctx, double max_err[2])
+{
+    if (!ctx || !max_err) return set_err(ctx, RF_ERR_INVALID_ARG, "NULL argument");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int rc = fan_join(ctx);
+    if (rc) return rc;
+    unsigned *d = nullptr;
+    HIPCHK(ctx, hipMalloc((void **)&d, 2 * sizeof(unsigned)));
+    unsigned h[2] = {0u, 0u};
+    hipError_t e = launch_fill_u32(d, 0u, 2, ctx->stream);
+    if (e == hipSuccess) e = launch_selftest_sincos(d, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return set_err(ctx, RF_ERR_HIP, std::string("selftest: ") + hipGetErrorString(e));
+    for (int k = 0; k < 2; ++k) {
+        float f;
+        std::memcpy(&f, &h[k], sizeof f);
+        max_err[k] = f;
+    }
+    return RF_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Scene upload
 // ---------------------------------------------------------------------------

@@ -248,6 +248,14 @@ def selftest_math(ctx: Context) -> dict:
     return {k: {"mismatches": int(m), "first": int(f)} for k, m, f in zip(names, mism, first)}
 
 
+def selftest_sincos(ctx: Context) -> tuple[float, float]:
+    """Largest absolute error of the fast emission's sin and cos of 2 pi u
+    over every u = k 2^-24 (all 2^24 emission uniforms), against FP64."""
+    err = np.zeros(2, dtype=np.float64)
+    check(lib().rf_selftest_sincos(ctx.handle, ptr(err)), "rf_selftest_sincos", ctx.handle)
+    return float(err[0]), float(err[1])
+
+
 def areas(prims: np.ndarray) -> np.ndarray:
     """Surface area of each primitive (object shape under its m transform, by quadrature)."""
     prims = np.ascontiguousarray(prims, dtype=PRIM_DTYPE)

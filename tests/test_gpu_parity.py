@@ -24,6 +24,16 @@ def test_compat_math_exhaustive(ctx):
     assert r["sqrt_c"]["mismatches"] == 0, r
 
 
+def test_fast_sincos_error_bound(ctx):
+    """The fast emission's sin and cos of 2 pi u come from v_sin_f32 /
+    v_cos_f32 (argument in revolutions).  Over all 2^24 emission uniforms
+    their largest error against FP64 is about 1 ulp at 1: at most 2^-22
+    (the sampled directions move by no more than that)."""
+    es, ec = rf.selftest_sincos(ctx)
+    print(f"[sincos] max abs error sin {es:.3e} cos {ec:.3e}", flush=True)
+    assert 0 < es <= 2.0 ** -22 and 0 < ec <= 2.0 ** -22
+
+
 @pytest.mark.parametrize("name", SCENE_NAMES)
 @pytest.mark.parametrize("math", ["compat", "fp64"])
 def test_hits_match_oracle(ctx, oracle, golden_scene, name, math):
