@@ -506,15 +506,26 @@ __device__ __forceinline__ void stack_c_search(const float (&m)[12], CPTR(BandRe
 // test.  An axial z row (invm[8] == invm[9] == 0) takes one FMA for o.sz: the
 // two dropped FMAs add a signed zero, which no range test can tell apart.
 // mode 1 walks every band; 2 and 3 (band offsets sorted, non-decreasing or
-// non-increasing) search them (stack_c_search).
+// non-increasing) search them (stack_c_search).  [b, e) is the part of the
+// run scanned (all of it, or a bounding surface's children).
+// Analytic positions (run.an_margin > 0, axial): as in stack_loop, each root
+// is first placed in the run's progression of offsets in FP64.  Band w's z
+// is (m[10] s_z + invm[11](w)) + o.cz t, two roundings, so with Z* its exact
+// value and p = o.cz t, |z - Z*| <= E = 2^-24 (|p| + 4.01) where |Z*| <= 2;
+// a root whose distance dc from the nearest position of [b, e) satisfies
+// |delta| dc <= 1/2 - m (m = dev + 2E + rounding) lies on that band with
+// z in (0, 1) and on no other, and one with |delta| dc >= 1/2 + m on none:
+// both range tests (z1 in [0, 1], z2 in (0, 1)) are decided, and the rays so
+// placed skip the search (their products become NaN there).
 // Returns the stack's algorithmic flops summed over the NR rays.
 // ev gains the stack's evaluations over the NR rays: a quadric per ray and a
-// z-range check per band looked at.
+// z-range check per band looked at (one per placed root).
 template <int NR>
-__device__ __forceinline__ uint32_t stack_c(CPTR(rf_primitive) P, CPTR(BandRec) BR, int b, int e, int mode,
-                                            float inv, const Ray32 (&ray)[NR], float (&tbest)[NR], int (&obj)[NR],
+__device__ __forceinline__ uint32_t stack_c(CPTR(rf_primitive) P, CPTR(BandRec) BR, const CylRun &run, int b, int e,
+                                            int mode, const Ray32 (&ray)[NR], float (&tbest)[NR], int (&obj)[NR],
                                             uint32_t &ev)
 {
+    const float inv = run.inv;
     const rf_primitive hs = ldc(P + b);
     float m[12];
 #pragma unroll
@@ -551,17 +562,60 @@ __device__ __forceinline__ uint32_t stack_c(CPTR(rf_primitive) P, CPTR(BandRec) 
         I2[r] = -1;
     }
     if (mode >= 2) {
-        uint32_t probes = 0;
-        if (axial) stack_c_search<NR, true>(m, BR, b, e, mode == 2, inv, ray, P1, P2, I1, I2, probes);
-        else stack_c_search<NR, false>(m, BR, b, e, mode == 2, inv, ray, P1, P2, I1, I2, probes);
+        uint32_t probes = 0, placed = 0;
+        bool sure[NR];
+        int K1[NR], K2[NR];
+        bool rest = true;
+        if (axial && run.an_margin > 0.0f) {
+            const float mfix = run.an_margin + 0x1.1p-21f;   // 2^-23 x 4.25
+            const float ha = 0.5f * run.an_ainv;
+            const double klo = (double)(b - run.begin), khi = (double)(e - 1 - run.begin);
+            bool left = false;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                // z of the root on the run's first band, less the range's centre
+                const double zb = __builtin_fma((double)m[10], (double)ray[r].sz, (double)run.r0) - 0.5;
+                const auto place = [&](float p, bool &sr) {
+                    const double x = (zb + (double)p) * run.an_ninv;
+                    const double k = __builtin_fmin(__builtin_fmax(__builtin_rint(x), klo), khi);
+                    const float dc = (float)__builtin_fabs(x - k);
+                    const float mm = __builtin_fmaf(0x1.002p-23f, __builtin_fabsf(p), mfix);
+                    const bool on = dc <= __builtin_fmaf(-mm, run.an_ainv, ha);
+                    sr = on || !(dc < __builtin_fmaf(mm, run.an_ainv, ha));
+                    return on ? run.begin + (int)k : -1;
+                };
+                bool s1, s2;
+                K1[r] = place(P1[r], s1);
+                K2[r] = place(P2[r], s2);
+                sure[r] = s1 && s2;
+                if (sure[r]) {
+                    P1[r] = qnan;
+                    P2[r] = qnan;
+                }
+                left |= !sure[r];
+            }
+            placed = 2 * NR;
+            rest = __builtin_amdgcn_ballot_w64(left);
+        } else {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) sure[r] = false;
+        }
+        if (rest) {
+            if (axial) stack_c_search<NR, true>(m, BR, b, e, mode == 2, inv, ray, P1, P2, I1, I2, probes);
+            else stack_c_search<NR, false>(m, BR, b, e, mode == 2, inv, ray, P1, P2, I1, I2, probes);
+        }
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
+            if (sure[r]) {
+                I1[r] = K1[r];
+                I2[r] = K2[r];
+            }
             const bool h1 = I1[r] >= 0;
             const float c = h1 ? T1[r] : (I2[r] >= 0 ? T2[r] : finf());
             if (c < tbest[r]) { tbest[r] = c; obj[r] = BR[min(h1 ? I1[r] : I2[r], e - 1)].orig; }
         }
-        ev += NR + probes;
-        return NR * kFlopsStackC + probes * ((axial ? kFlopsBandZ0Axial : kFlopsBandZ0) + 1u);
+        ev += NR + probes + placed;
+        return NR * kFlopsStackC + (probes + placed) * ((axial ? kFlopsBandZ0Axial : kFlopsBandZ0) + 1u);
     }
     if (axial) stack_c_bands<NR, true>(m, BR, b, e, ray, P1, P2, I1, I2);
     else stack_c_bands<NR, false>(m, BR, b, e, ray, P1, P2, I1, I2);
@@ -594,7 +648,7 @@ __device__ __forceinline__ uint32_t find_c_multi(CPTR(GeomDesc) G, const Ray32 (
         CylRun run{0, G->n, 0, 0};
         if (stacks) run = ldc(cst(G->c_runs) + j);
         if (run.stack)
-            sf += stack_c<NR>(P, cst(G->c_bands), run.begin, run.end, stacks >= 2 ? run.stack : 1, run.inv, ray, tbest,
+            sf += stack_c<NR>(P, cst(G->c_bands), run, run.begin, run.end, stacks >= 2 ? run.stack : 1, ray, tbest,
                               obj, ev);
         else
             scan_c_range<NR>(P, run.begin, run.end, ray, tbest, obj);
@@ -622,7 +676,7 @@ __device__ __forceinline__ int find_c_runs(CPTR(GeomDesc) G, int s, int e, const
         const int rb = max(run.begin, s), re = min(run.end, e);
         if (run.stack && re - rb >= 2) {
             uint32_t e1 = 0;
-            stack_c<1>(P, cst(G->c_bands), rb, re, stacks >= 2 ? run.stack : 1, run.inv, R1, tbest, obj, e1);
+            stack_c<1>(P, cst(G->c_bands), run, rb, re, stacks >= 2 ? run.stack : 1, R1, tbest, obj, e1);
             ev += e1;
         } else {
             scan_c_range<1>(P, rb, re, R1, tbest, obj);

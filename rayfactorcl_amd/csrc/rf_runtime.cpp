@@ -530,7 +530,11 @@ static int64_t build_compat_runs(const rf_primitive *prims, int32_t n, std::vect
     while (p < n) {
         int32_t q = stack_end(p);
         if (q - p >= 2) {
-            runs.push_back(stack_run(p, q - p, [&](int32_t k) { return prims[p + k].invm[11]; }));
+            const auto off = [&](int32_t k) { return prims[p + k].invm[11]; };
+            CylRun r = stack_run(p, q - p, off);
+            const bool axial = ((float_bits(prims[p].invm[8]) | float_bits(prims[p].invm[9])) << 1) == 0u;
+            analytic_positions(r, axial, off, [&](int32_t k) { return p + k; });
+            runs.push_back(r);
             delta -= (q - p) * flops_per_test(RF_CYLINDER);
             p = q;
             continue;

@@ -425,3 +425,23 @@ def test_analytic_positions_equal_per_test(ctx, dz, bands):
     print(f"[analytic] dz={dz} bands={bands}: {b.timing['tests'] / b.rays.sum():.2f} evaluations per ray "
           f"(per test {a.timing['tests'] / a.rays.sum():.0f})", flush=True)
     dev.close()
+
+
+@pytest.mark.parametrize("dz, bands", [(0.0, 500), (-2.5e4, 60)])
+def test_compat_analytic_positions_equal_per_test(ctx, oracle, dz, bands):
+    """The compat band stacks place each root analytically too (their range
+    tests z1 in [0, 1] and z2 in (0, 1) decided with a derived margin):
+    searched, per test and the CPU oracle's per-test scan agree bit for bit,
+    also far from the origin where the offsets' rounding widens the margin."""
+    sc = scenes.parse_xml(scenes.furnace_xml(wall_bands=bands, preform_bands=max(12, bands // 3)))
+    prims = _shift_z(sc.prims, dz)
+    dev = rf.DeviceScene(ctx, prims, None)
+    kw = dict(math="compat", rng="seeded", rays_per_emitter=1 << 14, form="persistent")
+    a = rf.solve(ctx, dev, per_test=True, **kw)
+    b = rf.solve(ctx, dev, per_test=0, **kw)
+    assert np.array_equal(a.hits, b.hits)
+    for e in (0, bands // 2, len(prims) - 1):
+        got = rf.trace(ctx, dev, e, 0, 2048, math="compat", rng="seeded")
+        want = oracle.trace(prims, None, e, 0, 2048, mode="compat", rng="seeded")
+        assert np.array_equal(got, want), e
+    dev.close()
