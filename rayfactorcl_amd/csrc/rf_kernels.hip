@@ -1604,13 +1604,22 @@ __device__ __forceinline__ void emit_group(const KParams &kp, uint32_t gid, int 
         R[0] = emit_fast(&E, r4);
     } else if constexpr (NR == 2) {
         workitem_floats(kp, gid, emitter, wi0 + (base >> 2), f);
-        // rays v = 0, 1 of the work item, or v = 2, 3
+        // rays v = 0, 1 of the work item, or v = 2, 3: the upper half takes the
+        // uniforms rotated by two more places.  Four selects on one condition,
+        // materialised, so that the compiler does not turn them into a
+        // per-lane index select for each of the eight uses.
         const bool upper = (base & 2) != 0;
+        float g[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            g[k] = upper ? f[(k + 2) & 3] : f[k];
+            asm volatile("" : "+v"(g[k]));
+        }
         float r4[4];
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) r4[j] = upper ? f[(i + j + 3) & 3] : f[(i + j + 1) & 3];
+            for (int j = 0; j < 4; ++j) r4[j] = g[(i + j + 1) & 3];
             R[i] = emit_fast(&E, r4);
         }
     } else {
