@@ -140,6 +140,13 @@ typedef struct rf_solve_opts {
                                  searched there when every scene's segment has at most this many
                                  records; 0 = the autotuner's choice (off until tuned; up to 8192
                                  records when on), -1 = off.  Same tallies either way. */
+    int32_t hierarchy;        /* RF_MATH_FAST plain scan with the VALU transform and band stacks
+                                 (per_test 0 or 2): 0 (the default) = where every scene of the solve
+                                 has at least 48 primitives outside band stacks, each type's
+                                 per-test primitives are found through a bounding-box hierarchy
+                                 (boxes the ray crosses before its best hit are visited, 1 or 2
+                                 rays per thread); -1 = every primitive tested.  Same tallies
+                                 either way (exact ties between primitives: see per_test). */
 } rf_solve_opts;
 
 typedef enum rf_transform {
@@ -185,7 +192,8 @@ enum {
     RF_SHAPE_BAND_TILE = 8,   /* band records read from an LDS copy */
     RF_SHAPE_COUNTED = 16,    /* evaluations and flops counted (always, except an uncounted band search) */
     RF_SHAPE_BV_BINNED = 32,  /* fast bounding-surface form with rays binned by surface */
-    RF_SHAPE_REFERENCE = 64   /* the reference dispatch shape (one work item per thread) */
+    RF_SHAPE_REFERENCE = 64,  /* the reference dispatch shape (one work item per thread) */
+    RF_SHAPE_HIERARCHY = 128  /* per-test primitives found through bounding-box hierarchies */
 };
 
 /* ------------------------------------------------------------------ */

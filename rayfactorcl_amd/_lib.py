@@ -46,6 +46,7 @@ RF_SHAPE_BAND_TILE = 8
 RF_SHAPE_COUNTED = 16
 RF_SHAPE_BV_BINNED = 32
 RF_SHAPE_REFERENCE = 64
+RF_SHAPE_HIERARCHY = 128
 
 MATH = {"compat": 0, "fp64": 1, "fast": 2}
 RNG = {"reference": 0, "seeded": 1}
@@ -60,6 +61,7 @@ class SolveOpts(ctypes.Structure):
         ("seed", c_uint32), ("geometry", c_uint32), ("refine", c_int32),
         ("rays_per_thread", c_int32), ("block_size", c_int32), ("transform", c_int32),
         ("refine_margin", c_float), ("per_test", c_int32), ("count_tests", c_int32), ("band_tile", c_int32),
+        ("hierarchy", c_int32),
     ]
 
 

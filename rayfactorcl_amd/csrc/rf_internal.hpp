@@ -56,6 +56,22 @@ struct CylRun {
 };
 static_assert(sizeof(CylRun) == 48, "CylRun layout");
 
+// Node of the bounding-volume hierarchy over the fast form's per-test
+// primitives (GeomDesc::bvh_root): the boxes of its two children (world
+// coordinates, padded) and, per child k, n[k] > 0: a leaf of the
+// (n[k] & 255) records bvh_prims[c[k] ..] of type segment n[k] >> 8;
+// n[k] == 0, c[k] >= 0: the node c[k]; n[k] == 0, c[k] < 0: no child (its box
+// is never tested).
+struct BvhNode {
+    float lo0[3], hi0[3], lo1[3], hi1[3];
+    int32_t c[2];
+    int32_t n[2];
+};
+static_assert(sizeof(BvhNode) == 64, "BvhNode layout");
+
+// Deepest hierarchy a traversal stack holds (BVH_STACK entries per lane).
+constexpr int BVH_STACK = 16;
+
 // Per-position band record of a stack: the z-row offset and the primitive's
 // index in the caller's order.
 struct BandRec {
@@ -118,6 +134,15 @@ struct GeomDesc {
     int32_t pad3;
     // the caller's primitives with their emission frames (fast form)
     const rfd::Emitter *emitters;
+    // fast form, scenes with many primitives outside band stacks: a
+    // hierarchy over those primitives (root node bvh_root), traversed instead
+    // of their segment scans, its leaves' records copied to bvh_prims in leaf
+    // order with FastPrim::type holding the record's traversal position;
+    // bvh_on = 0: no hierarchy (every ring may then be a world z-plane ring)
+    const BvhNode *bvh_nodes;
+    const rfd::FastPrim *bvh_prims;
+    int32_t bvh_root;
+    int32_t bvh_on;
 };
 
 // One work unit: work items [wi0, wi0+n_wi) of one emitter of one geometry.
@@ -179,6 +204,7 @@ struct LaunchCfg {
     int bvbin;       // 1: fast BV form with rays binned by bounding surface
     int count;       // fast band search: 1 = count its evaluations on the device
     int bt;          // fast band search: 1 = band records copied to LDS per unit
+    int bvh;         // fast plain form: traverse the scenes' hierarchies (every scene has them)
     size_t lds_bytes;
 };
 
@@ -213,9 +239,15 @@ constexpr uint32_t kFlopsBandD = 3;
 constexpr int64_t kFlopsZRing = 11;
 constexpr int64_t kFlopsZRingRay = 1;
 
+// Hierarchy traversal: the slab test of one child box (six FMAs; the
+// min/max reductions are compares) and, once per ray, the three reciprocals
+// of the direction and the origin terms (three multiplies).
+constexpr uint32_t kFlopsBox = 12;
+constexpr uint32_t kFlopsBoxRay = 6;
+
 // Algorithmic FP32 flops per ray-primitive test (counted from the reference
 // formulas: FMA = 2, mul/add = 1, rcp/sqrt = 1, compare/select = 0).
-inline int64_t flops_per_test(int type)
+__host__ __device__ constexpr int64_t flops_per_test(int type)
 {
     switch (type) {
     case RF_RECTANGLE: return 39;

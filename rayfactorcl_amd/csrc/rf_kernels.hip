@@ -1557,6 +1557,197 @@ __device__ __forceinline__ void zring_loop(CPTR(FastPrim) F, int begin, int end,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Hierarchies over the per-test primitives (BvhNode), one per type segment
+// ---------------------------------------------------------------------------
+#define GPTR(T) const __attribute__((address_space(1))) T *
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Copy of a record read through the global address space with 16-byte vector
+// loads (per-lane addresses: a leaf's records and the nodes a lane visits).
+template <typename T>
+__device__ __forceinline__ T ldg(GPTR(T) p)
+{
+    static_assert(sizeof(T) % 16 == 0, "record must be 16-byte sized");
+    u32x4 d[sizeof(T) / 16];
+    GPTR(u32x4) q = (GPTR(u32x4))p;
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(T) / 16; ++i) d[i] = q[i];
+    T v;
+    __builtin_memcpy(&v, d, sizeof(T));
+    return v;
+}
+
+// Slab terms of one ray: 1/c per axis (|c| below 1e-30 taken as +-1e-30, so
+// no 0 x inf arises) and -s/c; the ray crosses the plane x_a = w at
+// t = fma(w, inv[a], ns[a]).
+struct BoxRay {
+    float inv[3], ns[3];
+};
+
+__device__ __forceinline__ BoxRay box_ray(const Ray32 &R)
+{
+    BoxRay b;
+    const float c[3] = {R.cx, R.cy, R.cz}, s[3] = {R.sx, R.sy, R.sz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float ca = __builtin_fabsf(c[a]) < 1e-30f ? __builtin_copysignf(1e-30f, c[a]) : c[a];
+        b.inv[a] = rcp_fast(ca);
+        b.ns[a] = -s[a] * b.inv[a];
+    }
+    return b;
+}
+
+// Entry and exit distances of the ray's segment [0, tmax] through the box
+// [lo, hi]; the segment crosses the box when enter <= leave.
+__device__ __forceinline__ void box_span(const BoxRay &b, const float *lo, const float *hi, float tmax,
+                                         float &enter, float &leave)
+{
+    float n[3], f[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float u = __builtin_fmaf(lo[a], b.inv[a], b.ns[a]);
+        const float v = __builtin_fmaf(hi[a], b.inv[a], b.ns[a]);
+        n[a] = __builtin_fminf(u, v);
+        f[a] = __builtin_fmaxf(u, v);
+    }
+    enter = __builtin_fmaxf(__builtin_fmaxf(n[0], n[1]), __builtin_fmaxf(n[2], 0.0f));
+    leave = __builtin_fminf(__builtin_fminf(f[0], f[1]), __builtin_fminf(f[2], tmax));
+}
+
+template <int KIND>
+__device__ __forceinline__ constexpr uint32_t kind_flops()
+{
+    return (uint32_t)flops_per_test(KIND == SEG_CYL ? RF_CYLINDER : KIND == SEG_FRUSTUM ? RF_FRUSTUM
+                                    : KIND == SEG_SPHERE ? RF_SPHERE : KIND == SEG_RING ? RF_DISC : RF_RECTANGLE);
+}
+
+// The records of one leaf, of type segment KIND: tested as seg_loop tests
+// them.  Each record carries its traversal position p (FastPrim::type): a
+// hit at the best distance replaces the best only if that was found earlier
+// in this walk at a larger position (bp), so the walk's result is the linear
+// scan's (strict <, the earlier position wins a tie; positions increase from
+// segment to segment) whatever the visit order.  Flagged pairs queue as in
+// seg_loop; once the list is full the rest are dropped and `resume` falls to
+// (the cylinder segment's first position, slot 0) if it is above, so every
+// slot re-scans the quadric segments in full (finish_dropped).  The slots
+// walk one after the other, so a later slot can drop a pair that precedes
+// the first drop in scan order: hence the minimum.
+template <int KIND, bool REFINE, bool COUNT>
+__device__ __forceinline__ void bvh_leaf(GPTR(FastPrim) L, int c, int n, const Ray32 &R, float &tb, int &ib,
+                                         int &bp, Pending &pd, int slot, int key0, uint32_t &fl, uint32_t &ev)
+{
+    for (int j = c; j < c + n; ++j) {
+        const FastPrim q = ldg(L + j);
+        bool f;
+        float t = fast_test<KIND, REFINE>(q, R, f, pd.tau, pd.eps);
+        if constexpr (REFINE && KIND <= SEG_SPHERE) {
+            if (f) {
+                if (pd.count < PEND_K && pd.resume == NO_KEY) {
+                    pd.buf[pd.count * blockDim.x + threadIdx.x] = ((uint32_t)q.orig << 3) | (uint32_t)slot;
+                    ++pd.count;
+                } else {
+                    pd.resume = min(pd.resume, key0);
+                }
+                t = finf();
+            }
+        }
+        if (t < tb || (t == tb && q.type < bp)) { tb = t; ib = q.orig; bp = q.type; }
+    }
+    if constexpr (COUNT) {
+        fl += (uint32_t)n * kind_flops<KIND>();
+        ev += (uint32_t)n;
+    }
+}
+
+// One ray slot's walk of the hierarchy from `root`.  A child whose box the
+// ray crosses at or before its best distance tb is visited, the nearer entry
+// first (the other waits on the lane's LDS stack, entries blockDim.x apart).
+// A pruned box holds no hit at or before tb (the host pads every box beyond
+// the FP32 error of hits and of slab distances), so neither an FP32 hit nor
+// an FP64 re-solve inside it could change the result.
+template <bool REFINE, bool COUNT>
+__device__ __forceinline__ void bvh_walk(GPTR(BvhNode) N, GPTR(FastPrim) L, int root, int key0, const Ray32 &R,
+                                         const BoxRay &B, float &tb, int &ib, Pending &pd, int slot, uint32_t *stk,
+                                         uint32_t &fl, uint32_t &ev)
+{
+    int bp = -1;
+    int node = root;
+    int sp = 0;
+    while (node >= 0) {
+        const BvhNode nd = ldg(N + node);
+        float e0, l0, e1, l1;
+        box_span(B, nd.lo0, nd.hi0, tb, e0, l0);
+        box_span(B, nd.lo1, nd.hi1, tb, e1, l1);
+        const bool h0 = e0 <= l0;
+        const bool h1 = (e1 <= l1) & ((nd.c[1] >= 0) | (nd.n[1] > 0));
+        if constexpr (COUNT) {
+            fl += 2u * kFlopsBox;
+            ev += 2u;
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int nk = nd.n[k], ck = nd.c[k];
+            if ((k ? h1 : h0) && nk > 0) {
+                const int cnt = nk & 255;
+                switch (nk >> 8) {
+                case SEG_CYL: bvh_leaf<SEG_CYL, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev); break;
+                case SEG_FRUSTUM:
+                    bvh_leaf<SEG_FRUSTUM, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev);
+                    break;
+                case SEG_SPHERE:
+                    bvh_leaf<SEG_SPHERE, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev);
+                    break;
+                case SEG_RING: bvh_leaf<SEG_RING, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev); break;
+                default: bvh_leaf<SEG_RECT, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev); break;
+                }
+            }
+        }
+        const bool i0 = h0 & (nd.n[0] == 0), i1 = h1 & (nd.n[1] == 0);
+        if (i0 & i1) {
+            const bool near0 = e0 <= e1;
+            stk[sp * blockDim.x] = (uint32_t)(near0 ? nd.c[1] : nd.c[0]);
+            ++sp;
+            node = near0 ? nd.c[0] : nd.c[1];
+        } else if (i0) {
+            node = nd.c[0];
+        } else if (i1) {
+            node = nd.c[1];
+        } else {
+            node = sp > 0 ? (int)stk[--sp * blockDim.x] : -1;
+        }
+    }
+}
+
+// The hierarchy form of scan_fast: band stacks from the run list as there,
+// then per ray slot (those set in vmask) the hierarchy.
+// stk: the lane's traversal stack in LDS (BVH_STACK entries, blockDim.x apart).
+template <int NR, bool REFINE, bool COUNT>
+__device__ __forceinline__ void scan_bvh(CPTR(GeomDesc) G, const Ray32 (&R)[NR], float (&tb)[NR], int (&ib)[NR],
+                                         Pending &pd, int stacks, uint32_t &fl, uint32_t &ev, uint32_t vmask,
+                                         uint32_t *stk)
+{
+    CPTR(FastPrim) F = cst(G->fast);
+    const int n_runs = G->n_cyl_runs;
+    for (int j = 0; j < n_runs; ++j) {
+        const CylRun run = ldc(cst(G->cyl_runs) + j);
+        if (run.stack)
+            stack_loop<NR, REFINE, COUNT, false>(F, cst(G->bands), run, stacks >= 2 ? run.stack : 1, R, tb, ib, pd,
+                                                 fl, ev, vmask);
+    }
+    GPTR(BvhNode) N = (GPTR(BvhNode))G->bvh_nodes;
+    GPTR(FastPrim) L = (GPTR(FastPrim))G->bvh_prims;
+    const int root = G->bvh_root, key0 = G->seg_begin[SEG_CYL] * 8;
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        const bool act = (vmask >> i) & 1u;
+        const BoxRay B = box_ray(R[i]);
+        if (COUNT && act) fl += kFlopsBoxRay;
+        bvh_walk<REFINE, COUNT>(N, L, act ? root : -1, key0, R[i], B, tb[i], ib[i], pd, i, stk, fl, ev);
+    }
+}
+
 // stacks: the cylinder segment runs as its CylRun list (band stacks share the
 // quadric; 2: sorted stacks are searched); otherwise as one per-test run.
 // fl and ev gain the band stacks' flops and evaluations of the ray slots set
@@ -1784,11 +1975,13 @@ __device__ __forceinline__ void tally(const KParams &kp, CPTR(GeomDesc) G, int e
 // Unit body: the rays of work items [lo, hi) (unit-local) handled by this
 // thread, stepping by `stride` work items.
 // ---------------------------------------------------------------------------
-template <int MATH, bool BV, bool TRACE, int NR, bool REFINE, bool DIRECT, bool COUNT = true, bool BT = false>
+template <int MATH, bool BV, bool TRACE, int NR, bool REFINE, bool DIRECT, bool COUNT = true, bool BT = false,
+          bool BVH = false>
 __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, uint32_t gid, int emitter,
                                          int64_t wi0, int64_t n_wi, int64_t first, int64_t stride,
                                          uint32_t *bins, uint32_t *pend, uint64_t &tests, uint64_t &sflops,
-                                         uint64_t &sevals, uint32_t &nref, uint32_t &nover, BandsLds BL = BandsLds{})
+                                         uint64_t &sevals, uint32_t &nref, uint32_t &nover, BandsLds BL = BandsLds{},
+                                         uint32_t *bstk = nullptr)
 {
     const Emitter E = ldc(cst(G->emitters) + emitter);
     const int64_t ray0 = (wi0 - kp.trace_wi0) * 4;   // absolute ray index base (trace launches)
@@ -1807,8 +2000,11 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
             for (int i = 0; i < NR; ++i) { tb[i] = finf(); ib[i] = -1; }
             Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
             const int64_t left = n_rays - base;
-            scan_fast<NR, REFINE, COUNT, BT>(G, R, tb, ib, pd, kp.stacks, fl, ev,
-                                             left < NR ? (1u << left) - 1u : (1u << NR) - 1u, BL);
+            const uint32_t vmask = left < NR ? (1u << left) - 1u : (1u << NR) - 1u;
+            if constexpr (BVH)
+                scan_bvh<NR, REFINE, COUNT>(G, R, tb, ib, pd, kp.stacks, fl, ev, vmask, bstk);
+            else
+                scan_fast<NR, REFINE, COUNT, BT>(G, R, tb, ib, pd, kp.stacks, fl, ev, vmask, BL);
             if (COUNT && ((fl | ev) >> 30)) {
                 sflops += fl;
                 sevals += ev;
@@ -2501,15 +2697,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REFINE
 // the counters cost ~2%, and a launch counted once gives the same counts).
 // BT: each unit's band records (its geometry's cylinder segment) are copied
 // to LDS, where the searched stacks read them.
-template <int MATH, bool BV, bool TRACE, int NR, bool REFINE, bool COUNT = true, bool BT = false>
+// BVH: every scene's per-test primitives through its hierarchies (scan_bvh).
+template <int MATH, bool BV, bool TRACE, int NR, bool REFINE, bool COUNT = true, bool BT = false, bool BVH = false>
 __global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu((NR == 2 && !COUNT) ? 6 : (NR == 4 || (MATH == KM_COMPAT && !BV)) ? 5 : (MATH == KM_FP64 ? 4 : 1)))) k_queue(KParams kp)
+__attribute__((amdgpu_waves_per_eu(BVH ? 1 : (NR == 2 && !COUNT) ? 6 : (NR == 4 || (MATH == KM_COMPAT && !BV)) ? 5 : (MATH == KM_FP64 ? 4 : 1)))) k_queue(KParams kp)
 {
     // dynamic LDS: [bins: kp.bins x u32][pending re-solves: PEND_K x blockDim x u32]
     //              [BT: band records, 8 B each, from the next 8-byte boundary]
+    //              [BVH: traversal stacks, BVH_STACK x blockDim x u32]
     extern __shared__ __attribute__((aligned(16))) uint32_t smem[];
     uint32_t *bins = smem;
     uint32_t *pend = smem + (TRACE ? 0 : kp.bins);
+    uint32_t *bstk = pend + PEND_K * blockDim.x + threadIdx.x;
     uint64_t *sband = (uint64_t *)(smem + ((kp.bins + PEND_K * blockDim.x + 1u) & ~1u));
     __shared__ int s_unit;
     uint64_t tests = 0, sflops = 0, sevals = 0;
@@ -2538,9 +2737,9 @@ __attribute__((amdgpu_waves_per_eu((NR == 2 && !COUNT) ? 6 : (NR == 4 || (MATH =
         if (lds_tally)
             for (int i = threadIdx.x; i < n; i += blockDim.x) bins[i] = 0u;
         if (BT || lds_tally) __syncthreads();
-        run_unit<MATH, BV, TRACE, NR, REFINE, false, COUNT, BT>(kp, G, gid, emitter, wi0, n_wi, threadIdx.x,
-                                                                blockDim.x, bins, pend, tests, sflops, sevals, nref,
-                                                                nover, bl);
+        run_unit<MATH, BV, TRACE, NR, REFINE, false, COUNT, BT, BVH>(kp, G, gid, emitter, wi0, n_wi, threadIdx.x,
+                                                                     blockDim.x, bins, pend, tests, sflops, sevals,
+                                                                     nref, nover, bl, bstk);
         if (lds_tally) {
             __syncthreads();
             flush_row(kp, ur, bins);
@@ -2714,6 +2913,19 @@ hipError_t launch_fill_u64(unsigned long long *p, size_t count, hipStream_t stre
 template <int MATH, bool BV, bool TRACE, int NR, bool REFINE>
 static hipError_t go_queue(const LaunchCfg &c, const KParams &kp, hipStream_t s)
 {
+    if constexpr (MATH == KM_FAST && !BV && NR <= 2) {
+        if (c.bvh) {
+            bool count = true;
+            if constexpr (!TRACE) count = c.count != 0;
+            if (count)
+                hipLaunchKernelGGL((k_queue<MATH, BV, TRACE, NR, REFINE, true, false, true>), dim3(c.grid),
+                                   dim3(c.block), c.lds_bytes, s, kp);
+            else if constexpr (!TRACE)
+                hipLaunchKernelGGL((k_queue<MATH, BV, TRACE, NR, REFINE, false, false, true>), dim3(c.grid),
+                                   dim3(c.block), c.lds_bytes, s, kp);
+            return hipGetLastError();
+        }
+    }
     // the counter switch and the LDS band copy matter for the fast band search
     // only (1 and 2 rays per thread)
     if constexpr (MATH == KM_FAST && !BV && !TRACE && NR <= 2) {

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <numeric>
@@ -99,6 +100,8 @@ struct rf_scene {
     CylRun *d_cyl_runs = nullptr;        // fast form: cylinder runs (band stacks)
     BandRec *d_bands = nullptr;
     Emitter *d_emitters = nullptr;       // primitives with their emission frames
+    BvhNode *d_bvh_nodes = nullptr;      // fast form: hierarchies over the per-test primitives
+    FastPrim *d_bvh_prims = nullptr;     // their leaves' records
     GeomDesc *d_desc = nullptr;          // single-geometry descriptor
     GeomDesc desc{};                     // host copy
     int64_t flops_per_ray = 0;           // sum over primitives of F_type (plain kernels)
@@ -115,6 +118,7 @@ struct rf_scene {
     uint32_t *d_bv_cflops = nullptr;
     double flops_per_test_avg = 0.0;     // BV accounting
     bool has_search = false;             // a fast-form band stack with sorted offsets
+    bool bvh_on = false;                 // fast form: hierarchies over the per-test primitives
     std::vector<StackInfo> stack_info;   // the fast scan's band stacks
     int32_t stack_prims = 0;             // primitives inside them
     int32_t n_nonstack = 0;              // primitives outside the fast scan's band stacks
@@ -575,6 +579,166 @@ static int64_t build_cyl_runs(const std::vector<FastPrim> &recs, int32_t cb, int
     return delta;
 }
 
+// ---------------------------------------------------------------------------
+// Hierarchies of the fast form's per-test primitives (BvhNode)
+// ---------------------------------------------------------------------------
+// Scenes with at least this many primitives outside band stacks get them.
+static constexpr int32_t kBvhMin = 48;
+// Primitives per leaf: on 300- to 2000-primitive soups one primitive per
+// leaf traced 1.2x the rays per second of two, 1.4x of four, 2x of eight.
+static constexpr int kBvhLeaf = 1;
+
+struct WBox {
+    double lo[3], hi[3];
+};
+
+// World box of a primitive: its object-space shape's box (the unit cylinder
+// and the frustum of radii 1 and |util1| over z in [0, 1], the unit sphere,
+// the disc, annulus and rectangle in z = 0) through the inverse of invm (the
+// map every hit test applies), in FP64.  False when invm is singular or the
+// box is not finite.
+static bool prim_box(const rf_primitive &p, WBox &b)
+{
+    double lo[3] = {-1, -1, 0}, hi[3] = {1, 1, 1};
+    switch (p.type) {
+    case RF_FRUSTUM: {
+        const double rm = std::max(1.0, std::fabs((double)p.util1));
+        lo[0] = lo[1] = -rm; hi[0] = hi[1] = rm;
+        break;
+    }
+    case RF_SPHERE: lo[2] = -1; break;
+    case RF_DISC:
+    case RF_RECTANGLE: hi[2] = 0; break;
+    case RF_ANNULUS: {
+        const double ro = std::sqrt(std::max({(double)p.util1, (double)p.util2, 0.0}));
+        lo[0] = lo[1] = -ro; hi[0] = hi[1] = ro; hi[2] = 0;
+        break;
+    }
+    default: break;
+    }
+    // object = A world + v; world = A^-1 (object - v) by the adjugate
+    double A[3][3], v[3];
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) A[r][c] = p.invm[4 * r + c];
+        v[r] = p.invm[4 * r + 3];
+    }
+    double adj[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            const int r1 = (c + 1) % 3, r2 = (c + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
+            adj[r][c] = A[r1][c1] * A[r2][c2] - A[r1][c2] * A[r2][c1];
+        }
+    const double det = A[0][0] * adj[0][0] + A[0][1] * adj[1][0] + A[0][2] * adj[2][0];
+    if (!std::isfinite(det) || std::fabs(det) < 1e-300) return false;
+    for (int a = 0; a < 3; ++a) { b.lo[a] = INFINITY; b.hi[a] = -INFINITY; }
+    for (int c = 0; c < 8; ++c) {
+        const double o[3] = {((c & 1) ? hi[0] : lo[0]) - v[0], ((c & 2) ? hi[1] : lo[1]) - v[1],
+                             ((c & 4) ? hi[2] : lo[2]) - v[2]};
+        for (int a = 0; a < 3; ++a) {
+            const double w = (adj[a][0] * o[0] + adj[a][1] * o[1] + adj[a][2] * o[2]) / det;
+            b.lo[a] = std::min(b.lo[a], w);
+            b.hi[a] = std::max(b.hi[a], w);
+        }
+    }
+    for (int a = 0; a < 3; ++a)
+        if (!std::isfinite(b.lo[a]) || !std::isfinite(b.hi[a])) return false;
+    return true;
+}
+
+// Median-split hierarchy over items (traversal positions with their padded
+// boxes and type segments), appended to nodes / pos.  A leaf holds at most
+// kBvhLeaf positions of one type segment; a group small enough for a leaf
+// but of mixed segments is split between segments.  Returns the root node
+// index and the depth (the most stack entries a traversal needs).
+static int32_t build_bvh(std::vector<int32_t> items, const std::vector<WBox> &box, const std::vector<int> &kind,
+                         std::vector<BvhNode> &nodes, std::vector<int32_t> &pos, int &depth)
+{
+    struct Ref {
+        int32_t c, n;
+        WBox b;
+    };
+    const auto bound = [&](const int32_t *it, size_t cnt) {
+        WBox u;
+        for (int a = 0; a < 3; ++a) { u.lo[a] = INFINITY; u.hi[a] = -INFINITY; }
+        for (size_t k = 0; k < cnt; ++k)
+            for (int a = 0; a < 3; ++a) {
+                u.lo[a] = std::min(u.lo[a], box[it[k]].lo[a]);
+                u.hi[a] = std::max(u.hi[a], box[it[k]].hi[a]);
+            }
+        return u;
+    };
+    const auto node_of = [&](int32_t self, const Ref &l, const Ref &r) {
+        BvhNode &nd = nodes[self];
+        for (int a = 0; a < 3; ++a) {
+            nd.lo0[a] = (float)l.b.lo[a]; nd.hi0[a] = (float)l.b.hi[a];
+            nd.lo1[a] = (float)r.b.lo[a]; nd.hi1[a] = (float)r.b.hi[a];
+        }
+        nd.c[0] = l.c; nd.n[0] = l.n; nd.c[1] = r.c; nd.n[1] = r.n;
+        WBox u;
+        for (int a = 0; a < 3; ++a) { u.lo[a] = std::min(l.b.lo[a], r.b.lo[a]); u.hi[a] = std::max(l.b.hi[a], r.b.hi[a]); }
+        return Ref{self, 0, u};
+    };
+    depth = 0;
+    std::function<Ref(int32_t *, size_t, int)> rec = [&](int32_t *it, size_t cnt, int level) -> Ref {
+        depth = std::max(depth, level);
+        bool one = true;
+        for (size_t k = 1; k < cnt; ++k) one = one && kind[it[k]] == kind[it[0]];
+        if (cnt <= (size_t)kBvhLeaf && one) {
+            const int32_t first = (int32_t)pos.size();
+            pos.insert(pos.end(), it, it + cnt);
+            return Ref{first, (int32_t)cnt | (kind[it[0]] << 8), bound(it, cnt)};
+        }
+        size_t mid = cnt / 2;
+        if (cnt <= (size_t)kBvhLeaf) {
+            // mixed segments: split at the segment boundary nearest the middle
+            std::sort(it, it + cnt, [&](int32_t u, int32_t v) { return kind[u] < kind[v] || (kind[u] == kind[v] && u < v); });
+            size_t best = 0;
+            for (size_t k = 1; k < cnt; ++k)
+                if (kind[it[k]] != kind[it[k - 1]] &&
+                    (best == 0 || std::llabs((long long)(2 * k) - (long long)cnt) < std::llabs((long long)(2 * best) - (long long)cnt)))
+                    best = k;
+            mid = best;
+        } else {
+            WBox cb;
+            for (int a = 0; a < 3; ++a) { cb.lo[a] = INFINITY; cb.hi[a] = -INFINITY; }
+            for (size_t k = 0; k < cnt; ++k)
+                for (int a = 0; a < 3; ++a) {
+                    const double m = 0.5 * (box[it[k]].lo[a] + box[it[k]].hi[a]);
+                    cb.lo[a] = std::min(cb.lo[a], m);
+                    cb.hi[a] = std::max(cb.hi[a], m);
+                }
+            int ax = 0;
+            for (int a = 1; a < 3; ++a)
+                if (cb.hi[a] - cb.lo[a] > cb.hi[ax] - cb.lo[ax]) ax = a;
+            std::nth_element(it, it + mid, it + cnt, [&](int32_t u, int32_t v) {
+                const double cu = box[u].lo[ax] + box[u].hi[ax], cv = box[v].lo[ax] + box[v].hi[ax];
+                return cu < cv || (cu == cv && u < v);
+            });
+        }
+        const int32_t self = (int32_t)nodes.size();
+        nodes.push_back(BvhNode{});
+        const Ref l = rec(it, mid, level + 1), r = rec(it + mid, cnt - mid, level + 1);
+        return node_of(self, l, r);
+    };
+    if (items.empty()) return -1;
+    bool one = true;
+    for (size_t k = 1; k < items.size(); ++k) one = one && kind[items[k]] == kind[items[0]];
+    if (items.size() <= (size_t)kBvhLeaf && one) {
+        // a root whose one child is the leaf
+        const int32_t self = (int32_t)nodes.size();
+        nodes.push_back(BvhNode{});
+        const Ref l = rec(items.data(), items.size(), 1);
+        BvhNode &nd = nodes[self];
+        for (int a = 0; a < 3; ++a) {
+            nd.lo0[a] = (float)l.b.lo[a]; nd.hi0[a] = (float)l.b.hi[a];
+            nd.lo1[a] = 1.0f; nd.hi1[a] = -1.0f;
+        }
+        nd.c[0] = l.c; nd.n[0] = l.n; nd.c[1] = -1; nd.n[1] = 0;
+        return self;
+    }
+    return rec(items.data(), items.size(), 0).c;
+}
+
 // Emission frame of a primitive (EmitFrame): kind 1 for a cylinder whose
 // invm rows A, B are orthogonal, of equal length and orthogonal to its axis a
 // (m's third column), to a relative 1e-6; kind 0 (the per-ray frame) for every
@@ -616,6 +780,7 @@ extern "C" int64_t rf_scene_footprint(int32_t n, int32_t nbv)
 {
     if (n < 0 || nbv < 0) return -1;
     return (int64_t)n * (sizeof(rf_primitive) + sizeof(Emitter) + 2 * sizeof(FastPrim) + 64 + 8 + 2 * sizeof(BandRec) +
+                         sizeof(BvhNode) / 2 + sizeof(FastPrim) +
                          2 * sizeof(CylRun) + 4) + 9 * 256 +
            (int64_t)nbv * (sizeof(rf_bounding_volume) + sizeof(FastPrim) + sizeof(BvMeta) + (N_SEG + 1) * 4 +
                            4 + 8) +
@@ -628,6 +793,7 @@ static void free_scene_mem(rf_scene *s)
     (void)hipFree(s->d_quads); (void)hipFree(s->d_pos2orig); (void)hipFree(s->d_orig2pos);
     (void)hipFree(s->d_bvc_fast); (void)hipFree(s->d_bvc_seg);
     (void)hipFree(s->d_cyl_runs); (void)hipFree(s->d_bands); (void)hipFree(s->d_emitters);
+    (void)hipFree(s->d_bvh_nodes); (void)hipFree(s->d_bvh_prims);
     (void)hipFree(s->d_c_runs); (void)hipFree(s->d_c_bands); (void)hipFree(s->d_c_run_of);
     (void)hipFree(s->d_bvc_runs); (void)hipFree(s->d_bvc_run_off); (void)hipFree(s->d_bvc_bands);
     (void)hipFree(s->d_bv_cflops);
@@ -670,6 +836,44 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     s->prims.assign(prims, prims + n);
     if (nbv) s->bvs.assign(bvs, bvs + nbv);
 
+    // fast form: hierarchies over the per-test primitives when many lie
+    // outside band stacks.  Their scenes group the cylinder segment: every
+    // set of two or more cylinders with bitwise equal rows (all but the z
+    // offset) contiguous, in caller order, sets in order of their first
+    // member, then the cylinders of unique rows; so the band stacks come
+    // first and exact ties between a band and a per-test cylinder go to the
+    // band in every scan.  Their rings all keep the object-frame test.
+    bool bvh_on = false;
+    std::vector<int32_t> cyl_order;   // bvh_on: the cylinder segment's caller indices
+    {
+        std::vector<int32_t> ci;
+        for (int32_t i = 0; i < n; ++i)
+            if (prims[i].type == RF_CYLINDER) ci.push_back(i);
+        const auto row = [&](int32_t i) {
+            std::array<uint32_t, 11> k;
+            const FastPrim f = make_fast(prims[i].invm, prims[i].type, prims[i].util1, prims[i].util2, i);
+            std::memcpy(k.data(), f.r, sizeof k);
+            return k;
+        };
+        std::map<std::array<uint32_t, 11>, std::vector<int32_t>> sets;
+        std::vector<std::array<uint32_t, 11>> first;
+        for (int32_t i : ci) {
+            auto &v = sets[row(i)];
+            if (v.empty()) first.push_back(row(i));
+            v.push_back(i);
+        }
+        int32_t grouped = 0;
+        for (const auto &k : first) {
+            const auto &v = sets[k];
+            if (v.size() >= 2) { cyl_order.insert(cyl_order.end(), v.begin(), v.end()); grouped += (int32_t)v.size(); }
+        }
+        for (const auto &k : first) {
+            const auto &v = sets[k];
+            if (v.size() == 1) cyl_order.push_back(v[0]);
+        }
+        bvh_on = n - grouped >= kBvhMin;
+        if (!bvh_on) cyl_order.clear();
+    }
     // type-segmented traversal order (stable within a type)
     std::vector<FastPrim> fast, fast_orig(n), fast_bvs(nbv);
     std::vector<BvMeta> meta(nbv);
@@ -680,7 +884,7 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
             // world z-plane rings first (zring_loop), then the others
             for (int32_t i = 0; i < n; ++i) {
                 FastPrim z;
-                if (seg_of(prims[i].type) == k &&
+                if (!bvh_on && seg_of(prims[i].type) == k &&
                     zring_record(prims[i].invm, prims[i].type, prims[i].util1, prims[i].util2, i, z))
                     fast.push_back(z);
             }
@@ -689,9 +893,12 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
             for (int32_t i = 0; i < n; ++i) {
                 FastPrim z;
                 if (seg_of(prims[i].type) == k &&
-                    !zring_record(prims[i].invm, prims[i].type, prims[i].util1, prims[i].util2, i, z))
+                    (bvh_on || !zring_record(prims[i].invm, prims[i].type, prims[i].util1, prims[i].util2, i, z)))
                     fast.push_back(make_fast(prims[i].invm, prims[i].type, prims[i].util1, prims[i].util2, i));
             }
+        } else if (k == SEG_CYL && bvh_on) {
+            for (int32_t i : cyl_order)
+                fast.push_back(make_fast(prims[i].invm, prims[i].type, prims[i].util1, prims[i].util2, i));
         } else {
             for (int32_t i = 0; i < n; ++i)
                 if (seg_of(prims[i].type) == k)
@@ -767,6 +974,68 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     }
     std::vector<BandRec> bands(n + 1, BandRec{0.f, 0});
     for (int32_t k = seg_begin[SEG_CYL]; k < seg_end[SEG_CYL]; ++k) bands[k + 1] = BandRec{fast[k].r[11], fast[k].orig};
+    // hierarchy over the per-test positions (the cylinders of per-test runs,
+    // every other segment whole), leaves of one segment, boxes padded by
+    // 1e-4 of the scene's extent plus 1e-5 of its largest coordinate: far
+    // beyond the FP32 error of a hit point (FP32 test or FP64 re-solve, a few
+    // ulps of the coordinates) and of the kernel's slab entry and exit
+    // distances (one FMA of a box face with the reciprocal direction, a few
+    // ulps of the largest coordinate over |c|), so a pruned box holds no hit
+    // at or before the best one
+    std::vector<BvhNode> bvh_nodes;
+    std::vector<int32_t> bvh_pos;
+    std::vector<FastPrim> bvh_prims;
+    int32_t bvh_root = -1;
+    if (bvh_on) {
+        std::vector<WBox> box(n);
+        WBox all;
+        for (int a = 0; a < 3; ++a) { all.lo[a] = INFINITY; all.hi[a] = -INFINITY; }
+        for (int32_t p = 0; p < n && bvh_on; ++p) {
+            bvh_on = prim_box(prims[fast[p].orig], box[p]);
+            for (int a = 0; a < 3 && bvh_on; ++a) {
+                all.lo[a] = std::min(all.lo[a], box[p].lo[a]);
+                all.hi[a] = std::max(all.hi[a], box[p].hi[a]);
+            }
+        }
+        if (bvh_on) {
+            double diag = 0, big = 0;
+            for (int a = 0; a < 3; ++a) {
+                diag += (all.hi[a] - all.lo[a]) * (all.hi[a] - all.lo[a]);
+                big = std::max({big, std::fabs(all.lo[a]), std::fabs(all.hi[a])});
+            }
+            const double pad = 1e-4 * std::sqrt(diag) + 1e-5 * big + 1e-30;
+            for (WBox &b : box)
+                for (int a = 0; a < 3; ++a) { b.lo[a] -= pad; b.hi[a] += pad; }
+            std::vector<int32_t> items;
+            std::vector<int> kind(n, 0);
+            for (int k = 0; k < N_SEG; ++k) {
+                for (int32_t p = seg_begin[k]; p < seg_end[k]; ++p) kind[p] = k;
+                if (k == SEG_CYL) {
+                    for (const CylRun &r : runs)
+                        if (!r.stack)
+                            for (int32_t p = r.begin; p < r.end; ++p) items.push_back(p);
+                } else {
+                    for (int32_t p = seg_begin[k]; p < seg_end[k]; ++p) items.push_back(p);
+                }
+            }
+            int deepest = 0;
+            bvh_root = build_bvh(items, box, kind, bvh_nodes, bvh_pos, deepest);
+            // a traversal keeps at most one entry per level on its stack
+            if (deepest >= BVH_STACK) bvh_on = false;
+            // leaf records in leaf order, each carrying its traversal position
+            for (int32_t p : bvh_pos) {
+                FastPrim f = fast[p];
+                f.type = p;
+                bvh_prims.push_back(f);
+            }
+        }
+        if (!bvh_on) {
+            bvh_nodes.clear();
+            bvh_prims.clear();
+            bvh_root = -1;
+        }
+    }
+    s->bvh_on = bvh_on;
     // compat scan: stacks over the caller's order, band records {invm[11], index}
     std::vector<CylRun> c_runs;
     const int64_t fcnonstack = fsum + build_compat_runs(prims, n, c_runs);
@@ -834,6 +1103,8 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
         for (int32_t k = c_runs[j].begin; k < c_runs[j].end; ++k) c_run_of[k] = (int32_t)j;
     up((void **)&s->d_c_run_of, c_run_of.data(), c_run_of.size() * sizeof(int32_t));
     up((void **)&s->d_bands, bands.data(), bands.size() * sizeof(BandRec));
+    up((void **)&s->d_bvh_nodes, bvh_nodes.data(), bvh_nodes.size() * sizeof(BvhNode));
+    up((void **)&s->d_bvh_prims, bvh_prims.data(), bvh_prims.size() * sizeof(FastPrim));
     std::vector<Emitter> emitters(n);
     for (int32_t i = 0; i < n; ++i) {
         std::memcpy(static_cast<rf_primitive *>(&emitters[i]), &prims[i], sizeof(rf_primitive));
@@ -867,6 +1138,10 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     d.bvc_bands = s->d_bvc_bands ? s->d_bvc_bands + 1 : nullptr;
     d.bv_cflops = s->d_bv_cflops;
     d.emitters = s->d_emitters;
+    d.bvh_nodes = s->d_bvh_nodes;
+    d.bvh_prims = s->d_bvh_prims;
+    d.bvh_root = bvh_root;
+    d.bvh_on = bvh_on ? 1 : 0;
     up((void **)&s->d_desc, &d, sizeof d);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
@@ -936,6 +1211,7 @@ static int check_opts(rf_ctx *ctx, const rf_solve_opts *o)
         return set_err(ctx, RF_ERR_INVALID_ARG, "block_size must be 0 or a multiple of 64 in [64, 256]");
     if (o->count_tests != 0 && o->count_tests != 1) return set_err(ctx, RF_ERR_INVALID_ARG, "count_tests must be 0 or 1");
     if (o->band_tile < -1) return set_err(ctx, RF_ERR_INVALID_ARG, "band_tile must be -1, 0 or positive");
+    if (o->hierarchy != 0 && o->hierarchy != -1) return set_err(ctx, RF_ERR_INVALID_ARG, "hierarchy must be 0 or -1");
     return RF_OK;
 }
 
@@ -1038,6 +1314,27 @@ static void band_switches(const rf_ctx *ctx, const rf_solve_opts *o, LaunchCfg &
         c.bt = 1;
         c.lds_bytes = bytes;
     }
+}
+
+// Rays per thread of the hierarchy form (compiled for 1 and 2).
+static constexpr int kBvhMaxRpt = 2;
+
+// The hierarchy form of the fast plain scan (VALU transform, band stacks on,
+// the caller's hierarchy switch on) when every scene of the launch has
+// hierarchies: at most kBvhMaxRpt rays per thread, no LDS band copy, the
+// device counters as count_tests asks, and the lanes' traversal stacks after
+// the re-solve list (the LDS tally is dropped if they would not fit).
+static void bvh_switch(const rf_solve_opts *o, LaunchCfg &c, KParams &kp, bool all)
+{
+    c.bvh = 0;
+    if (!all || o->hierarchy != 0 || c.math != KM_FAST || c.use_bv || c.direct || c.mfma || !kp.stacks) return;
+    c.bvh = 1;
+    c.bt = 0;
+    c.count = o->count_tests;
+    c.nr = std::min(c.nr, kBvhMaxRpt);
+    const size_t fixed = (size_t)(kPendK + BVH_STACK) * c.block * sizeof(uint32_t);
+    if (!c.trace && kp.bins && fixed + (size_t)kp.bins * sizeof(uint32_t) > kMaxLds) kp.bins = 0;
+    c.lds_bytes = fixed + (c.trace ? 0 : (size_t)kp.bins * sizeof(uint32_t));
 }
 
 static int ensure_queue(rf_ctx *ctx)
@@ -1187,6 +1484,7 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
     c.lds_bytes = ((c.trace || kp.bins == 0) ? 0 : (size_t)kp.bins * sizeof(uint32_t)) +
                   (size_t)kPendK * block * sizeof(uint32_t);
     band_switches(ctx, o, c, kp, cyl_records(s), s->has_search);
+    bvh_switch(o, c, kp, s->bvh_on);
     if (c.math == KM_FAST && c.use_bv && s->desc.bv_binned) {
         c.bvbin = 1;
         kp.bv_slots = s->n_bvs;
@@ -1403,6 +1701,9 @@ static int prepare_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms,
     int32_t max_cyl = 0;
     for (int g = 0; g < n_geoms; ++g) max_cyl = std::max(max_cyl, cyl_records(scenes[g]));
     band_switches(ctx, o, c, kp, max_cyl, searched);
+    bool all_bvh = true;
+    for (int g = 0; g < n_geoms; ++g) all_bvh = all_bvh && scenes[g]->bvh_on;
+    bvh_switch(o, c, kp, all_bvh);
     if (c.math == KM_FAST && use_bv) {
         bool binned = true;
         int slots = 0;
@@ -1639,6 +1940,12 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
         }
     }
     const LaunchCfg &lc = ctx->last_cfg;
+    if (lc.bvh) {
+        // the hierarchies' box and primitive tests and the band stacks'
+        // evaluations, all counted on the device (or not at all)
+        t.tests = lc.count ? (int64_t)st[4] : -1;
+        t.flops = lc.count ? (int64_t)st[3] : -1;
+    }
     t.shape_rays_per_thread = lc.mfma ? 0 : (lc.direct || o->math != RF_MATH_FAST || o->use_bvs ? 4 : lc.nr);
     t.shape_block = lc.block;
     t.shape_flags = (lc.mfma ? RF_SHAPE_MFMA : 0) | (lc.bvbin ? RF_SHAPE_BV_BINNED : 0) |
@@ -1647,7 +1954,8 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
                     (uses_stacks(ctx, o) && o->per_test == 0 && lc.math == KM_FAST && lc.nr <= 2 && !lc.mfma &&
                              !lc.use_bv
                          ? RF_SHAPE_SEARCHED : 0) |
-                    (lc.count || !(lc.math == KM_FAST && lc.nr <= 2) ? RF_SHAPE_COUNTED : 0);
+                    (lc.count || !(lc.math == KM_FAST && lc.nr <= 2) ? RF_SHAPE_COUNTED : 0) |
+                    (lc.bvh ? RF_SHAPE_HIERARCHY : 0);
     t.wall_ms = now_ms() - w0;
     if (timing) *timing = t;
     return RF_OK;
@@ -1738,6 +2046,7 @@ extern "C" int32_t rf_solve_noalloc(rf_ctx *ctx, rf_scene *s, const rf_solve_opt
     c.grid = (int)std::min<int64_t>(std::max<int64_t>(n_units, 1), groups);
     c.lds_bytes = (kp.bins ? (size_t)kp.bins * sizeof(uint32_t) : 0) + (size_t)kPendK * block * sizeof(uint32_t);
     band_switches(ctx, o, c, kp, cyl_records(s), s->has_search);
+    bvh_switch(o, c, kp, s->bvh_on);
     if (c.math == KM_FAST && c.use_bv && s->desc.bv_binned) {
         c.bvbin = 1;
         kp.bv_slots = s->n_bvs;
@@ -1892,6 +2201,7 @@ static bool same_tallies(const rf_solve_opts &a, const rf_solve_opts &b)
     return a.math == b.math && a.rng == b.rng && a.use_bvs == b.use_bvs && a.budget == b.budget &&
            (a.budget != RF_BUDGET_REFERENCE || a.n_samples == b.n_samples) && a.seed == b.seed &&
            a.rays_per_thread == b.rays_per_thread && a.block_size == b.block_size && a.per_test == b.per_test &&
+           a.hierarchy == b.hierarchy &&
            (!fast || (a.refine == b.refine && a.transform == b.transform &&
                       (!a.refine || margin(a) == margin(b))));
 }

@@ -25,7 +25,7 @@ def make_opts(math: str = "compat", rng: str = "reference", use_bvs: bool = Fals
               form: str = "persistent", refine: bool = True, seed: int = 0, geometry: int = 0,
               rays_per_thread: int = 0, block_size: int = 0, transform: str = "auto",
               refine_margin: float = 0.0, per_test: bool | int = False, count_tests: bool = True,
-              band_tile: int = 0) -> SolveOpts:
+              band_tile: int = 0, hierarchy: bool = True) -> SolveOpts:
     """Solve options.  The ray budget is the reference's launch geometry
     (``n_samples``, main.cpp:172-173) unless ``rays_per_emitter`` is given.
     ``per_test``: False / 0 = band stacks, sorted ones searched; True / 1 =
@@ -34,7 +34,9 @@ def make_opts(math: str = "compat", rng: str = "reference", use_bvs: bool = Fals
     search counts its evaluations (timing ``tests`` / ``flops`` are -1 without).
     ``band_tile``: LDS copy of the band records for the fast band search
     (0 = the autotuner's choice, off until tuned; -1 = off; n = at most n
-    records)."""
+    records).  ``hierarchy``: the fast form finds the per-test primitives of
+    scenes with at least 48 of them through bounding-box hierarchies (False:
+    every primitive tested); the same tallies either way."""
     o = SolveOpts()
     lib().rf_solve_opts_default(byref(o))
     o.math = MATH[math]
@@ -51,6 +53,7 @@ def make_opts(math: str = "compat", rng: str = "reference", use_bvs: bool = Fals
     o.per_test = int(per_test)
     o.count_tests = int(bool(count_tests))
     o.band_tile = int(band_tile)
+    o.hierarchy = 0 if hierarchy else -1
     if rays_per_emitter is not None:
         o.budget = 1
         o.rays_per_emitter = int(rays_per_emitter)
