@@ -1661,63 +1661,70 @@ __device__ __forceinline__ void bvh_leaf(GPTR(FastPrim) L, int c, int n, const R
     }
 }
 
-// One ray slot's walk of the hierarchy from `root`.  A child whose box the
-// ray crosses at or before its best distance tb is visited, the nearer entry
-// first (the other waits on the lane's LDS stack, entries blockDim.x apart).
-// A pruned box holds no hit at or before tb (the host pads every box beyond
-// the FP32 error of hits and of slab distances), so neither an FP32 hit nor
-// an FP64 re-solve inside it could change the result.
+// One node visit of a hierarchy walk: the node's two child boxes against the
+// ray's segment [0, tb], the primitive of a crossed leaf child tested, and
+// `node` set to the next node to visit (the nearer crossed inner child, the
+// other one pushed on the lane's LDS stack, entries blockDim.x apart; else
+// the top of the stack), -1 when the walk is over.  A pruned box holds no hit
+// at or before tb (the host pads every box beyond the FP32 error of hits and
+// of slab distances), so neither an FP32 hit nor an FP64 re-solve inside it
+// could change the result.
+template <bool REFINE, bool COUNT>
+__device__ __forceinline__ void bvh_step(GPTR(BvhNode) N, GPTR(FastPrim) L, int &node, int &sp, int &bp, int key0,
+                                         const Ray32 &R, const BoxRay &B, float &tb, int &ib, Pending &pd, int slot,
+                                         uint32_t *stk, uint32_t &fl, uint32_t &ev)
+{
+    const BvhNode nd = ldg(N + node);
+    float e0, l0, e1, l1;
+    box_span(B, nd.lo0, nd.hi0, tb, e0, l0);
+    box_span(B, nd.lo1, nd.hi1, tb, e1, l1);
+    const bool h0 = e0 <= l0;
+    const bool h1 = (e1 <= l1) & ((nd.c[1] >= 0) | (nd.n[1] > 0));
+    if constexpr (COUNT) {
+        fl += 2u * kFlopsBox;
+        ev += 2u;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int nk = nd.n[k], ck = nd.c[k];
+        if ((k ? h1 : h0) && nk > 0) {
+            const int cnt = nk & 255;
+            switch (nk >> 8) {
+            case SEG_CYL: bvh_leaf<SEG_CYL, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev); break;
+            case SEG_FRUSTUM:
+                bvh_leaf<SEG_FRUSTUM, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev);
+                break;
+            case SEG_SPHERE:
+                bvh_leaf<SEG_SPHERE, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev);
+                break;
+            case SEG_RING: bvh_leaf<SEG_RING, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev); break;
+            default: bvh_leaf<SEG_RECT, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev); break;
+            }
+        }
+    }
+    const bool i0 = h0 & (nd.n[0] == 0), i1 = h1 & (nd.n[1] == 0);
+    if (i0 & i1) {
+        const bool near0 = e0 <= e1;
+        stk[sp * blockDim.x] = (uint32_t)(near0 ? nd.c[1] : nd.c[0]);
+        ++sp;
+        node = near0 ? nd.c[0] : nd.c[1];
+    } else if (i0) {
+        node = nd.c[0];
+    } else if (i1) {
+        node = nd.c[1];
+    } else {
+        node = sp > 0 ? (int)stk[--sp * blockDim.x] : -1;
+    }
+}
+
+// One ray slot's walk of the hierarchy from `root` (-1: none).
 template <bool REFINE, bool COUNT>
 __device__ __forceinline__ void bvh_walk(GPTR(BvhNode) N, GPTR(FastPrim) L, int root, int key0, const Ray32 &R,
                                          const BoxRay &B, float &tb, int &ib, Pending &pd, int slot, uint32_t *stk,
                                          uint32_t &fl, uint32_t &ev)
 {
-    int bp = -1;
-    int node = root;
-    int sp = 0;
-    while (node >= 0) {
-        const BvhNode nd = ldg(N + node);
-        float e0, l0, e1, l1;
-        box_span(B, nd.lo0, nd.hi0, tb, e0, l0);
-        box_span(B, nd.lo1, nd.hi1, tb, e1, l1);
-        const bool h0 = e0 <= l0;
-        const bool h1 = (e1 <= l1) & ((nd.c[1] >= 0) | (nd.n[1] > 0));
-        if constexpr (COUNT) {
-            fl += 2u * kFlopsBox;
-            ev += 2u;
-        }
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const int nk = nd.n[k], ck = nd.c[k];
-            if ((k ? h1 : h0) && nk > 0) {
-                const int cnt = nk & 255;
-                switch (nk >> 8) {
-                case SEG_CYL: bvh_leaf<SEG_CYL, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev); break;
-                case SEG_FRUSTUM:
-                    bvh_leaf<SEG_FRUSTUM, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev);
-                    break;
-                case SEG_SPHERE:
-                    bvh_leaf<SEG_SPHERE, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev);
-                    break;
-                case SEG_RING: bvh_leaf<SEG_RING, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev); break;
-                default: bvh_leaf<SEG_RECT, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev); break;
-                }
-            }
-        }
-        const bool i0 = h0 & (nd.n[0] == 0), i1 = h1 & (nd.n[1] == 0);
-        if (i0 & i1) {
-            const bool near0 = e0 <= e1;
-            stk[sp * blockDim.x] = (uint32_t)(near0 ? nd.c[1] : nd.c[0]);
-            ++sp;
-            node = near0 ? nd.c[0] : nd.c[1];
-        } else if (i0) {
-            node = nd.c[0];
-        } else if (i1) {
-            node = nd.c[1];
-        } else {
-            node = sp > 0 ? (int)stk[--sp * blockDim.x] : -1;
-        }
-    }
+    int bp = -1, node = root, sp = 0;
+    while (node >= 0) bvh_step<REFINE, COUNT>(N, L, node, sp, bp, key0, R, B, tb, ib, pd, slot, stk, fl, ev);
 }
 
 // The hierarchy form of scan_fast: band stacks from the run list as there,
@@ -1971,6 +1978,119 @@ __device__ __forceinline__ void tally(const KParams &kp, CPTR(GeomDesc) G, int e
     }
 }
 
+// Direction cell of a ray for the hierarchy form's tile sort: the octant
+// (sign bits) and, on the octahedron |x| + |y| + |z| = 1, |x| and |y| in
+// quarters; 16 cells per octant, cell BVH_CELLS - 1 never taken (|x| and |y|
+// cannot both reach three quarters).
+constexpr int BVH_CELLS = 128;
+
+__device__ __forceinline__ int dir_cell(const Ray32 &R)
+{
+    const float ax = __builtin_fabsf(R.cx), ay = __builtin_fabsf(R.cy), az = __builtin_fabsf(R.cz);
+    const float q = 4.0f * rcp_fast(ax + ay + az);
+    const int qa = min(3, (int)(ax * q)), qb = min(3, (int)(ay * q));
+    const int oct = (R.cx < 0.0f ? 1 : 0) | (R.cy < 0.0f ? 2 : 0) | (R.cz < 0.0f ? 4 : 0);
+    return oct * 16 + qa * 4 + qb;
+}
+
+// Unit body of the hierarchy form.  Each pass takes the group's next
+// blockDim.x x NR rays (a tile), sorts them by direction cell in LDS (a
+// counting sort), and gives lane t of the group the tile's sorted rays t,
+// t + blockDim.x, ...: the lanes of a wave walk rays of one emitter in
+// neighbouring directions, which visit much the same nodes.  A ray is emitted
+// again from its index after the sort (the same ray, as for an FP64
+// re-solve), and tallied under its own index.
+template <bool TRACE, int NR, bool REFINE, bool COUNT>
+__device__ __forceinline__ void run_unit_bvh(const KParams &kp, CPTR(GeomDesc) G, const Emitter &E, uint32_t gid,
+                                             int emitter, int64_t wi0, int64_t n_wi, uint32_t *bins, uint32_t *pend,
+                                             uint64_t &sflops, uint64_t &sevals, uint32_t &nref, uint32_t &nover,
+                                             uint32_t *bstk)
+{
+    static_assert(NR <= 8, "tile of at most 8 x 256 rays");
+    __shared__ uint32_t s_cnt[BVH_CELLS];
+    __shared__ uint16_t s_ray[256 * NR];
+    const int64_t ray0 = (wi0 - kp.trace_wi0) * 4;
+    const int64_t n_rays = n_wi * 4;
+    const int tile = (int)blockDim.x * NR;
+    uint32_t fl = 0, ev = 0;
+    for (int64_t p0 = 0; p0 < n_rays; p0 += tile) {
+        // this lane's rays of the tile before the sort: NR consecutive
+        const int own = (int)threadIdx.x * NR;
+        Ray32 R[NR];
+        emit_group<NR>(kp, gid, emitter, wi0, p0 + own, E, R);
+        for (int k = threadIdx.x; k < BVH_CELLS; k += blockDim.x) s_cnt[k] = 0u;
+        __syncthreads();
+        int cell[NR];
+        uint32_t rank[NR];
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            cell[i] = p0 + own + i < n_rays ? dir_cell(R[i]) : BVH_CELLS - 1;
+            rank[i] = atomicAdd(&s_cnt[cell[i]], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            // exclusive prefix over the cells, two per lane of the first wave
+            const uint32_t c0 = s_cnt[2 * threadIdx.x], c1 = s_cnt[2 * threadIdx.x + 1];
+            uint32_t incl = c0 + c1;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t o = __shfl_up(incl, d);
+                if ((int)threadIdx.x >= d) incl += o;
+            }
+            const uint32_t excl = incl - c0 - c1;
+            s_cnt[2 * threadIdx.x] = excl;
+            s_cnt[2 * threadIdx.x + 1] = excl + c0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NR; ++i) s_ray[s_cnt[cell[i]] + rank[i]] = (uint16_t)(own + i);
+        __syncthreads();
+        int idx[NR];
+        float tb[NR];
+        int ib[NR];
+        uint32_t vmask = 0;
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            idx[i] = s_ray[threadIdx.x + i * blockDim.x];
+            const int64_t r = p0 + idx[i];
+            if (r < n_rays) vmask |= 1u << i;
+            float f[4], r4[4];
+            workitem_floats(kp, gid, emitter, wi0 + (r >> 2), f);
+            rotate_rands(f, (int)(r & 3), r4);
+            R[i] = emit_fast(&E, r4);
+            tb[i] = finf();
+            ib[i] = -1;
+        }
+        Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
+        scan_bvh<NR, REFINE, COUNT>(G, R, tb, ib, pd, kp.stacks, fl, ev, vmask, bstk);
+        if (COUNT && ((fl | ev) >> 30)) {
+            sflops += fl;
+            sevals += ev;
+            fl = ev = 0;
+        }
+        if constexpr (REFINE) {
+            const auto ray_of = [&](int k) {
+                int v = idx[0];
+#pragma unroll
+                for (int i = 1; i < NR; ++i) v = k == i ? idx[i] : v;
+                return p0 + v;
+            };
+            if (__builtin_amdgcn_ballot_w64(pd.count > 0))
+                resolve_pending<NR>(kp, G, E, gid, emitter, wi0, ray_of, tb, ib, pd, nref);
+            if (__builtin_amdgcn_ballot_w64(pd.resume != NO_KEY))
+                finish_dropped<NR>(kp, G, E, gid, emitter, wi0, ray_of, cst(G->fast), G->seg_begin, G->seg_end, tb,
+                                   ib, pd, nref, nover);
+        }
+#pragma unroll
+        for (int i = 0; i < NR; ++i)
+            if ((vmask >> i) & 1u) tally<TRACE, false>(kp, G, emitter, ray0 + p0 + idx[i], ib[i], bins);
+    }
+    if constexpr (COUNT) {
+        sflops += fl;
+        sevals += ev;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Unit body: the rays of work items [lo, hi) (unit-local) handled by this
 // thread, stepping by `stride` work items.
@@ -1985,7 +2105,10 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
 {
     const Emitter E = ldc(cst(G->emitters) + emitter);
     const int64_t ray0 = (wi0 - kp.trace_wi0) * 4;   // absolute ray index base (trace launches)
-    if constexpr (MATH == KM_FAST && !BV && !DIRECT) {
+    if constexpr (MATH == KM_FAST && !BV && !DIRECT && BVH) {
+        run_unit_bvh<TRACE, NR, REFINE, COUNT>(kp, G, E, gid, emitter, wi0, n_wi, bins, pend, sflops, sevals, nref,
+                                               nover, bstk);
+    } else if constexpr (MATH == KM_FAST && !BV && !DIRECT) {
         // NR rays per thread: ray groups first, first+stride, ... of NR consecutive rays
         const int64_t n_rays = n_wi * 4;
         // the unit's band-stack flops and evaluations, in 32 bits (folded into
@@ -2001,10 +2124,7 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
             Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
             const int64_t left = n_rays - base;
             const uint32_t vmask = left < NR ? (1u << left) - 1u : (1u << NR) - 1u;
-            if constexpr (BVH)
-                scan_bvh<NR, REFINE, COUNT>(G, R, tb, ib, pd, kp.stacks, fl, ev, vmask, bstk);
-            else
-                scan_fast<NR, REFINE, COUNT, BT>(G, R, tb, ib, pd, kp.stacks, fl, ev, vmask, BL);
+            scan_fast<NR, REFINE, COUNT, BT>(G, R, tb, ib, pd, kp.stacks, fl, ev, vmask, BL);
             if (COUNT && ((fl | ev) >> 30)) {
                 sflops += fl;
                 sevals += ev;
@@ -2697,7 +2817,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REFINE
 // the counters cost ~2%, and a launch counted once gives the same counts).
 // BT: each unit's band records (its geometry's cylinder segment) are copied
 // to LDS, where the searched stacks read them.
-// BVH: every scene's per-test primitives through its hierarchies (scan_bvh).
+// BVH: every scene's per-test primitives through its hierarchy (scan_bvh).
 template <int MATH, bool BV, bool TRACE, int NR, bool REFINE, bool COUNT = true, bool BT = false, bool BVH = false>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(BVH ? 1 : (NR == 2 && !COUNT) ? 6 : (NR == 4 || (MATH == KM_COMPAT && !BV)) ? 5 : (MATH == KM_FP64 ? 4 : 1)))) k_queue(KParams kp)
@@ -2913,7 +3033,7 @@ hipError_t launch_fill_u64(unsigned long long *p, size_t count, hipStream_t stre
 template <int MATH, bool BV, bool TRACE, int NR, bool REFINE>
 static hipError_t go_queue(const LaunchCfg &c, const KParams &kp, hipStream_t s)
 {
-    if constexpr (MATH == KM_FAST && !BV && NR <= 2) {
+    if constexpr (MATH == KM_FAST && !BV) {
         if (c.bvh) {
             bool count = true;
             if constexpr (!TRACE) count = c.count != 0;

@@ -592,6 +592,15 @@ struct WBox {
     double lo[3], hi[3];
 };
 
+static double box_area(const WBox &b)
+{
+    const double x = b.hi[0] - b.lo[0], y = b.hi[1] - b.lo[1], z = b.hi[2] - b.lo[2];
+    return x * y + y * z + z * x;
+}
+
+// Centroid bins per axis of the surface-area split.
+static constexpr int kSahBins = 32;
+
 // World box of a primitive: its object-space shape's box (the unit cylinder
 // and the frustum of radii 1 and |util1| over z in [0, 1], the unit sphere,
 // the disc, annulus and rectangle in z = 0) through the inverse of invm (the
@@ -645,8 +654,10 @@ static bool prim_box(const rf_primitive &p, WBox &b)
     return true;
 }
 
-// Median-split hierarchy over items (traversal positions with their padded
-// boxes and type segments), appended to nodes / pos.  A leaf holds at most
+// Hierarchy over items (traversal positions with their padded boxes and type
+// segments), split by the surface-area heuristic (the split of the centroid
+// bins that minimises count x surface area summed over both sides) while the
+// depth allows, appended to nodes / pos.  A leaf holds at most
 // kBvhLeaf positions of one type segment; a group small enough for a leaf
 // but of mixed segments is split between segments.  Returns the root node
 // index and the depth (the most stack entries a traversal needs).
@@ -707,13 +718,69 @@ static int32_t build_bvh(std::vector<int32_t> items, const std::vector<WBox> &bo
                     cb.lo[a] = std::min(cb.lo[a], m);
                     cb.hi[a] = std::max(cb.hi[a], m);
                 }
-            int ax = 0;
-            for (int a = 1; a < 3; ++a)
-                if (cb.hi[a] - cb.lo[a] > cb.hi[ax] - cb.lo[ax]) ax = a;
-            std::nth_element(it, it + mid, it + cnt, [&](int32_t u, int32_t v) {
-                const double cu = box[u].lo[ax] + box[u].hi[ax], cv = box[v].lo[ax] + box[v].hi[ax];
-                return cu < cv || (cu == cv && u < v);
-            });
+            // surface-area split over kSahBins centroid bins per axis, while
+            // the levels left allow an unbalanced split; else the median on
+            // the longest centroid axis
+            int lg = 0;
+            while (((size_t)1 << lg) < cnt) ++lg;
+            bool split = false;
+            if (level + lg + 4 < BVH_STACK) {
+                double best = INFINITY;
+                int bax = -1, bs = 0;
+                for (int a = 0; a < 3; ++a) {
+                    const double ext = cb.hi[a] - cb.lo[a];
+                    if (!(ext > 0.0)) continue;
+                    int64_t n_in[kSahBins] = {};
+                    WBox bb[kSahBins];
+                    for (WBox &w : bb)
+                        for (int d = 0; d < 3; ++d) { w.lo[d] = INFINITY; w.hi[d] = -INFINITY; }
+                    for (size_t k = 0; k < cnt; ++k) {
+                        const WBox &w = box[it[k]];
+                        const int b = std::min(kSahBins - 1, (int)(kSahBins * (0.5 * (w.lo[a] + w.hi[a]) - cb.lo[a]) / ext));
+                        ++n_in[b];
+                        for (int d = 0; d < 3; ++d) { bb[b].lo[d] = std::min(bb[b].lo[d], w.lo[d]); bb[b].hi[d] = std::max(bb[b].hi[d], w.hi[d]); }
+                    }
+                    double ra[kSahBins];
+                    int64_t rn[kSahBins];
+                    WBox acc;
+                    for (int d = 0; d < 3; ++d) { acc.lo[d] = INFINITY; acc.hi[d] = -INFINITY; }
+                    int64_t num = 0;
+                    for (int b = kSahBins - 1; b >= 1; --b) {
+                        num += n_in[b];
+                        for (int d = 0; d < 3; ++d) { acc.lo[d] = std::min(acc.lo[d], bb[b].lo[d]); acc.hi[d] = std::max(acc.hi[d], bb[b].hi[d]); }
+                        ra[b] = num ? box_area(acc) : 0.0;
+                        rn[b] = num;
+                    }
+                    for (int d = 0; d < 3; ++d) { acc.lo[d] = INFINITY; acc.hi[d] = -INFINITY; }
+                    num = 0;
+                    for (int b = 0; b + 1 < kSahBins; ++b) {
+                        num += n_in[b];
+                        for (int d = 0; d < 3; ++d) { acc.lo[d] = std::min(acc.lo[d], bb[b].lo[d]); acc.hi[d] = std::max(acc.hi[d], bb[b].hi[d]); }
+                        if (!num || !rn[b + 1]) continue;
+                        const double c = (double)num * box_area(acc) + (double)rn[b + 1] * ra[b + 1];
+                        if (c < best) { best = c; bax = a; bs = b + 1; }
+                    }
+                }
+                if (bax >= 0) {
+                    const double ext = cb.hi[bax] - cb.lo[bax];
+                    int32_t *m = std::partition(it, it + cnt, [&](int32_t u) {
+                        const WBox &w = box[u];
+                        return std::min(kSahBins - 1, (int)(kSahBins * (0.5 * (w.lo[bax] + w.hi[bax]) - cb.lo[bax]) / ext)) < bs;
+                    });
+                    mid = (size_t)(m - it);
+                    split = mid > 0 && mid < cnt;
+                }
+            }
+            if (!split) {
+                mid = cnt / 2;
+                int ax = 0;
+                for (int a = 1; a < 3; ++a)
+                    if (cb.hi[a] - cb.lo[a] > cb.hi[ax] - cb.lo[ax]) ax = a;
+                std::nth_element(it, it + mid, it + cnt, [&](int32_t u, int32_t v) {
+                    const double cu = box[u].lo[ax] + box[u].hi[ax], cv = box[v].lo[ax] + box[v].hi[ax];
+                    return cu < cv || (cu == cv && u < v);
+                });
+            }
         }
         const int32_t self = (int32_t)nodes.size();
         nodes.push_back(BvhNode{});
@@ -1316,12 +1383,12 @@ static void band_switches(const rf_ctx *ctx, const rf_solve_opts *o, LaunchCfg &
     }
 }
 
-// Rays per thread of the hierarchy form (compiled for 1 and 2).
-static constexpr int kBvhMaxRpt = 2;
+// Rays per thread of the hierarchy form (compiled for 1, 2, 4 and 8).
+static constexpr int kBvhMaxRpt = 8;
 
 // The hierarchy form of the fast plain scan (VALU transform, band stacks on,
-// the caller's hierarchy switch on) when every scene of the launch has
-// hierarchies: at most kBvhMaxRpt rays per thread, no LDS band copy, the
+// the caller's hierarchy switch on) when every scene of the launch has a
+// hierarchy: at most kBvhMaxRpt rays per thread, no LDS band copy, the
 // device counters as count_tests asks, and the lanes' traversal stacks after
 // the re-solve list (the LDS tally is dropped if they would not fit).
 static void bvh_switch(const rf_solve_opts *o, LaunchCfg &c, KParams &kp, bool all)
