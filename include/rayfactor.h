@@ -142,11 +142,12 @@ typedef struct rf_solve_opts {
                                  records when on), -1 = off.  Same tallies either way. */
     int32_t hierarchy;        /* RF_MATH_FAST plain scan with the VALU transform and band stacks
                                  (per_test 0 or 2): 0 (the default) = where every scene of the solve
-                                 has at least 48 primitives outside band stacks, each type's
-                                 per-test primitives are found through a bounding-box hierarchy
-                                 (boxes the ray crosses before its best hit are visited, 1 or 2
-                                 rays per thread); -1 = every primitive tested.  Same tallies
-                                 either way (exact ties between primitives: see per_test). */
+                                 has at least 48 primitives outside band stacks, those are found
+                                 through a bounding-box hierarchy (only boxes the ray crosses before
+                                 its best hit are opened; each group's rays sorted by direction
+                                 first; 2 rays per thread unless rays_per_thread asks); -1 = every
+                                 primitive tested.  Same tallies either way, exact ties between
+                                 primitives included. */
 } rf_solve_opts;
 
 typedef enum rf_transform {

@@ -1980,110 +1980,130 @@ __device__ __forceinline__ void tally(const KParams &kp, CPTR(GeomDesc) G, int e
 
 // Direction cell of a ray for the hierarchy form's tile sort: the octant
 // (sign bits) and, on the octahedron |x| + |y| + |z| = 1, |x| and |y| in
-// quarters; 16 cells per octant, cell BVH_CELLS - 1 never taken (|x| and |y|
-// cannot both reach three quarters).
-constexpr int BVH_CELLS = 128;
+// eighths, interleaved (Morton order, so that consecutive cells are
+// neighbours); 64 cells per octant.  Cell BVH_CELLS - 1 is never taken (|x|
+// and |y| cannot both reach seven eighths): it sorts the tile's missing rays
+// last.
+constexpr int BVH_CELLS = 512;
+constexpr int BVH_TILE_PASSES = 4;   // sub-passes of blockDim.x x NR rays per sorted tile
 
 __device__ __forceinline__ int dir_cell(const Ray32 &R)
 {
     const float ax = __builtin_fabsf(R.cx), ay = __builtin_fabsf(R.cy), az = __builtin_fabsf(R.cz);
-    const float q = 4.0f * rcp_fast(ax + ay + az);
-    const int qa = min(3, (int)(ax * q)), qb = min(3, (int)(ay * q));
+    const float q = 8.0f * rcp_fast(ax + ay + az);
+    const int qa = min(7, (int)(ax * q)), qb = min(7, (int)(ay * q));
     const int oct = (R.cx < 0.0f ? 1 : 0) | (R.cy < 0.0f ? 2 : 0) | (R.cz < 0.0f ? 4 : 0);
-    return oct * 16 + qa * 4 + qb;
+    const int m = (qa & 1) | ((qb & 1) << 1) | ((qa & 2) << 1) | ((qb & 2) << 2) | ((qa & 4) << 2) | ((qb & 4) << 3);
+    return oct * 64 + m;
 }
 
-// Unit body of the hierarchy form.  Each pass takes the group's next
-// blockDim.x x NR rays (a tile), sorts them by direction cell in LDS (a
-// counting sort), and gives lane t of the group the tile's sorted rays t,
-// t + blockDim.x, ...: the lanes of a wave walk rays of one emitter in
-// neighbouring directions, which visit much the same nodes.  A ray is emitted
-// again from its index after the sort (the same ray, as for an FP64
-// re-solve), and tallied under its own index.
+// Unit body of the hierarchy form.  The group takes its rays in tiles of
+// BVH_TILE_PASSES x blockDim.x x NR, counting-sorts each tile by direction
+// cell in LDS, and walks it in BVH_TILE_PASSES sub-passes in which lane t
+// holds sorted rays t, t + blockDim.x, ... of the sub-pass: the lanes of a
+// wave walk rays of one emitter in neighbouring directions, which visit much
+// the same nodes.  A ray is emitted again from its index after the sort (the
+// same ray, as for an FP64 re-solve) and tallied under its own index; the
+// order within a cell does not matter, as no ray's result depends on its lane.
 template <bool TRACE, int NR, bool REFINE, bool COUNT>
 __device__ __forceinline__ void run_unit_bvh(const KParams &kp, CPTR(GeomDesc) G, const Emitter &E, uint32_t gid,
                                              int emitter, int64_t wi0, int64_t n_wi, uint32_t *bins, uint32_t *pend,
                                              uint64_t &sflops, uint64_t &sevals, uint32_t &nref, uint32_t &nover,
                                              uint32_t *bstk)
 {
-    static_assert(NR <= 8, "tile of at most 8 x 256 rays");
+    static_assert(NR <= 8, "tile of at most 8 x 256 rays per sub-pass");
+    constexpr int TMAX = BVH_TILE_PASSES * 256 * NR;
     __shared__ uint32_t s_cnt[BVH_CELLS];
-    __shared__ uint16_t s_ray[256 * NR];
+    __shared__ uint16_t s_ray[TMAX];
+    __shared__ uint16_t s_cell[TMAX];
     const int64_t ray0 = (wi0 - kp.trace_wi0) * 4;
     const int64_t n_rays = n_wi * 4;
-    const int tile = (int)blockDim.x * NR;
+    const int sub = (int)blockDim.x * NR;
+    const int tile = BVH_TILE_PASSES * sub;
     uint32_t fl = 0, ev = 0;
     for (int64_t p0 = 0; p0 < n_rays; p0 += tile) {
-        // this lane's rays of the tile before the sort: NR consecutive
-        const int own = (int)threadIdx.x * NR;
-        Ray32 R[NR];
-        emit_group<NR>(kp, gid, emitter, wi0, p0 + own, E, R);
         for (int k = threadIdx.x; k < BVH_CELLS; k += blockDim.x) s_cnt[k] = 0u;
         __syncthreads();
-        int cell[NR];
-        uint32_t rank[NR];
+        // cells of this lane's rays (NR consecutive per sub-pass) and the histogram
+        for (int sp = 0; sp < BVH_TILE_PASSES; ++sp) {
+            const int own = sp * sub + (int)threadIdx.x * NR;
+            Ray32 R[NR];
+            emit_group<NR>(kp, gid, emitter, wi0, p0 + own, E, R);
 #pragma unroll
-        for (int i = 0; i < NR; ++i) {
-            cell[i] = p0 + own + i < n_rays ? dir_cell(R[i]) : BVH_CELLS - 1;
-            rank[i] = atomicAdd(&s_cnt[cell[i]], 1u);
+            for (int i = 0; i < NR; ++i) {
+                const int c = p0 + own + i < n_rays ? dir_cell(R[i]) : BVH_CELLS - 1;
+                s_cell[own + i] = (uint16_t)c;
+                atomicAdd(&s_cnt[c], 1u);
+            }
         }
         __syncthreads();
         if (threadIdx.x < 64) {
-            // exclusive prefix over the cells, two per lane of the first wave
-            const uint32_t c0 = s_cnt[2 * threadIdx.x], c1 = s_cnt[2 * threadIdx.x + 1];
-            uint32_t incl = c0 + c1;
+            // exclusive prefix over the cells, BVH_CELLS / 64 per lane of the first wave
+            constexpr int PER = BVH_CELLS / 64;
+            uint32_t c[PER], sum = 0;
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                c[k] = s_cnt[PER * threadIdx.x + k];
+                sum += c[k];
+            }
+            uint32_t incl = sum;
 #pragma unroll
             for (int d = 1; d < 64; d <<= 1) {
                 const uint32_t o = __shfl_up(incl, d);
                 if ((int)threadIdx.x >= d) incl += o;
             }
-            const uint32_t excl = incl - c0 - c1;
-            s_cnt[2 * threadIdx.x] = excl;
-            s_cnt[2 * threadIdx.x + 1] = excl + c0;
+            uint32_t run = incl - sum;
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                s_cnt[PER * threadIdx.x + k] = run;
+                run += c[k];
+            }
         }
         __syncthreads();
-#pragma unroll
-        for (int i = 0; i < NR; ++i) s_ray[s_cnt[cell[i]] + rank[i]] = (uint16_t)(own + i);
+        for (int k = threadIdx.x; k < tile; k += blockDim.x) s_ray[atomicAdd(&s_cnt[s_cell[k]], 1u)] = (uint16_t)k;
         __syncthreads();
-        int idx[NR];
-        float tb[NR];
-        int ib[NR];
-        uint32_t vmask = 0;
+        for (int sp = 0; sp < BVH_TILE_PASSES; ++sp) {
+            Ray32 R[NR];
+            int idx[NR];
+            float tb[NR];
+            int ib[NR];
+            uint32_t vmask = 0;
 #pragma unroll
-        for (int i = 0; i < NR; ++i) {
-            idx[i] = s_ray[threadIdx.x + i * blockDim.x];
-            const int64_t r = p0 + idx[i];
-            if (r < n_rays) vmask |= 1u << i;
-            float f[4], r4[4];
-            workitem_floats(kp, gid, emitter, wi0 + (r >> 2), f);
-            rotate_rands(f, (int)(r & 3), r4);
-            R[i] = emit_fast(&E, r4);
-            tb[i] = finf();
-            ib[i] = -1;
-        }
-        Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
-        scan_bvh<NR, REFINE, COUNT>(G, R, tb, ib, pd, kp.stacks, fl, ev, vmask, bstk);
-        if (COUNT && ((fl | ev) >> 30)) {
-            sflops += fl;
-            sevals += ev;
-            fl = ev = 0;
-        }
-        if constexpr (REFINE) {
-            const auto ray_of = [&](int k) {
-                int v = idx[0];
+            for (int i = 0; i < NR; ++i) {
+                idx[i] = s_ray[sp * sub + (int)threadIdx.x + i * (int)blockDim.x];
+                const int64_t r = p0 + idx[i];
+                if (r < n_rays) vmask |= 1u << i;
+                float f[4], r4[4];
+                workitem_floats(kp, gid, emitter, wi0 + (r >> 2), f);
+                rotate_rands(f, (int)(r & 3), r4);
+                R[i] = emit_fast(&E, r4);
+                tb[i] = finf();
+                ib[i] = -1;
+            }
+            Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};
+            scan_bvh<NR, REFINE, COUNT>(G, R, tb, ib, pd, kp.stacks, fl, ev, vmask, bstk);
+            if (COUNT && ((fl | ev) >> 30)) {
+                sflops += fl;
+                sevals += ev;
+                fl = ev = 0;
+            }
+            if constexpr (REFINE) {
+                const auto ray_of = [&](int k) {
+                    int v = idx[0];
 #pragma unroll
-                for (int i = 1; i < NR; ++i) v = k == i ? idx[i] : v;
-                return p0 + v;
-            };
-            if (__builtin_amdgcn_ballot_w64(pd.count > 0))
-                resolve_pending<NR>(kp, G, E, gid, emitter, wi0, ray_of, tb, ib, pd, nref);
-            if (__builtin_amdgcn_ballot_w64(pd.resume != NO_KEY))
-                finish_dropped<NR>(kp, G, E, gid, emitter, wi0, ray_of, cst(G->fast), G->seg_begin, G->seg_end, tb,
-                                   ib, pd, nref, nover);
-        }
+                    for (int i = 1; i < NR; ++i) v = k == i ? idx[i] : v;
+                    return p0 + v;
+                };
+                if (__builtin_amdgcn_ballot_w64(pd.count > 0))
+                    resolve_pending<NR>(kp, G, E, gid, emitter, wi0, ray_of, tb, ib, pd, nref);
+                if (__builtin_amdgcn_ballot_w64(pd.resume != NO_KEY))
+                    finish_dropped<NR>(kp, G, E, gid, emitter, wi0, ray_of, cst(G->fast), G->seg_begin, G->seg_end,
+                                       tb, ib, pd, nref, nover);
+            }
 #pragma unroll
-        for (int i = 0; i < NR; ++i)
-            if ((vmask >> i) & 1u) tally<TRACE, false>(kp, G, emitter, ray0 + p0 + idx[i], ib[i], bins);
+            for (int i = 0; i < NR; ++i)
+                if ((vmask >> i) & 1u) tally<TRACE, false>(kp, G, emitter, ray0 + p0 + idx[i], ib[i], bins);
+        }
     }
     if constexpr (COUNT) {
         sflops += fl;

@@ -1383,12 +1383,14 @@ static void band_switches(const rf_ctx *ctx, const rf_solve_opts *o, LaunchCfg &
     }
 }
 
-// Rays per thread of the hierarchy form (compiled for 1, 2, 4 and 8).
-static constexpr int kBvhMaxRpt = 8;
+// Rays per thread of the hierarchy form (compiled for 1, 2, 4 and 8; 2 unless
+// the caller asks: with four sub-passes per tile, 2048-ray tiles measured
+// best on 300- to 2000-primitive soups, 1.2x 4 rays per thread, 1.4x 1).
+static constexpr int kBvhRpt = 2;
 
 // The hierarchy form of the fast plain scan (VALU transform, band stacks on,
 // the caller's hierarchy switch on) when every scene of the launch has a
-// hierarchy: at most kBvhMaxRpt rays per thread, no LDS band copy, the
+// hierarchy: kBvhRpt rays per thread unless asked, no LDS band copy, the
 // device counters as count_tests asks, and the lanes' traversal stacks after
 // the re-solve list (the LDS tally is dropped if they would not fit).
 static void bvh_switch(const rf_solve_opts *o, LaunchCfg &c, KParams &kp, bool all)
@@ -1398,9 +1400,12 @@ static void bvh_switch(const rf_solve_opts *o, LaunchCfg &c, KParams &kp, bool a
     c.bvh = 1;
     c.bt = 0;
     c.count = o->count_tests;
-    c.nr = std::min(c.nr, kBvhMaxRpt);
+    c.nr = o->rays_per_thread ? o->rays_per_thread : kBvhRpt;
     const size_t fixed = (size_t)(kPendK + BVH_STACK) * c.block * sizeof(uint32_t);
-    if (!c.trace && kp.bins && fixed + (size_t)kp.bins * sizeof(uint32_t) > kMaxLds) kp.bins = 0;
+    // the kernel's own LDS: the tile sort's cell counts and two 16-bit words
+    // per ray of a tile (four sub-passes of up to 256 x nr rays)
+    const size_t sort = 512 * sizeof(uint32_t) + (size_t)4 * 256 * c.nr * 2 * sizeof(uint16_t);
+    if (!c.trace && kp.bins && fixed + sort + (size_t)kp.bins * sizeof(uint32_t) > kMaxLds) kp.bins = 0;
     c.lds_bytes = fixed + (c.trace ? 0 : (size_t)kp.bins * sizeof(uint32_t));
 }
 

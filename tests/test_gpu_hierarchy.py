@@ -43,7 +43,7 @@ def test_soup_tallies_equal_scan(ctx, refine, rpt):
     dev, a, b = _pair(ctx, _soup(400), rays_per_emitter=1 << 13, refine=refine, rays_per_thread=rpt)
     assert a.timing["shape_flags"] & _lib.RF_SHAPE_HIERARCHY
     assert not b.timing["shape_flags"] & _lib.RF_SHAPE_HIERARCHY
-    assert a.timing["shape_rays_per_thread"] == (rpt or 4)
+    assert a.timing["shape_rays_per_thread"] == (rpt or 2)
     assert np.array_equal(a.hits, b.hits)
     # far fewer tests than the scan of every primitive
     assert 0 < a.timing["tests"] < 0.5 * b.timing["tests"]
