@@ -2125,10 +2125,13 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
 {
     const Emitter E = ldc(cst(G->emitters) + emitter);
     const int64_t ray0 = (wi0 - kp.trace_wi0) * 4;   // absolute ray index base (trace launches)
-    if constexpr (MATH == KM_FAST && !BV && !DIRECT && BVH) {
-        run_unit_bvh<TRACE, NR, REFINE, COUNT>(kp, G, E, gid, emitter, wi0, n_wi, bins, pend, sflops, sevals, nref,
-                                               nover, bstk);
-    } else if constexpr (MATH == KM_FAST && !BV && !DIRECT) {
+    if constexpr (MATH == KM_FAST && !BV && !DIRECT) {
+        // a scene with a hierarchy, in a launch of the hierarchy form
+        if (BVH && G->bvh_on) {
+            run_unit_bvh<TRACE, NR, REFINE, COUNT>(kp, G, E, gid, emitter, wi0, n_wi, bins, pend, sflops, sevals,
+                                                   nref, nover, bstk);
+            return;
+        }
         // NR rays per thread: ray groups first, first+stride, ... of NR consecutive rays
         const int64_t n_rays = n_wi * 4;
         // the unit's band-stack flops and evaluations, in 32 bits (folded into
@@ -2837,7 +2840,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REFINE
 // the counters cost ~2%, and a launch counted once gives the same counts).
 // BT: each unit's band records (its geometry's cylinder segment) are copied
 // to LDS, where the searched stacks read them.
-// BVH: every scene's per-test primitives through its hierarchy (scan_bvh).
+// BVH: the units of scenes with a hierarchy run run_unit_bvh, the others the
+// scan.
 template <int MATH, bool BV, bool TRACE, int NR, bool REFINE, bool COUNT = true, bool BT = false, bool BVH = false>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(BVH ? 1 : (NR == 2 && !COUNT) ? 6 : (NR == 4 || (MATH == KM_COMPAT && !BV)) ? 5 : (MATH == KM_FP64 ? 4 : 1)))) k_queue(KParams kp)

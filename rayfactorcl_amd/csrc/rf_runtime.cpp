@@ -1389,14 +1389,15 @@ static void band_switches(const rf_ctx *ctx, const rf_solve_opts *o, LaunchCfg &
 static constexpr int kBvhRpt = 2;
 
 // The hierarchy form of the fast plain scan (VALU transform, band stacks on,
-// the caller's hierarchy switch on) when every scene of the launch has a
-// hierarchy: kBvhRpt rays per thread unless asked, no LDS band copy, the
+// the caller's hierarchy switch on) when some scene of the launch has a
+// hierarchy (the units of the others scan): kBvhRpt rays per thread unless
+// asked, no LDS band copy, the
 // device counters as count_tests asks, and the lanes' traversal stacks after
 // the re-solve list (the LDS tally is dropped if they would not fit).
-static void bvh_switch(const rf_solve_opts *o, LaunchCfg &c, KParams &kp, bool all)
+static void bvh_switch(const rf_solve_opts *o, LaunchCfg &c, KParams &kp, bool any)
 {
     c.bvh = 0;
-    if (!all || o->hierarchy != 0 || c.math != KM_FAST || c.use_bv || c.direct || c.mfma || !kp.stacks) return;
+    if (!any || o->hierarchy != 0 || c.math != KM_FAST || c.use_bv || c.direct || c.mfma || !kp.stacks) return;
     c.bvh = 1;
     c.bt = 0;
     c.count = o->count_tests;
@@ -1773,9 +1774,9 @@ static int prepare_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms,
     int32_t max_cyl = 0;
     for (int g = 0; g < n_geoms; ++g) max_cyl = std::max(max_cyl, cyl_records(scenes[g]));
     band_switches(ctx, o, c, kp, max_cyl, searched);
-    bool all_bvh = true;
-    for (int g = 0; g < n_geoms; ++g) all_bvh = all_bvh && scenes[g]->bvh_on;
-    bvh_switch(o, c, kp, all_bvh);
+    bool any_bvh = false;
+    for (int g = 0; g < n_geoms; ++g) any_bvh = any_bvh || scenes[g]->bvh_on;
+    bvh_switch(o, c, kp, any_bvh);
     if (c.math == KM_FAST && use_bv) {
         bool binned = true;
         int slots = 0;
@@ -2014,9 +2015,20 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
     const LaunchCfg &lc = ctx->last_cfg;
     if (lc.bvh) {
         // the hierarchies' box and primitive tests and the band stacks'
-        // evaluations, all counted on the device (or not at all)
-        t.tests = lc.count ? (int64_t)st[4] : -1;
-        t.flops = lc.count ? (int64_t)st[3] : -1;
+        // evaluations, counted on the device (or not at all); the per-test
+        // primitives of scenes without a hierarchy, one test each
+        t.tests = t.flops = -1;
+        if (lc.count) {
+            t.tests = (int64_t)st[4];
+            t.flops = (int64_t)st[3];
+            for (int g = 0; g < n_geoms; ++g) {
+                if (scenes[g]->bvh_on) continue;
+                int64_t r = 0;
+                for (int64_t w : budgets[g]) r += 4 * w;
+                t.tests += r * scenes[g]->n_nonstack;
+                t.flops += r * scenes[g]->flops_per_ray_nonstack;
+            }
+        }
     }
     t.shape_rays_per_thread = lc.mfma ? 0 : (lc.direct || o->math != RF_MATH_FAST || o->use_bvs ? 4 : lc.nr);
     t.shape_block = lc.block;

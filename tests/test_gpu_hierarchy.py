@@ -110,7 +110,8 @@ def test_far_from_origin(ctx):
 
 def test_batched_mixed(ctx):
     """Soups of several sizes batched: hierarchies; with a furnace (no
-    hierarchy) in the batch: the scan for all.  Same tallies each way."""
+    hierarchy) in the batch: its units scan in the same launch.  Same
+    tallies each way."""
     soups = [rf.DeviceScene(ctx, _soup(n, seed=20 + n), None) for n in (60, 150, 700)]
     fur = rf.DeviceScene(ctx, scenes.parse_xml(scenes.furnace_xml()).prims, None)
     ra, ta = rf.batch_solve(ctx, soups, hierarchy=True, rays_per_emitter=1 << 12, **KW)
@@ -118,10 +119,17 @@ def test_batched_mixed(ctx):
     assert ta["shape_flags"] & _lib.RF_SHAPE_HIERARCHY
     for x, y in zip(ra, rb):
         assert np.array_equal(x.hits, y.hits)
+    # each scene's random stream follows its index in the batch
     rc, tc = rf.batch_solve(ctx, soups + [fur], rays_per_emitter=1 << 12, **KW)
-    assert not tc["shape_flags"] & _lib.RF_SHAPE_HIERARCHY
+    rd, td = rf.batch_solve(ctx, soups + [fur], hierarchy=False, rays_per_emitter=1 << 12, **KW)
+    assert tc["shape_flags"] & _lib.RF_SHAPE_HIERARCHY
+    assert not td["shape_flags"] & _lib.RF_SHAPE_HIERARCHY
+    for x, y in zip(rc, rd):
+        assert np.array_equal(x.hits, y.hits)
     for x, y in zip(rc, rb):
         assert np.array_equal(x.hits, y.hits)
+    # the furnace's scan tests counted on the host, the soups' on the device
+    assert 0 < tc["tests"] < td["tests"]
 
 
 def test_uncounted(ctx):
