@@ -189,9 +189,12 @@ def config3(ctx, quick: bool, cpu_seconds: float = 10.0, threads: int = 16) -> d
     b = solve(ctx, dev, math="fp64", rng="seeded", rays_per_emitter=rays, form="persistent")
     c = solve(ctx, dev, math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", refine=False)
     d = solve(ctx, dev, math="compat", rng="seeded", rays_per_emitter=rays, form="persistent")
+    # the fast form without its bounding-box hierarchy: every primitive tested
+    e = solve(ctx, dev, math="fast", rng="seeded", rays_per_emitter=rays, form="persistent", hierarchy=False)
     frac, nr = _ray_diff(ctx, dev, sc.n, 1 << 14, False)
     out = {"config": 3, "scene": "random soup of 1000 primitives (cylinder, frustum, disc, annulus, sphere)",
            "primitives": sc.n, "rays_per_emitter": rays, "fast": _rate(a), "fp64": _rate(b), "compat": _rate(d),
+           "fast_every_primitive": _rate(e), "fast_hierarchy_equals_every_primitive": bool(np.array_equal(a.hits, e.hits)),
            "fast_vs_fp64_max_dvf": _dvf(a, b), "fast_unrefined_vs_fp64_max_dvf": _dvf(c, b),
            "differing_rays": frac, "rays_compared": nr}
     if cpu_seconds > 0:
