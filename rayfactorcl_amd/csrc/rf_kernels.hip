@@ -1782,6 +1782,27 @@ __device__ __forceinline__ void scan_fast(CPTR(GeomDesc) G, const Ray32 (&R)[NR]
     seg_loop<SEG_RECT, NR, REFINE, false>(F, G->seg_begin[SEG_RECT], G->seg_end[SEG_RECT], R, tb, ib, pd);
 }
 
+// Rays 0, 1 (upper = false) or 2, 3 of a work item whose four uniforms are
+// f: the upper half takes the uniforms rotated by two more places.  Four
+// selects on one condition, materialised, so that the compiler does not turn
+// them into a per-lane index select for each of the eight uses.
+__device__ __forceinline__ void emit_half(const float (&f)[4], bool upper, const Emitter &E, Ray32 (&R)[2])
+{
+    float g[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        g[k] = upper ? f[(k + 2) & 3] : f[k];
+        asm volatile("" : "+v"(g[k]));
+    }
+    float r4[4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r4[j] = g[(i + j + 1) & 3];
+        R[i] = emit_fast(&E, r4);
+    }
+}
+
 // The NR rays of ray group `base` (unit-local ray index) of one emitter.
 // base is a multiple of NR (NR = 1, 2, 4, 8), so a group never straddles two
 // work items unless NR > 4, and each ray's rotation of its work item's four
@@ -2132,16 +2153,38 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                                                    nref, nover, bstk);
             return;
         }
-        // NR rays per thread: ray groups first, first+stride, ... of NR consecutive rays
+        // NR rays per thread: ray groups first, first+stride, ... of NR
+        // consecutive rays; with 2 rays per thread, work items first,
+        // first+stride, ... whose two halves are consecutive groups of the
+        // thread (one Philox block for both)
         const int64_t n_rays = n_wi * 4;
         // the unit's band-stack flops and evaluations, in 32 bits (folded into
         // the 64-bit totals before they could wrap)
         uint32_t fl = 0, ev = 0;
-        for (int64_t base = first * NR; base < n_rays; base += stride * NR) {
+        // (kept in LDS between the halves, not in registers across the scan)
+        constexpr bool PAIR = NR == 2;
+        __shared__ float s_fw[PAIR ? 4 * 256 : 1];
+        const int64_t step = PAIR ? stride * 4 : stride * NR;
+        int half = 1;
+        for (int64_t base = PAIR ? first * 4 : first * NR; base < n_rays; base += PAIR ? (half ? step - 2 : 2) : step) {
             Ray32 R[NR];
             float tb[NR];
             int ib[NR];
-            emit_group<NR>(kp, gid, emitter, wi0, base, E, R);
+            if constexpr (PAIR) {
+                half ^= 1;
+                float fw[4];
+                if (!half) {
+                    workitem_floats(kp, gid, emitter, wi0 + (base >> 2), fw);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) s_fw[k * blockDim.x + threadIdx.x] = fw[k];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) fw[k] = s_fw[k * blockDim.x + threadIdx.x];
+                }
+                emit_half(fw, half != 0, E, R);
+            } else {
+                emit_group<NR>(kp, gid, emitter, wi0, base, E, R);
+            }
 #pragma unroll
             for (int i = 0; i < NR; ++i) { tb[i] = finf(); ib[i] = -1; }
             Pending pd{pend, 0, NO_KEY, 0, vcopy(kp.tau), vcopy(EPS_F)};

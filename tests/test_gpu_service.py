@@ -1,5 +1,7 @@
 """Streaming service (resident scenes, HBM-footprint admission, batched
 flushes) and the autotuner."""
+import math
+
 import numpy as np
 import pytest
 
@@ -100,7 +102,9 @@ def test_stream_cost_admission_and_packing(ctx):
     o_small = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=4000)
     o_big = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=40000)
     small_cost = 4000 * 20 * 20
-    big_cost = 40000 * 300 * 300
+    # 300 primitives outside band stacks: the hierarchy's scan-equivalent
+    # cost, 6 sqrt(300) tests per ray (20 get no hierarchy)
+    big_cost = 40000 * 300 * 6.0 * math.sqrt(300)
     st = rf.Stream(ctx, 1 << 30)
     st.set_cost_limits(max_queued_tests=big_cost + 3 * small_cost, max_launch_tests=4 * small_cost)
     big = st.submit(sc_big.prims, o_big)
@@ -126,7 +130,7 @@ def test_stream_chunks_a_large_request(ctx):
     ob = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=20000, seed=7)
     os_ = rf.make_opts(math="fast", rng="seeded", rays_per_emitter=4000, seed=7)
     st = rf.Stream(ctx, 1 << 30)
-    cost_big = 20000 * 120 * 120
+    cost_big = 20000 * 120 * 6.0 * math.sqrt(120)     # the hierarchy's scan-equivalent cost
     st.set_cost_limits(0.0, cost_big / 6)              # about 20 emitters per chunk
     rb = st.submit(big.prims, ob)
     steps, done = 0, 0
