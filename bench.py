@@ -88,10 +88,13 @@ def kernel_name(t: dict, refine: bool) -> str:
     if f & L.RF_SHAPE_MFMA:
         return f"rf::k_queue_mfma<{r}, false>"
     nr = t["shape_rays_per_thread"]
+    counted = str(bool(f & L.RF_SHAPE_COUNTED)).lower()
+    if f & L.RF_SHAPE_HIERARCHY:
+        return f"rf::k_queue<2, false, false, {nr}, {r}, {counted}, false, true>"
     if nr <= 2 and f & L.RF_SHAPE_SEARCHED:
-        return (f"rf::k_queue<2, false, false, {nr}, {r}, {str(bool(f & L.RF_SHAPE_COUNTED)).lower()}, "
-                f"{str(bool(f & L.RF_SHAPE_BAND_TILE)).lower()}>")
-    return f"rf::k_queue<2, false, false, {nr}, {r}>"
+        return (f"rf::k_queue<2, false, false, {nr}, {r}, {counted}, "
+                f"{str(bool(f & L.RF_SHAPE_BAND_TILE)).lower()}, false>")
+    return f"rf::k_queue<2, false, false, {nr}, {r}, true, false, false>"
 
 
 def cylinder_form(t: dict) -> str:

@@ -28,11 +28,13 @@ def test_bench_kernel_label_from_reported_shape():
     import bench
     from rayfactorcl_amd import _lib as L
     t = {"shape_flags": L.RF_SHAPE_STACKS | L.RF_SHAPE_SEARCHED | L.RF_SHAPE_BAND_TILE, "shape_rays_per_thread": 2}
-    assert bench.kernel_name(t, True) == "rf::k_queue<2, false, false, 2, true, false, true>"
+    assert bench.kernel_name(t, True) == "rf::k_queue<2, false, false, 2, true, false, true, false>"
     assert "LDS" in bench.cylinder_form(t)
     t = {"shape_flags": L.RF_SHAPE_STACKS | L.RF_SHAPE_COUNTED, "shape_rays_per_thread": 4}
-    assert bench.kernel_name(t, True) == "rf::k_queue<2, false, false, 4, true>"
+    assert bench.kernel_name(t, True) == "rf::k_queue<2, false, false, 4, true, true, false, false>"
     assert bench.cylinder_form(t).endswith("a z-range test per band")
     t = {"shape_flags": L.RF_SHAPE_MFMA | L.RF_SHAPE_COUNTED, "shape_rays_per_thread": 0}
     assert bench.kernel_name(t, False) == "rf::k_queue_mfma<false, false>"
     assert bench.cylinder_form(t) == "every test in full"
+    t = {"shape_flags": L.RF_SHAPE_STACKS | L.RF_SHAPE_HIERARCHY, "shape_rays_per_thread": 2}
+    assert bench.kernel_name(t, True) == "rf::k_queue<2, false, false, 2, true, false, false, true>"
