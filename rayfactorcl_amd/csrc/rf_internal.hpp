@@ -69,8 +69,10 @@ struct BvhNode {
 };
 static_assert(sizeof(BvhNode) == 64, "BvhNode layout");
 
-// Deepest hierarchy a traversal stack holds (BVH_STACK entries per lane).
-constexpr int BVH_STACK = 16;
+// Deepest hierarchy a scene may have: a walk keeps at most one stack entry
+// per level, and a launch gives each lane as many LDS entries as its deepest
+// scene needs.
+constexpr int BVH_MAX_DEPTH = 40;
 
 // Per-position band record of a stack: the z-row offset and the primitive's
 // index in the caller's order.

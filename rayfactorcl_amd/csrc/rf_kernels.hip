@@ -1729,7 +1729,8 @@ __device__ __forceinline__ void bvh_walk(GPTR(BvhNode) N, GPTR(FastPrim) L, int 
 
 // The hierarchy form of scan_fast: band stacks from the run list as there,
 // then per ray slot (those set in vmask) the hierarchy.
-// stk: the lane's traversal stack in LDS (BVH_STACK entries, blockDim.x apart).
+// stk: the lane's traversal stack in LDS (the deepest hierarchy's depth + 1
+// entries, blockDim.x apart, sized by the host).
 template <int NR, bool REFINE, bool COUNT>
 __device__ __forceinline__ void scan_bvh(CPTR(GeomDesc) G, const Ray32 (&R)[NR], float (&tb)[NR], int (&ib)[NR],
                                          Pending &pd, int stacks, uint32_t &fl, uint32_t &ev, uint32_t vmask,
@@ -2891,7 +2892,7 @@ __attribute__((amdgpu_waves_per_eu(BVH ? 1 : (NR == 2 && !COUNT) ? 6 : (NR == 4 
 {
     // dynamic LDS: [bins: kp.bins x u32][pending re-solves: PEND_K x blockDim x u32]
     //              [BT: band records, 8 B each, from the next 8-byte boundary]
-    //              [BVH: traversal stacks, BVH_STACK x blockDim x u32]
+    //              [BVH: traversal stacks, (depth + 1) x blockDim x u32]
     extern __shared__ __attribute__((aligned(16))) uint32_t smem[];
     uint32_t *bins = smem;
     uint32_t *pend = smem + (TRACE ? 0 : kp.bins);

@@ -119,6 +119,7 @@ struct rf_scene {
     double flops_per_test_avg = 0.0;     // BV accounting
     bool has_search = false;             // a fast-form band stack with sorted offsets
     bool bvh_on = false;                 // fast form: hierarchies over the per-test primitives
+    int bvh_depth = 0;                   // its depth: traversal stack entries a walk needs
     std::vector<StackInfo> stack_info;   // the fast scan's band stacks
     int32_t stack_prims = 0;             // primitives inside them
     int32_t n_nonstack = 0;              // primitives outside the fast scan's band stacks
@@ -724,7 +725,7 @@ static int32_t build_bvh(std::vector<int32_t> items, const std::vector<WBox> &bo
             int lg = 0;
             while (((size_t)1 << lg) < cnt) ++lg;
             bool split = false;
-            if (level + lg + 4 < BVH_STACK) {
+            if (level + lg + 4 < BVH_MAX_DEPTH) {
                 double best = INFINITY;
                 int bax = -1, bs = 0;
                 for (int a = 0; a < 3; ++a) {
@@ -1053,6 +1054,7 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     std::vector<int32_t> bvh_pos;
     std::vector<FastPrim> bvh_prims;
     int32_t bvh_root = -1;
+    int bvh_depth = 0;
     if (bvh_on) {
         std::vector<WBox> box(n);
         WBox all;
@@ -1088,7 +1090,8 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
             int deepest = 0;
             bvh_root = build_bvh(items, box, kind, bvh_nodes, bvh_pos, deepest);
             // a traversal keeps at most one entry per level on its stack
-            if (deepest >= BVH_STACK) bvh_on = false;
+            if (deepest >= BVH_MAX_DEPTH) bvh_on = false;
+            bvh_depth = deepest;
             // leaf records in leaf order, each carrying its traversal position
             for (int32_t p : bvh_pos) {
                 FastPrim f = fast[p];
@@ -1103,6 +1106,7 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
         }
     }
     s->bvh_on = bvh_on;
+    s->bvh_depth = bvh_on ? bvh_depth : 0;
     // compat scan: stacks over the caller's order, band records {invm[11], index}
     std::vector<CylRun> c_runs;
     const int64_t fcnonstack = fsum + build_compat_runs(prims, n, c_runs);
@@ -1390,23 +1394,27 @@ static constexpr int kBvhRpt = 2;
 
 // The hierarchy form of the fast plain scan (VALU transform, band stacks on,
 // the caller's hierarchy switch on) when some scene of the launch has a
-// hierarchy (the units of the others scan): kBvhRpt rays per thread unless
-// asked, no LDS band copy, the
-// device counters as count_tests asks, and the lanes' traversal stacks after
-// the re-solve list (the LDS tally is dropped if they would not fit).
-static void bvh_switch(const rf_solve_opts *o, LaunchCfg &c, KParams &kp, bool any)
+// hierarchy (depth: the deepest one's; the units of the others scan):
+// kBvhRpt rays per thread unless asked, no LDS band copy, the device counters
+// as count_tests asks, and each lane's traversal stack (depth + 1 entries)
+// after the re-solve list (the LDS tally is dropped if they would not fit).
+static void bvh_switch(const rf_solve_opts *o, LaunchCfg &c, KParams &kp, int depth)
 {
     c.bvh = 0;
-    if (!any || o->hierarchy != 0 || c.math != KM_FAST || c.use_bv || c.direct || c.mfma || !kp.stacks) return;
+    if (depth <= 0 || o->hierarchy != 0 || c.math != KM_FAST || c.use_bv || c.direct || c.mfma || !kp.stacks) return;
+    int nr = o->rays_per_thread ? o->rays_per_thread : kBvhRpt;
+    const size_t fixed = (size_t)(kPendK + depth + 1) * c.block * sizeof(uint32_t);
+    // the kernel's own LDS: the tile sort's cell counts and two 16-bit words
+    // per ray of a tile (four sub-passes of up to 256 x nr rays); fewer rays
+    // per thread where a deep hierarchy leaves no room, the scan if none fits
+    const auto sort = [](int r) { return 512 * sizeof(uint32_t) + (size_t)4 * 256 * r * 2 * sizeof(uint16_t); };
+    while (nr > 1 && fixed + sort(nr) > kMaxLds) nr /= 2;
+    if (fixed + sort(nr) > kMaxLds) return;
     c.bvh = 1;
     c.bt = 0;
     c.count = o->count_tests;
-    c.nr = o->rays_per_thread ? o->rays_per_thread : kBvhRpt;
-    const size_t fixed = (size_t)(kPendK + BVH_STACK) * c.block * sizeof(uint32_t);
-    // the kernel's own LDS: the tile sort's cell counts and two 16-bit words
-    // per ray of a tile (four sub-passes of up to 256 x nr rays)
-    const size_t sort = 512 * sizeof(uint32_t) + (size_t)4 * 256 * c.nr * 2 * sizeof(uint16_t);
-    if (!c.trace && kp.bins && fixed + sort + (size_t)kp.bins * sizeof(uint32_t) > kMaxLds) kp.bins = 0;
+    c.nr = nr;
+    if (!c.trace && kp.bins && fixed + sort(nr) + (size_t)kp.bins * sizeof(uint32_t) > kMaxLds) kp.bins = 0;
     c.lds_bytes = fixed + (c.trace ? 0 : (size_t)kp.bins * sizeof(uint32_t));
 }
 
@@ -1557,7 +1565,7 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
     c.lds_bytes = ((c.trace || kp.bins == 0) ? 0 : (size_t)kp.bins * sizeof(uint32_t)) +
                   (size_t)kPendK * block * sizeof(uint32_t);
     band_switches(ctx, o, c, kp, cyl_records(s), s->has_search);
-    bvh_switch(o, c, kp, s->bvh_on);
+    bvh_switch(o, c, kp, s->bvh_depth);
     if (c.math == KM_FAST && c.use_bv && s->desc.bv_binned) {
         c.bvbin = 1;
         kp.bv_slots = s->n_bvs;
@@ -1774,9 +1782,9 @@ static int prepare_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms,
     int32_t max_cyl = 0;
     for (int g = 0; g < n_geoms; ++g) max_cyl = std::max(max_cyl, cyl_records(scenes[g]));
     band_switches(ctx, o, c, kp, max_cyl, searched);
-    bool any_bvh = false;
-    for (int g = 0; g < n_geoms; ++g) any_bvh = any_bvh || scenes[g]->bvh_on;
-    bvh_switch(o, c, kp, any_bvh);
+    int bvh_depth = 0;
+    for (int g = 0; g < n_geoms; ++g) bvh_depth = std::max(bvh_depth, scenes[g]->bvh_depth);
+    bvh_switch(o, c, kp, bvh_depth);
     if (c.math == KM_FAST && use_bv) {
         bool binned = true;
         int slots = 0;
@@ -2130,7 +2138,7 @@ extern "C" int32_t rf_solve_noalloc(rf_ctx *ctx, rf_scene *s, const rf_solve_opt
     c.grid = (int)std::min<int64_t>(std::max<int64_t>(n_units, 1), groups);
     c.lds_bytes = (kp.bins ? (size_t)kp.bins * sizeof(uint32_t) : 0) + (size_t)kPendK * block * sizeof(uint32_t);
     band_switches(ctx, o, c, kp, cyl_records(s), s->has_search);
-    bvh_switch(o, c, kp, s->bvh_on);
+    bvh_switch(o, c, kp, s->bvh_depth);
     if (c.math == KM_FAST && c.use_bv && s->desc.bv_binned) {
         c.bvbin = 1;
         kp.bv_slots = s->n_bvs;
