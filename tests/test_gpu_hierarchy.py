@@ -163,3 +163,16 @@ def test_large_scene_keeps_hierarchy(ctx):
         diff += int((a != b).sum())
         hit += int((b >= 0).sum())
     assert hit > 0 and diff == 0
+
+
+def test_config3_scene_equal_scan(ctx):
+    """The 1000-primitive soup of configuration 3 at 2^16 rays per emitter
+    (6.6e7 rays): identical tallies with and without the hierarchy, FP64
+    re-solves on."""
+    prims = scenes.parse_xml(scenes.soup_xml(1000, seed=0x52465331)).prims
+    dev, a, b = _pair(ctx, prims, rays_per_emitter=1 << 16)
+    assert a.timing["shape_flags"] & _lib.RF_SHAPE_HIERARCHY
+    assert a.timing["refined"] > 0
+    assert np.array_equal(a.hits, b.hits)
+    print(f"[hierarchy] config-3 soup: {a.timing['rays'] / a.timing['kernel_ms'] * 1e3:.3e} rays/s against "
+          f"{b.timing['rays'] / b.timing['kernel_ms'] * 1e3:.3e} for the full scan", flush=True)
