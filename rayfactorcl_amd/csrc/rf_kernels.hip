@@ -1623,8 +1623,8 @@ __device__ __forceinline__ constexpr uint32_t kind_flops()
                                     : KIND == SEG_SPHERE ? RF_SPHERE : KIND == SEG_RING ? RF_DISC : RF_RECTANGLE);
 }
 
-// The records of one leaf, of type segment KIND: tested as seg_loop tests
-// them.  Each record carries its traversal position p (FastPrim::type): a
+// The n records of one leaf from c, of type segment KIND (q0: the first,
+// loaded by the caller): tested as seg_loop tests them.  Each record carries its traversal position p (FastPrim::type): a
 // hit at the best distance replaces the best only if that was found earlier
 // in this walk at a larger position (bp), so the walk's result is the linear
 // scan's (strict <, the earlier position wins a tie; positions increase from
@@ -1635,11 +1635,12 @@ __device__ __forceinline__ constexpr uint32_t kind_flops()
 // walk one after the other, so a later slot can drop a pair that precedes
 // the first drop in scan order: hence the minimum.
 template <int KIND, bool REFINE, bool COUNT>
-__device__ __forceinline__ void bvh_leaf(GPTR(FastPrim) L, int c, int n, const Ray32 &R, float &tb, int &ib,
-                                         int &bp, Pending &pd, int slot, int key0, uint32_t &fl, uint32_t &ev)
+__device__ __forceinline__ void bvh_leaf(GPTR(FastPrim) L, const FastPrim &q0, int c, int n, const Ray32 &R, float &tb,
+                                         int &ib, int &bp, Pending &pd, int slot, int key0, uint32_t &fl,
+                                         uint32_t &ev)
 {
-    for (int j = c; j < c + n; ++j) {
-        const FastPrim q = ldg(L + j);
+    for (int j = 0; j < n; ++j) {
+        const FastPrim q = j == 0 ? q0 : ldg(L + c + j);
         bool f;
         float t = fast_test<KIND, REFINE>(q, R, f, pd.tau, pd.eps);
         if constexpr (REFINE && KIND <= SEG_SPHERE) {
@@ -1661,20 +1662,22 @@ __device__ __forceinline__ void bvh_leaf(GPTR(FastPrim) L, int c, int n, const R
     }
 }
 
-// One node visit of a hierarchy walk: the node's two child boxes against the
-// ray's segment [0, tb], the primitive of a crossed leaf child tested, and
-// `node` set to the next node to visit (the nearer crossed inner child, the
-// other one pushed on the lane's LDS stack, entries blockDim.x apart; else
-// the top of the stack), -1 when the walk is over.  A pruned box holds no hit
+// One node visit of a hierarchy walk (nd: the node's record, loaded): the
+// node's two child boxes against the ray's segment [0, tb], `node` set to the
+// next node to visit (the nearer crossed inner child, the other one pushed on
+// the lane's LDS stack, entries blockDim.x apart; else the top of the stack;
+// -1 when the walk is over) and its record loaded into nd, then the
+// primitives of the crossed leaf children tested.  The visiting order and the
+// tb each box is tested against are those of testing the leaves first: a
+// node's boxes are tested only when it is visited, after the leaves before it.  A pruned box holds no hit
 // at or before tb (the host pads every box beyond the FP32 error of hits and
 // of slab distances), so neither an FP32 hit nor an FP64 re-solve inside it
 // could change the result.
 template <bool REFINE, bool COUNT>
-__device__ __forceinline__ void bvh_step(GPTR(BvhNode) N, GPTR(FastPrim) L, int &node, int &sp, int &bp, int key0,
-                                         const Ray32 &R, const BoxRay &B, float &tb, int &ib, Pending &pd, int slot,
-                                         uint32_t *stk, uint32_t &fl, uint32_t &ev)
+__device__ __forceinline__ void bvh_step(GPTR(BvhNode) N, GPTR(FastPrim) L, int &node, BvhNode &nd, int &sp, int &bp,
+                                         int key0, const Ray32 &R, const BoxRay &B, float &tb, int &ib, Pending &pd,
+                                         int slot, uint32_t *stk, uint32_t &fl, uint32_t &ev)
 {
-    const BvhNode nd = ldg(N + node);
     float e0, l0, e1, l1;
     box_span(B, nd.lo0, nd.hi0, tb, e0, l0);
     box_span(B, nd.lo1, nd.hi1, tb, e1, l1);
@@ -1684,36 +1687,48 @@ __device__ __forceinline__ void bvh_step(GPTR(BvhNode) N, GPTR(FastPrim) L, int 
         fl += 2u * kFlopsBox;
         ev += 2u;
     }
+    const int c[2] = {nd.c[0], nd.c[1]}, n[2] = {nd.n[0], nd.n[1]};
+    const bool leaf[2] = {h0 && n[0] > 0, h1 && n[1] > 0};
+    // the crossed leaves' first records, then the next node: their loads
+    // are in flight together, and the next node's under the leaf tests
+    FastPrim q[2];
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int nk = nd.n[k], ck = nd.c[k];
-        if ((k ? h1 : h0) && nk > 0) {
-            const int cnt = nk & 255;
-            switch (nk >> 8) {
-            case SEG_CYL: bvh_leaf<SEG_CYL, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev); break;
-            case SEG_FRUSTUM:
-                bvh_leaf<SEG_FRUSTUM, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev);
-                break;
-            case SEG_SPHERE:
-                bvh_leaf<SEG_SPHERE, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev);
-                break;
-            case SEG_RING: bvh_leaf<SEG_RING, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev); break;
-            default: bvh_leaf<SEG_RECT, REFINE, COUNT>(L, ck, cnt, R, tb, ib, bp, pd, slot, key0, fl, ev); break;
-            }
-        }
-    }
-    const bool i0 = h0 & (nd.n[0] == 0), i1 = h1 & (nd.n[1] == 0);
+    for (int k = 0; k < 2; ++k)
+        if (leaf[k]) q[k] = ldg(L + c[k]);
+    const bool i0 = h0 & (n[0] == 0), i1 = h1 & (n[1] == 0);
     if (i0 & i1) {
         const bool near0 = e0 <= e1;
-        stk[sp * blockDim.x] = (uint32_t)(near0 ? nd.c[1] : nd.c[0]);
+        stk[sp * blockDim.x] = (uint32_t)(near0 ? c[1] : c[0]);
         ++sp;
-        node = near0 ? nd.c[0] : nd.c[1];
+        node = near0 ? c[0] : c[1];
     } else if (i0) {
-        node = nd.c[0];
+        node = c[0];
     } else if (i1) {
-        node = nd.c[1];
+        node = c[1];
     } else {
         node = sp > 0 ? (int)stk[--sp * blockDim.x] : -1;
+    }
+    if (node >= 0) nd = ldg(N + node);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        if (leaf[k]) {
+            const int cnt = n[k] & 255;
+            switch (n[k] >> 8) {
+            case SEG_CYL: bvh_leaf<SEG_CYL, REFINE, COUNT>(L, q[k], c[k], cnt, R, tb, ib, bp, pd, slot, key0, fl, ev); break;
+            case SEG_FRUSTUM:
+                bvh_leaf<SEG_FRUSTUM, REFINE, COUNT>(L, q[k], c[k], cnt, R, tb, ib, bp, pd, slot, key0, fl, ev);
+                break;
+            case SEG_SPHERE:
+                bvh_leaf<SEG_SPHERE, REFINE, COUNT>(L, q[k], c[k], cnt, R, tb, ib, bp, pd, slot, key0, fl, ev);
+                break;
+            case SEG_RING:
+                bvh_leaf<SEG_RING, REFINE, COUNT>(L, q[k], c[k], cnt, R, tb, ib, bp, pd, slot, key0, fl, ev);
+                break;
+            default:
+                bvh_leaf<SEG_RECT, REFINE, COUNT>(L, q[k], c[k], cnt, R, tb, ib, bp, pd, slot, key0, fl, ev);
+                break;
+            }
+        }
     }
 }
 
@@ -1724,7 +1739,9 @@ __device__ __forceinline__ void bvh_walk(GPTR(BvhNode) N, GPTR(FastPrim) L, int 
                                          uint32_t &fl, uint32_t &ev)
 {
     int bp = -1, node = root, sp = 0;
-    while (node >= 0) bvh_step<REFINE, COUNT>(N, L, node, sp, bp, key0, R, B, tb, ib, pd, slot, stk, fl, ev);
+    BvhNode nd;
+    if (node >= 0) nd = ldg(N + node);
+    while (node >= 0) bvh_step<REFINE, COUNT>(N, L, node, nd, sp, bp, key0, R, B, tb, ib, pd, slot, stk, fl, ev);
 }
 
 // The hierarchy form of scan_fast: band stacks from the run list as there,
@@ -2885,10 +2902,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REFINE
 // BT: each unit's band records (its geometry's cylinder segment) are copied
 // to LDS, where the searched stacks read them.
 // BVH: the units of scenes with a hierarchy run run_unit_bvh, the others the
-// scan.
+// scan; 4 waves per SIMD (128 VGPRs: the walk keeps the next node's record
+// and two leaf records in flight; the compiler's own choice, 136 VGPRs at 3
+// waves, measured 20% slower).
 template <int MATH, bool BV, bool TRACE, int NR, bool REFINE, bool COUNT = true, bool BT = false, bool BVH = false>
 __global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(BVH ? 1 : (NR == 2 && !COUNT) ? 6 : (NR == 4 || (MATH == KM_COMPAT && !BV)) ? 5 : (MATH == KM_FP64 ? 4 : 1)))) k_queue(KParams kp)
+__attribute__((amdgpu_waves_per_eu(BVH ? 4 : (NR == 2 && !COUNT) ? 6 : (NR == 4 || (MATH == KM_COMPAT && !BV)) ? 5 : (MATH == KM_FP64 ? 4 : 1)))) k_queue(KParams kp)
 {
     // dynamic LDS: [bins: kp.bins x u32][pending re-solves: PEND_K x blockDim x u32]
     //              [BT: band records, 8 B each, from the next 8-byte boundary]
