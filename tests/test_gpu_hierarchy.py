@@ -176,3 +176,41 @@ def test_config3_scene_equal_scan(ctx):
     assert np.array_equal(a.hits, b.hits)
     print(f"[hierarchy] config-3 soup: {a.timing['rays'] / a.timing['kernel_ms'] * 1e3:.3e} rays/s against "
           f"{b.timing['rays'] / b.timing['kernel_ms'] * 1e3:.3e} for the full scan", flush=True)
+
+
+def _box_zoo(n_rect=60, n_soup=150, seed=43):
+    """A closed box [0,10]^3 of six axis-aligned rectangles (edges shared, so
+    rays meet exact ties where faces meet), random rectangles inside, and a
+    soup: every leaf type, axis-parallel rays' slabs, and ties."""
+    P = scenes._prim
+    rng = np.random.default_rng(seed)
+    body = [
+        P("rectangle", scale=(5.0, 5.0, 1.0), translate=(5.0, 5.0, 0.0)),
+        P("rectangle", scale=(5.0, 5.0, 1.0), rotate=scenes.FLIP_X, translate=(5.0, 5.0, 10.0)),
+        P("rectangle", scale=(5.0, 5.0, 1.0), rotate=(90.0, (0.0, 1.0, 0.0)), translate=(0.0, 5.0, 5.0)),
+        P("rectangle", scale=(5.0, 5.0, 1.0), rotate=(-90.0, (0.0, 1.0, 0.0)), translate=(10.0, 5.0, 5.0)),
+        P("rectangle", scale=(5.0, 5.0, 1.0), rotate=(-90.0, (1.0, 0.0, 0.0)), translate=(5.0, 0.0, 5.0)),
+        P("rectangle", scale=(5.0, 5.0, 1.0), rotate=(90.0, (1.0, 0.0, 0.0)), translate=(5.0, 10.0, 5.0)),
+    ]
+    for _ in range(n_rect):
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        body.append(P("rectangle", scale=(float(rng.uniform(0.2, 1.0)), float(rng.uniform(0.2, 1.0)), 1.0),
+                      rotate=(float(rng.uniform(0.0, 360.0)), tuple(float(x) for x in v)),
+                      translate=tuple(float(x) for x in rng.uniform(1.0, 9.0, 3))))
+    box = scenes.parse_xml(scenes._doc("box zoo", "".join(body))).prims
+    return np.concatenate([box, _soup(n_soup, seed=seed)])
+
+
+@pytest.mark.parametrize("refine", [True, False])
+def test_box_zoo_equal_scan(ctx, refine):
+    prims = _box_zoo()
+    dev, a, b = _pair(ctx, prims, rays_per_emitter=1 << 13, refine=refine)
+    assert a.timing["shape_flags"] & _lib.RF_SHAPE_HIERARCHY
+    assert np.array_equal(a.hits, b.hits)
+    # the box closes the scene: almost every ray hits something
+    assert a.hits.sum() > 0.95 * a.rays.sum()
+    for e in (0, 3, 10, 100, len(prims) - 1):
+        ta = rf.trace(ctx, dev, e, 0, 2048, math="fast", hierarchy=True, refine=refine)
+        tb = rf.trace(ctx, dev, e, 0, 2048, math="fast", hierarchy=False, refine=refine)
+        assert np.array_equal(ta, tb), e
