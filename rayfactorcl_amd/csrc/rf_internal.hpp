@@ -206,7 +206,7 @@ struct LaunchCfg {
     int bvbin;       // 1: fast BV form with rays binned by bounding surface
     int count;       // fast band search: 1 = count its evaluations on the device
     int bt;          // fast band search: 1 = band records copied to LDS per unit
-    int bvh;         // fast plain form: traverse the scenes' hierarchies (every scene has them)
+    int bvh;         // fast plain form: units of scenes with a hierarchy walk it (run_unit_bvh)
     size_t lds_bytes;
 };
 

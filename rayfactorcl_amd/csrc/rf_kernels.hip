@@ -1558,7 +1558,7 @@ __device__ __forceinline__ void zring_loop(CPTR(FastPrim) F, int begin, int end,
 }
 
 // ---------------------------------------------------------------------------
-// Hierarchies over the per-test primitives (BvhNode), one per type segment
+// The hierarchy over a scene's per-test primitives (BvhNode), leaves typed
 // ---------------------------------------------------------------------------
 #define GPTR(T) const __attribute__((address_space(1))) T *
 

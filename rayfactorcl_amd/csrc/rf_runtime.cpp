@@ -100,7 +100,7 @@ struct rf_scene {
     CylRun *d_cyl_runs = nullptr;        // fast form: cylinder runs (band stacks)
     BandRec *d_bands = nullptr;
     Emitter *d_emitters = nullptr;       // primitives with their emission frames
-    BvhNode *d_bvh_nodes = nullptr;      // fast form: hierarchies over the per-test primitives
+    BvhNode *d_bvh_nodes = nullptr;      // fast form: hierarchy over the per-test primitives
     FastPrim *d_bvh_prims = nullptr;     // their leaves' records
     GeomDesc *d_desc = nullptr;          // single-geometry descriptor
     GeomDesc desc{};                     // host copy
@@ -118,7 +118,7 @@ struct rf_scene {
     uint32_t *d_bv_cflops = nullptr;
     double flops_per_test_avg = 0.0;     // BV accounting
     bool has_search = false;             // a fast-form band stack with sorted offsets
-    bool bvh_on = false;                 // fast form: hierarchies over the per-test primitives
+    bool bvh_on = false;                 // fast form: a hierarchy over the per-test primitives
     int bvh_depth = 0;                   // its depth: traversal stack entries a walk needs
     std::vector<StackInfo> stack_info;   // the fast scan's band stacks
     int32_t stack_prims = 0;             // primitives inside them
@@ -904,7 +904,7 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     s->prims.assign(prims, prims + n);
     if (nbv) s->bvs.assign(bvs, bvs + nbv);
 
-    // fast form: hierarchies over the per-test primitives when many lie
+    // fast form: a hierarchy over the per-test primitives when many lie
     // outside band stacks.  Their scenes group the cylinder segment: every
     // set of two or more cylinders with bitwise equal rows (all but the z
     // offset) contiguous, in caller order, sets in order of their first
@@ -2022,7 +2022,7 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
     }
     const LaunchCfg &lc = ctx->last_cfg;
     if (lc.bvh) {
-        // the hierarchies' box and primitive tests and the band stacks'
+        // the hierarchy's box and primitive tests and the band stacks'
         // evaluations, counted on the device (or not at all); the per-test
         // primitives of scenes without a hierarchy, one test each
         t.tests = t.flops = -1;
