@@ -140,14 +140,15 @@ typedef struct rf_solve_opts {
                                  searched there when every scene's segment has at most this many
                                  records; 0 = the autotuner's choice (off until tuned; up to 8192
                                  records when on), -1 = off.  Same tallies either way. */
-    int32_t hierarchy;        /* RF_MATH_FAST plain scan with the VALU transform and band stacks
-                                 (per_test 0 or 2): 0 (the default) = where every scene of the solve
-                                 has at least 48 primitives outside band stacks, those are found
-                                 through a bounding-box hierarchy (only boxes the ray crosses before
-                                 its best hit are opened; each group's rays sorted by direction
-                                 first; 2 rays per thread unless rays_per_thread asks); -1 = every
-                                 primitive tested.  Same tallies either way, exact ties between
-                                 primitives included. */
+    int32_t hierarchy;        /* Plain scans (no bounding surfaces, per_test 0 or 2): 0 (the default)
+                                 = in scenes with at least 48 primitives outside band stacks, the
+                                 RF_MATH_FAST form with the VALU transform finds those through a
+                                 bounding-box hierarchy (only boxes the ray crosses before its best
+                                 hit are opened; each group's rays sorted by direction first; 2 rays
+                                 per thread unless rays_per_thread asks), and RF_MATH_FP64 does the
+                                 same where the scene has no band stacks; -1 = every primitive
+                                 tested.  Same tallies either way, exact ties between primitives
+                                 included. */
 } rf_solve_opts;
 
 typedef enum rf_transform {

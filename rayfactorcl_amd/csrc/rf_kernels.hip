@@ -1744,6 +1744,71 @@ __device__ __forceinline__ void bvh_walk(GPTR(BvhNode) N, GPTR(FastPrim) L, int 
     while (node >= 0) bvh_step<REFINE, COUNT>(N, L, node, nd, sp, bp, key0, R, B, tb, ib, pd, slot, stk, fl, ev);
 }
 
+// FP64 form, a scene whose hierarchy holds every primitive (bvh_on == 2: no
+// band stacks): the walk with the FP64 test of each leaf's record in the
+// caller's order (hit_d, as find_d), the nearest hit and, at equal distance,
+// the lowest index: find_d's result (strict < in the caller's order).  The
+// boxes are crossed in FP32 on the rounded ray, whose rounding moves it by
+// far less than the boxes' padding, against t_best rounded to FP32 (the same
+// margin argument as the fast form's walk).
+__device__ __forceinline__ int find_d_bvh(CPTR(GeomDesc) G, const Ray64 &ray, uint32_t *stk, uint64_t &sflops,
+                                          uint64_t &ev)
+{
+    GPTR(BvhNode) N = (GPTR(BvhNode))G->bvh_nodes;
+    GPTR(FastPrim) L = (GPTR(FastPrim))G->bvh_prims;
+    GPTR(rf_primitive) P = (GPTR(rf_primitive))G->prims;
+    Ray32 r32;
+    r32.sx = (float)ray.sx; r32.sy = (float)ray.sy; r32.sz = (float)ray.sz;
+    r32.cx = (float)ray.cx; r32.cy = (float)ray.cy; r32.cz = (float)ray.cz;
+    const BoxRay B = box_ray(r32);
+    double tbest = dinf();
+    int obj = -1;
+    uint32_t tests = 0, fl = kFlopsBoxRay;
+    int node = G->bvh_root, sp = 0;
+    while (node >= 0) {
+        const BvhNode nd = ldg(N + node);
+        const float tb = (float)tbest;
+        float e0, l0, e1, l1;
+        box_span(B, nd.lo0, nd.hi0, tb, e0, l0);
+        box_span(B, nd.lo1, nd.hi1, tb, e1, l1);
+        const bool h0 = e0 <= l0;
+        const bool h1 = (e1 <= l1) & ((nd.c[1] >= 0) | (nd.n[1] > 0));
+        tests += 2u;
+        fl += 2u * kFlopsBox;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if ((k ? h1 : h0) && nd.n[k] > 0) {
+                const int c = nd.c[k], cnt = nd.n[k] & 255;
+                for (int j = c; j < c + cnt; ++j) {
+                    // the record's caller index (FastPrim::orig)
+                    const int w = ((GPTR(int32_t))(L + j))[offsetof(FastPrim, orig) / 4];
+                    const rf_primitive p = ldg(P + w);
+                    const double t = hit_d(p.type, p.invm, p.util1, p.util2, ray);
+                    if (t < tbest || (t == tbest && w < obj)) { tbest = t; obj = w; }
+                    ++tests;
+                    fl += (uint32_t)flops_per_test(p.type);
+                }
+            }
+        }
+        const bool i0 = h0 & (nd.n[0] == 0), i1 = h1 & (nd.n[1] == 0);
+        if (i0 & i1) {
+            const bool near0 = e0 <= e1;
+            stk[sp * blockDim.x] = (uint32_t)(near0 ? nd.c[1] : nd.c[0]);
+            ++sp;
+            node = near0 ? nd.c[0] : nd.c[1];
+        } else if (i0) {
+            node = nd.c[0];
+        } else if (i1) {
+            node = nd.c[1];
+        } else {
+            node = sp > 0 ? (int)stk[--sp * blockDim.x] : -1;
+        }
+    }
+    ev += tests;
+    sflops += fl;
+    return obj;
+}
+
 // The hierarchy form of scan_fast: band stacks from the run list as there,
 // then per ray slot (those set in vmask) the hierarchy.
 // stk: the lane's traversal stack in LDS (the deepest hierarchy's depth + 1
@@ -2263,8 +2328,12 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                 } else if constexpr (MATH == KM_FP64) {
                     const double d4[4] = {r4[0], r4[1], r4[2], r4[3]};
                     const Ray64 ray = emit_d(&E, d4);
-                    idx = BV ? find_bv_cd<KM_FP64>(G, ray, tests, kp.stacks)
-                             : find_d_runs(G, 0, G->n, ray, kp.stacks, sflops, sevals);
+                    if constexpr (BV)
+                        idx = find_bv_cd<KM_FP64>(G, ray, tests, kp.stacks);
+                    else if (BVH && G->bvh_on == 2)
+                        idx = find_d_bvh(G, ray, bstk, sflops, sevals);
+                    else
+                        idx = find_d_runs(G, 0, G->n, ray, kp.stacks, sflops, sevals);
                 } else {
                     const Ray32 ray = emit_fast(&E, r4);
                     if constexpr (BV) {
@@ -3120,7 +3189,7 @@ hipError_t launch_fill_u64(unsigned long long *p, size_t count, hipStream_t stre
 template <int MATH, bool BV, bool TRACE, int NR, bool REFINE>
 static hipError_t go_queue(const LaunchCfg &c, const KParams &kp, hipStream_t s)
 {
-    if constexpr (MATH == KM_FAST && !BV) {
+    if constexpr ((MATH == KM_FAST || MATH == KM_FP64) && !BV) {
         if (c.bvh) {
             bool count = true;
             if constexpr (!TRACE) count = c.count != 0;

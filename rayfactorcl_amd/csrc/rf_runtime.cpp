@@ -1212,7 +1212,7 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     d.bvh_nodes = s->d_bvh_nodes;
     d.bvh_prims = s->d_bvh_prims;
     d.bvh_root = bvh_root;
-    d.bvh_on = bvh_on ? 1 : 0;
+    d.bvh_on = bvh_on ? (s->stack_prims == 0 ? 2 : 1) : 0;
     up((void **)&s->d_desc, &d, sizeof d);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
@@ -1387,6 +1387,15 @@ static void band_switches(const rf_ctx *ctx, const rf_solve_opts *o, LaunchCfg &
     }
 }
 
+// Whether a solve's units of scene s walk its hierarchy: the fast form where
+// the scene has one, the FP64 form where it holds every primitive (no band
+// stacks); the hierarchy's depth if so, else 0.
+static int walked_depth(const rf_scene *s, const rf_solve_opts *o)
+{
+    if (o->math == RF_MATH_FP64) return s->desc.bvh_on == 2 ? s->bvh_depth : 0;
+    return o->math == RF_MATH_FAST && s->bvh_on ? s->bvh_depth : 0;
+}
+
 // Rays per thread of the hierarchy form (compiled for 1, 2, 4 and 8; 2 unless
 // the caller asks: with four sub-passes per tile, 2048-ray tiles measured
 // best on 300- to 2000-primitive soups, 1.2x 4 rays per thread, 1.4x 1).
@@ -1401,7 +1410,9 @@ static constexpr int kBvhRpt = 2;
 static void bvh_switch(const rf_solve_opts *o, LaunchCfg &c, KParams &kp, int depth)
 {
     c.bvh = 0;
-    if (depth <= 0 || o->hierarchy != 0 || c.math != KM_FAST || c.use_bv || c.direct || c.mfma || !kp.stacks) return;
+    if (depth <= 0 || o->hierarchy != 0 || (c.math != KM_FAST && c.math != KM_FP64) || c.use_bv || c.direct ||
+        c.mfma || !kp.stacks)
+        return;
     int nr = o->rays_per_thread ? o->rays_per_thread : kBvhRpt;
     const size_t fixed = (size_t)(kPendK + depth + 1) * c.block * sizeof(uint32_t);
     // the kernel's own LDS: the tile sort's cell counts and two 16-bit words
@@ -1412,7 +1423,7 @@ static void bvh_switch(const rf_solve_opts *o, LaunchCfg &c, KParams &kp, int de
     if (fixed + sort(nr) > kMaxLds) return;
     c.bvh = 1;
     c.bt = 0;
-    c.count = o->count_tests;
+    c.count = c.math == KM_FP64 ? 1 : o->count_tests;   // the FP64 walk always counts
     c.nr = nr;
     if (!c.trace && kp.bins && fixed + sort(nr) + (size_t)kp.bins * sizeof(uint32_t) > kMaxLds) kp.bins = 0;
     c.lds_bytes = fixed + (c.trace ? 0 : (size_t)kp.bins * sizeof(uint32_t));
@@ -1565,7 +1576,7 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
     c.lds_bytes = ((c.trace || kp.bins == 0) ? 0 : (size_t)kp.bins * sizeof(uint32_t)) +
                   (size_t)kPendK * block * sizeof(uint32_t);
     band_switches(ctx, o, c, kp, cyl_records(s), s->has_search);
-    bvh_switch(o, c, kp, s->bvh_depth);
+    bvh_switch(o, c, kp, walked_depth(s, o));
     if (c.math == KM_FAST && c.use_bv && s->desc.bv_binned) {
         c.bvbin = 1;
         kp.bv_slots = s->n_bvs;
@@ -1783,7 +1794,7 @@ static int prepare_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms,
     for (int g = 0; g < n_geoms; ++g) max_cyl = std::max(max_cyl, cyl_records(scenes[g]));
     band_switches(ctx, o, c, kp, max_cyl, searched);
     int bvh_depth = 0;
-    for (int g = 0; g < n_geoms; ++g) bvh_depth = std::max(bvh_depth, scenes[g]->bvh_depth);
+    for (int g = 0; g < n_geoms; ++g) bvh_depth = std::max(bvh_depth, walked_depth(scenes[g], o));
     bvh_switch(o, c, kp, bvh_depth);
     if (c.math == KM_FAST && use_bv) {
         bool binned = true;
@@ -2029,12 +2040,13 @@ static int solve_impl(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms, const r
         if (lc.count) {
             t.tests = (int64_t)st[4];
             t.flops = (int64_t)st[3];
+            const bool fast = o->math == RF_MATH_FAST;
             for (int g = 0; g < n_geoms; ++g) {
-                if (scenes[g]->bvh_on) continue;
+                if (walked_depth(scenes[g], o)) continue;
                 int64_t r = 0;
                 for (int64_t w : budgets[g]) r += 4 * w;
-                t.tests += r * scenes[g]->n_nonstack;
-                t.flops += r * scenes[g]->flops_per_ray_nonstack;
+                t.tests += r * (fast ? scenes[g]->n_nonstack : scenes[g]->n_cnonstack);
+                t.flops += r * (fast ? scenes[g]->flops_per_ray_nonstack : scenes[g]->flops_per_ray_cnonstack);
             }
         }
     }
@@ -2138,7 +2150,7 @@ extern "C" int32_t rf_solve_noalloc(rf_ctx *ctx, rf_scene *s, const rf_solve_opt
     c.grid = (int)std::min<int64_t>(std::max<int64_t>(n_units, 1), groups);
     c.lds_bytes = (kp.bins ? (size_t)kp.bins * sizeof(uint32_t) : 0) + (size_t)kPendK * block * sizeof(uint32_t);
     band_switches(ctx, o, c, kp, cyl_records(s), s->has_search);
-    bvh_switch(o, c, kp, s->bvh_depth);
+    bvh_switch(o, c, kp, walked_depth(s, o));
     if (c.math == KM_FAST && c.use_bv && s->desc.bv_binned) {
         c.bvbin = 1;
         kp.bv_slots = s->n_bvs;

@@ -214,3 +214,35 @@ def test_box_zoo_equal_scan(ctx, refine):
         ta = rf.trace(ctx, dev, e, 0, 2048, math="fast", hierarchy=True, refine=refine)
         tb = rf.trace(ctx, dev, e, 0, 2048, math="fast", hierarchy=False, refine=refine)
         assert np.array_equal(ta, tb), e
+
+
+def test_fp64_walk_equal_scan(ctx):
+    """The FP64 form walks the hierarchy where it holds every primitive (no
+    band stacks): the same tallies as its scan; with band stacks in the
+    scene it scans."""
+    dev = rf.DeviceScene(ctx, _soup(400, seed=47), None)
+    kw = dict(math="fp64", rng="seeded", form="persistent", rays_per_emitter=1 << 12)
+    a = rf.solve(ctx, dev, hierarchy=True, **kw)
+    b = rf.solve(ctx, dev, hierarchy=False, **kw)
+    assert a.timing["shape_flags"] & _lib.RF_SHAPE_HIERARCHY
+    assert not b.timing["shape_flags"] & _lib.RF_SHAPE_HIERARCHY
+    assert np.array_equal(a.hits, b.hits)
+    assert 0 < a.timing["tests"] < 0.5 * b.timing["tests"]
+    f = scenes.parse_xml(scenes.furnace_xml(wall_bands=40, preform_bands=12)).prims
+    dev2 = rf.DeviceScene(ctx, np.concatenate([f, _soup(200, seed=7)]), None)
+    c = rf.solve(ctx, dev2, hierarchy=True, **kw)
+    d = rf.solve(ctx, dev2, hierarchy=False, **kw)
+    assert not c.timing["shape_flags"] & _lib.RF_SHAPE_HIERARCHY
+    assert np.array_equal(c.hits, d.hits)
+
+
+def test_fp64_walk_traces_match_oracle(ctx, oracle):
+    """FP64 traces through the hierarchy, ray for ray against the CPU oracle's
+    FP64 form (a scan of every primitive in the caller's order), on the box
+    zoo: ties where the box's faces meet go to the lower index in both."""
+    prims = _box_zoo(n_rect=40, n_soup=100, seed=53)
+    dev = rf.DeviceScene(ctx, prims, None)
+    for e in (0, 5, 60, len(prims) - 1):
+        a = rf.trace(ctx, dev, e, 0, 1024, math="fp64", rng="seeded")
+        b = oracle.trace(prims, None, e, 0, 1024, mode="fp64", rng="seeded")
+        assert np.array_equal(a, b), e
