@@ -2101,7 +2101,8 @@ __device__ __forceinline__ int dir_cell(const Ray32 &R)
     return oct * 64 + m;
 }
 
-// Unit body of the hierarchy form.  The group takes its rays in tiles of
+// Unit body of the hierarchy form (MATH: fast, or FP64 on a scene whose
+// hierarchy holds every primitive).  The group takes its rays in tiles of
 // BVH_TILE_PASSES x blockDim.x x NR, counting-sorts each tile by direction
 // cell in LDS, and walks it in BVH_TILE_PASSES sub-passes in which lane t
 // holds sorted rays t, t + blockDim.x, ... of the sub-pass: the lanes of a
@@ -2109,7 +2110,7 @@ __device__ __forceinline__ int dir_cell(const Ray32 &R)
 // the same nodes.  A ray is emitted again from its index after the sort (the
 // same ray, as for an FP64 re-solve) and tallied under its own index; the
 // order within a cell does not matter, as no ray's result depends on its lane.
-template <bool TRACE, int NR, bool REFINE, bool COUNT>
+template <int MATH, bool TRACE, int NR, bool REFINE, bool COUNT>
 __device__ __forceinline__ void run_unit_bvh(const KParams &kp, CPTR(GeomDesc) G, const Emitter &E, uint32_t gid,
                                              int emitter, int64_t wi0, int64_t n_wi, uint32_t *bins, uint32_t *pend,
                                              uint64_t &sflops, uint64_t &sevals, uint32_t &nref, uint32_t &nover,
@@ -2132,7 +2133,22 @@ __device__ __forceinline__ void run_unit_bvh(const KParams &kp, CPTR(GeomDesc) G
         for (int sp = 0; sp < BVH_TILE_PASSES; ++sp) {
             const int own = sp * sub + (int)threadIdx.x * NR;
             Ray32 R[NR];
-            emit_group<NR>(kp, gid, emitter, wi0, p0 + own, E, R);
+            if constexpr (MATH == KM_FAST) {
+                emit_group<NR>(kp, gid, emitter, wi0, p0 + own, E, R);
+            } else {
+#pragma unroll
+                for (int i = 0; i < NR; ++i) {
+                    const int64_t r = p0 + own + i;
+                    float f[4], r4[4];
+                    workitem_floats(kp, gid, emitter, wi0 + (r >> 2), f);
+                    rotate_rands(f, (int)(r & 3), r4);
+                    const double d4[4] = {r4[0], r4[1], r4[2], r4[3]};
+                    const Ray64 rd = emit_d(&E, d4);
+                    R[i].cx = (float)rd.cx;
+                    R[i].cy = (float)rd.cy;
+                    R[i].cz = (float)rd.cz;
+                }
+            }
 #pragma unroll
             for (int i = 0; i < NR; ++i) {
                 const int c = p0 + own + i < n_rays ? dir_cell(R[i]) : BVH_CELLS - 1;
@@ -2167,6 +2183,22 @@ __device__ __forceinline__ void run_unit_bvh(const KParams &kp, CPTR(GeomDesc) G
         for (int k = threadIdx.x; k < tile; k += blockDim.x) s_ray[atomicAdd(&s_cnt[s_cell[k]], 1u)] = (uint16_t)k;
         __syncthreads();
         for (int sp = 0; sp < BVH_TILE_PASSES; ++sp) {
+            if constexpr (MATH == KM_FP64) {
+                // FP64: each sorted ray emitted again, walked and tallied
+#pragma unroll
+                for (int i = 0; i < NR; ++i) {
+                    const int64_t r = p0 + s_ray[sp * sub + (int)threadIdx.x + i * (int)blockDim.x];
+                    if (r >= n_rays) continue;
+                    float f[4], r4[4];
+                    workitem_floats(kp, gid, emitter, wi0 + (r >> 2), f);
+                    rotate_rands(f, (int)(r & 3), r4);
+                    const double d4[4] = {r4[0], r4[1], r4[2], r4[3]};
+                    const Ray64 rd = emit_d(&E, d4);
+                    const int idx = find_d_bvh(G, rd, bstk, sflops, sevals);
+                    tally<TRACE, false>(kp, G, emitter, ray0 + r, idx, bins);
+                }
+                continue;
+            }
             Ray32 R[NR];
             int idx[NR];
             float tb[NR];
@@ -2229,11 +2261,20 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
 {
     const Emitter E = ldc(cst(G->emitters) + emitter);
     const int64_t ray0 = (wi0 - kp.trace_wi0) * 4;   // absolute ray index base (trace launches)
+    if constexpr (MATH == KM_FP64 && !BV && !DIRECT && BVH) {
+        // FP64, a scene whose hierarchy holds every primitive: sorted tiles
+        // of 2 rays per thread
+        if (G->bvh_on == 2) {
+            run_unit_bvh<KM_FP64, TRACE, 2, false, true>(kp, G, E, gid, emitter, wi0, n_wi, bins, pend, sflops, sevals,
+                                                         nref, nover, bstk);
+            return;
+        }
+    }
     if constexpr (MATH == KM_FAST && !BV && !DIRECT) {
         // a scene with a hierarchy, in a launch of the hierarchy form
         if (BVH && G->bvh_on) {
-            run_unit_bvh<TRACE, NR, REFINE, COUNT>(kp, G, E, gid, emitter, wi0, n_wi, bins, pend, sflops, sevals,
-                                                   nref, nover, bstk);
+            run_unit_bvh<KM_FAST, TRACE, NR, REFINE, COUNT>(kp, G, E, gid, emitter, wi0, n_wi, bins, pend, sflops,
+                                                            sevals, nref, nover, bstk);
             return;
         }
         // NR rays per thread: ray groups first, first+stride, ... of NR
@@ -2328,12 +2369,8 @@ __device__ __forceinline__ void run_unit(const KParams &kp, CPTR(GeomDesc) G, ui
                 } else if constexpr (MATH == KM_FP64) {
                     const double d4[4] = {r4[0], r4[1], r4[2], r4[3]};
                     const Ray64 ray = emit_d(&E, d4);
-                    if constexpr (BV)
-                        idx = find_bv_cd<KM_FP64>(G, ray, tests, kp.stacks);
-                    else if (BVH && G->bvh_on == 2)
-                        idx = find_d_bvh(G, ray, bstk, sflops, sevals);
-                    else
-                        idx = find_d_runs(G, 0, G->n, ray, kp.stacks, sflops, sevals);
+                    idx = BV ? find_bv_cd<KM_FP64>(G, ray, tests, kp.stacks)
+                             : find_d_runs(G, 0, G->n, ray, kp.stacks, sflops, sevals);
                 } else {
                     const Ray32 ray = emit_fast(&E, r4);
                     if constexpr (BV) {
