@@ -1413,13 +1413,14 @@ static void bvh_switch(const rf_solve_opts *o, LaunchCfg &c, KParams &kp, int de
     if (depth <= 0 || o->hierarchy != 0 || (c.math != KM_FAST && c.math != KM_FP64) || c.use_bv || c.direct ||
         c.mfma || !kp.stacks)
         return;
-    int nr = o->rays_per_thread ? o->rays_per_thread : kBvhRpt;
+    // (the FP64 walk's tiles are of 2 rays per thread whatever the option)
+    int nr = c.math == KM_FP64 ? 2 : (o->rays_per_thread ? o->rays_per_thread : kBvhRpt);
     const size_t fixed = (size_t)(kPendK + depth + 1) * c.block * sizeof(uint32_t);
     // the kernel's own LDS: the tile sort's cell counts and two 16-bit words
     // per ray of a tile (four sub-passes of up to 256 x nr rays); fewer rays
     // per thread where a deep hierarchy leaves no room, the scan if none fits
     const auto sort = [](int r) { return 512 * sizeof(uint32_t) + (size_t)4 * 256 * r * 2 * sizeof(uint16_t); };
-    while (nr > 1 && fixed + sort(nr) > kMaxLds) nr /= 2;
+    while (c.math == KM_FAST && nr > 1 && fixed + sort(nr) > kMaxLds) nr /= 2;
     if (fixed + sort(nr) > kMaxLds) return;
     c.bvh = 1;
     c.bt = 0;
