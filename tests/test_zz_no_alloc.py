@@ -47,6 +47,9 @@ def test_noalloc_matches_solve_on_consecutive_calls(ctx, golden_scene):
     big = scenes.sweep(8, seed=0x52465331)
     big = max(big, key=lambda g: g.n)                                   # searched band stacks
     cases = [
+        # random soups: the hierarchy form, fast and FP64
+        (scenes.parse_xml(scenes.soup_xml(700, seed=61)), None, dict(math="fast", rng="seeded", rays_per_emitter=4000)),
+        (scenes.parse_xml(scenes.soup_xml(600, seed=62)), None, dict(math="fp64", rng="seeded", rays_per_emitter=2000)),
         (big, None, dict(math="fast", rng="seeded", rays_per_emitter=20000, seed=3)),
         (scenes.parse_xml(scenes.furnace_xml(with_bv=True)), "bv",
          dict(math="fast", rng="seeded", rays_per_emitter=12000, use_bvs=True)),
