@@ -2329,16 +2329,18 @@ extern "C" int32_t rf_debug_fail_solve(rf_ctx *ctx, int32_t nth)
 // evaluations when it is searched (guess, check, walk), one per band when it
 // is walked.  The fast form's hierarchy costs about as much as a scan of
 // 6 sqrt(m) primitives for m primitives outside the stacks (measured on random
-// soups of 80 to 2000 primitives, within 10%).  Admission and packing weigh
-// requests with it.
+// soups of 80 to 2000 primitives, within 10%); the FP64 walk, used where the
+// hierarchy holds every primitive, gains as much or more and is weighed the
+// same way.  Admission and packing weigh requests with it.
 static double scan_tests_per_ray(const rf_scene *s, const rf_solve_opts &o)
 {
     if (o.per_test == 1) return (double)s->n;
     double t = 0.0;
     for (const StackInfo &k : s->stack_info) t += 1.0 + (o.per_test == 0 && k.sorted ? 6.0 : (double)k.bands);
     const double m = (double)(s->n - s->stack_prims);
-    const bool bvh = s->bvh_on && o.math == RF_MATH_FAST && !o.use_bvs && o.hierarchy == 0 &&
-                     o.transform != RF_TRANSFORM_MFMA;
+    const bool bvh = !o.use_bvs && o.hierarchy == 0 &&
+                     ((o.math == RF_MATH_FAST && s->bvh_on && o.transform != RF_TRANSFORM_MFMA) ||
+                      (o.math == RF_MATH_FP64 && s->desc.bvh_on == 2));
     return t + (bvh ? std::min(m, 6.0 * std::sqrt(m)) : m);
 }
 

@@ -121,6 +121,23 @@ def test_stream_cost_admission_and_packing(ctx):
     st.close()
 
 
+def test_stream_cost_of_the_fp64_walk(ctx):
+    """An FP64 request over a scene whose hierarchy holds every primitive is
+    weighed at the walk's scan-equivalent cost, 6 sqrt(m) tests per ray; with
+    the hierarchy switched off, at the full scan's m."""
+    sc = scenes.parse_xml(scenes.soup_xml(300, seed=2))
+    st = rf.Stream(ctx, 1 << 30)
+    walk = st.submit(sc.prims, rf.make_opts(math="fp64", rng="seeded", rays_per_emitter=1000))
+    walk_cost = 1000 * 300 * 6.0 * math.sqrt(300)
+    assert st.queued_cost() == pytest.approx(walk_cost)
+    scan = st.submit(sc.prims, rf.make_opts(math="fp64", rng="seeded", rays_per_emitter=1000,
+                                            hierarchy=False))
+    assert st.queued_cost() == pytest.approx(walk_cost + 1000 * 300 * 300)
+    assert st.flush() == 2
+    assert np.array_equal(st.collect(walk)[0], st.collect(scan)[0])
+    st.close()
+
+
 def test_stream_chunks_a_large_request(ctx):
     """A request costlier than the launch limit runs in chunks of emitters,
     one per step; its tallies are rf_solve's, and a small request submitted
