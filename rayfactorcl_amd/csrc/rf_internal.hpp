@@ -57,17 +57,21 @@ struct CylRun {
 static_assert(sizeof(CylRun) == 48, "CylRun layout");
 
 // Node of the bounding-volume hierarchy over the fast form's per-test
-// primitives (GeomDesc::bvh_root): the boxes of its two children (world
-// coordinates, padded) and, per child k, n[k] > 0: a leaf of the
-// (n[k] & 255) records bvh_prims[c[k] ..] of type segment n[k] >> 8;
-// n[k] == 0, c[k] >= 0: the node c[k]; n[k] == 0, c[k] < 0: no child (its box
-// is never tested).
+// primitives (GeomDesc::bvh_root), 32 bytes: the boxes of its two children
+// as IEEE half-precision bit patterns in the scene's box frame (x - bvh_c) /
+// bvh_s (bvh_s a power of two), lower corners rounded down and upper corners
+// up, so each holds the padded world box; then one word per child k:
+// ref[k] = index | count << 26 | segment << 29 with index sign-extended from
+// 26 bits.  count > 0: a leaf of the count records bvh_prims[index ..] of
+// type segment; count == 0, index >= 0: the node index; index == -1: no
+// child (its box is never tested).
 struct BvhNode {
-    float lo0[3], hi0[3], lo1[3], hi1[3];
-    int32_t c[2];
-    int32_t n[2];
+    uint16_t box[12];     // lo0 xyz, hi0 xyz, lo1 xyz, hi1 xyz
+    uint32_t ref[2];
 };
-static_assert(sizeof(BvhNode) == 64, "BvhNode layout");
+static_assert(sizeof(BvhNode) == 32, "BvhNode layout");
+constexpr int BVH_INDEX_BITS = 26;
+constexpr uint32_t BVH_NO_CHILD = (1u << BVH_INDEX_BITS) - 1u;
 
 // Deepest hierarchy a scene may have: a walk keeps at most one stack entry
 // per level, and a launch gives each lane as many LDS entries as its deepest
@@ -145,6 +149,9 @@ struct GeomDesc {
     const rfd::FastPrim *bvh_prims;
     int32_t bvh_root;
     int32_t bvh_on;
+    // the nodes' box frame: world x = bvh_c + bvh_s * (stored half value)
+    float bvh_c[3];
+    float bvh_s;
 };
 
 // One work unit: work items [wi0, wi0+n_wi) of one emitter of one geometry.

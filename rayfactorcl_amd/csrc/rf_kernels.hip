@@ -1579,36 +1579,52 @@ __device__ __forceinline__ T ldg(GPTR(T) p)
     return v;
 }
 
-// Slab terms of one ray: 1/c per axis (|c| below 1e-30 taken as +-1e-30, so
-// no 0 x inf arises) and -s/c; the ray crosses the plane x_a = w at
+// Slab terms of one ray in the scene's box frame (world x = bvh_c + bvh_s h):
+// with i = 1/c per axis (|c| below 1e-30 taken as +-1e-30, so no 0 x inf
+// arises), inv = bvh_s i (exact: bvh_s is a power of two) and
+// ns = (bvh_c - s) i; the ray crosses the plane h_a = w at
 // t = fma(w, inv[a], ns[a]).
 struct BoxRay {
     float inv[3], ns[3];
 };
 
-__device__ __forceinline__ BoxRay box_ray(const Ray32 &R)
+__device__ __forceinline__ BoxRay box_ray(const Ray32 &R, CPTR(GeomDesc) G)
 {
     BoxRay b;
     const float c[3] = {R.cx, R.cy, R.cz}, s[3] = {R.sx, R.sy, R.sz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const float ca = __builtin_fabsf(c[a]) < 1e-30f ? __builtin_copysignf(1e-30f, c[a]) : c[a];
-        b.inv[a] = rcp_fast(ca);
-        b.ns[a] = -s[a] * b.inv[a];
+        const float i = rcp_fast(ca);
+        b.ns[a] = (G->bvh_c[a] - s[a]) * i;
+        b.inv[a] = G->bvh_s * i;
     }
     return b;
 }
 
-// Entry and exit distances of the ray's segment [0, tmax] through the box
-// [lo, hi]; the segment crosses the box when enter <= leave.
-__device__ __forceinline__ void box_span(const BoxRay &b, const float *lo, const float *hi, float tmax,
-                                         float &enter, float &leave)
+// Child k's reference of a node: the index (sign-extended; -1: no child) and
+// the leaf word count | segment << 3 (0: not a leaf).
+__device__ __forceinline__ int bvh_index(uint32_t r)
+{
+    return (int32_t)(r << (32 - BVH_INDEX_BITS)) >> (32 - BVH_INDEX_BITS);
+}
+__device__ __forceinline__ int bvh_leafword(uint32_t r) { return (int)(r >> BVH_INDEX_BITS); }
+
+__device__ __forceinline__ float box_half(const BvhNode &nd, int i)
+{
+    return (float)__builtin_bit_cast(_Float16, nd.box[i]);
+}
+
+// Entry and exit distances of the ray's segment [0, tmax] through child k's
+// box of node nd; the segment crosses the box when enter <= leave.
+__device__ __forceinline__ void box_span(const BoxRay &b, const BvhNode &nd, int k, float tmax, float &enter,
+                                         float &leave)
 {
     float n[3], f[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const float u = __builtin_fmaf(lo[a], b.inv[a], b.ns[a]);
-        const float v = __builtin_fmaf(hi[a], b.inv[a], b.ns[a]);
+        const float u = __builtin_fmaf(box_half(nd, 6 * k + a), b.inv[a], b.ns[a]);
+        const float v = __builtin_fmaf(box_half(nd, 6 * k + 3 + a), b.inv[a], b.ns[a]);
         n[a] = __builtin_fminf(u, v);
         f[a] = __builtin_fmaxf(u, v);
     }
@@ -1669,9 +1685,10 @@ __device__ __forceinline__ void bvh_leaf(GPTR(FastPrim) L, const FastPrim &q0, i
 // -1 when the walk is over) and its record loaded into nd, then the
 // primitives of the crossed leaf children tested.  The visiting order and the
 // tb each box is tested against are those of testing the leaves first: a
-// node's boxes are tested only when it is visited, after the leaves before it.  A pruned box holds no hit
-// at or before tb (the host pads every box beyond the FP32 error of hits and
-// of slab distances), so neither an FP32 hit nor an FP64 re-solve inside it
+// node's boxes are tested only when it is visited, after the leaves before
+// it.  A pruned box holds no hit at or before tb (the host pads every box
+// beyond the FP32 error of hits and of slab distances, and rounds its stored
+// corners outward), so neither an FP32 hit nor an FP64 re-solve inside it
 // could change the result.
 template <bool REFINE, bool COUNT>
 __device__ __forceinline__ void bvh_step(GPTR(BvhNode) N, GPTR(FastPrim) L, int &node, BvhNode &nd, int &sp, int &bp,
@@ -1679,15 +1696,16 @@ __device__ __forceinline__ void bvh_step(GPTR(BvhNode) N, GPTR(FastPrim) L, int 
                                          int slot, uint32_t *stk, uint32_t &fl, uint32_t &ev)
 {
     float e0, l0, e1, l1;
-    box_span(B, nd.lo0, nd.hi0, tb, e0, l0);
-    box_span(B, nd.lo1, nd.hi1, tb, e1, l1);
+    box_span(B, nd, 0, tb, e0, l0);
+    box_span(B, nd, 1, tb, e1, l1);
+    const int c[2] = {bvh_index(nd.ref[0]), bvh_index(nd.ref[1])};
+    const int n[2] = {bvh_leafword(nd.ref[0]), bvh_leafword(nd.ref[1])};
     const bool h0 = e0 <= l0;
-    const bool h1 = (e1 <= l1) & ((nd.c[1] >= 0) | (nd.n[1] > 0));
+    const bool h1 = (e1 <= l1) & (c[1] >= 0);
     if constexpr (COUNT) {
         fl += 2u * kFlopsBox;
         ev += 2u;
     }
-    const int c[2] = {nd.c[0], nd.c[1]}, n[2] = {nd.n[0], nd.n[1]};
     const bool leaf[2] = {h0 && n[0] > 0, h1 && n[1] > 0};
     // the crossed leaves' first records, then the next node: their loads
     // are in flight together, and the next node's under the leaf tests
@@ -1712,8 +1730,8 @@ __device__ __forceinline__ void bvh_step(GPTR(BvhNode) N, GPTR(FastPrim) L, int 
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         if (leaf[k]) {
-            const int cnt = n[k] & 255;
-            switch (n[k] >> 8) {
+            const int cnt = n[k] & 7;
+            switch (n[k] >> 3) {
             case SEG_CYL: bvh_leaf<SEG_CYL, REFINE, COUNT>(L, q[k], c[k], cnt, R, tb, ib, bp, pd, slot, key0, fl, ev); break;
             case SEG_FRUSTUM:
                 bvh_leaf<SEG_FRUSTUM, REFINE, COUNT>(L, q[k], c[k], cnt, R, tb, ib, bp, pd, slot, key0, fl, ev);
@@ -1760,7 +1778,7 @@ __device__ __forceinline__ int find_d_bvh(CPTR(GeomDesc) G, const Ray64 &ray, ui
     Ray32 r32;
     r32.sx = (float)ray.sx; r32.sy = (float)ray.sy; r32.sz = (float)ray.sz;
     r32.cx = (float)ray.cx; r32.cy = (float)ray.cy; r32.cz = (float)ray.cz;
-    const BoxRay B = box_ray(r32);
+    const BoxRay B = box_ray(r32, G);
     double tbest = dinf();
     int obj = -1;
     uint32_t tests = 0, fl = kFlopsBoxRay;
@@ -1769,16 +1787,18 @@ __device__ __forceinline__ int find_d_bvh(CPTR(GeomDesc) G, const Ray64 &ray, ui
         const BvhNode nd = ldg(N + node);
         const float tb = (float)tbest;
         float e0, l0, e1, l1;
-        box_span(B, nd.lo0, nd.hi0, tb, e0, l0);
-        box_span(B, nd.lo1, nd.hi1, tb, e1, l1);
+        box_span(B, nd, 0, tb, e0, l0);
+        box_span(B, nd, 1, tb, e1, l1);
+        const int ci[2] = {bvh_index(nd.ref[0]), bvh_index(nd.ref[1])};
+        const int ni[2] = {bvh_leafword(nd.ref[0]), bvh_leafword(nd.ref[1])};
         const bool h0 = e0 <= l0;
-        const bool h1 = (e1 <= l1) & ((nd.c[1] >= 0) | (nd.n[1] > 0));
+        const bool h1 = (e1 <= l1) & (ci[1] >= 0);
         tests += 2u;
         fl += 2u * kFlopsBox;
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            if ((k ? h1 : h0) && nd.n[k] > 0) {
-                const int c = nd.c[k], cnt = nd.n[k] & 255;
+            if ((k ? h1 : h0) && ni[k] > 0) {
+                const int c = ci[k], cnt = ni[k] & 7;
                 for (int j = c; j < c + cnt; ++j) {
                     // the record's caller index (FastPrim::orig)
                     const int w = ((GPTR(int32_t))(L + j))[offsetof(FastPrim, orig) / 4];
@@ -1790,16 +1810,16 @@ __device__ __forceinline__ int find_d_bvh(CPTR(GeomDesc) G, const Ray64 &ray, ui
                 }
             }
         }
-        const bool i0 = h0 & (nd.n[0] == 0), i1 = h1 & (nd.n[1] == 0);
+        const bool i0 = h0 & (ni[0] == 0), i1 = h1 & (ni[1] == 0);
         if (i0 & i1) {
             const bool near0 = e0 <= e1;
-            stk[sp * blockDim.x] = (uint32_t)(near0 ? nd.c[1] : nd.c[0]);
+            stk[sp * blockDim.x] = (uint32_t)(near0 ? ci[1] : ci[0]);
             ++sp;
-            node = near0 ? nd.c[0] : nd.c[1];
+            node = near0 ? ci[0] : ci[1];
         } else if (i0) {
-            node = nd.c[0];
+            node = ci[0];
         } else if (i1) {
-            node = nd.c[1];
+            node = ci[1];
         } else {
             node = sp > 0 ? (int)stk[--sp * blockDim.x] : -1;
         }
@@ -1832,7 +1852,7 @@ __device__ __forceinline__ void scan_bvh(CPTR(GeomDesc) G, const Ray32 (&R)[NR],
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
         const bool act = (vmask >> i) & 1u;
-        const BoxRay B = box_ray(R[i]);
+        const BoxRay B = box_ray(R[i], G);
         if (COUNT && act) fl += kFlopsBoxRay;
         bvh_walk<REFINE, COUNT>(N, L, act ? root : -1, key0, R[i], B, tb[i], ib[i], pd, i, stk, fl, ev);
     }

@@ -655,6 +655,39 @@ static bool prim_box(const rf_primitive &p, WBox &b)
     return true;
 }
 
+// Box frame of a scene's hierarchy nodes: world x = c + s h for a stored
+// half-precision h, with s a power of two.
+struct BoxFrame {
+    float c[3];
+    float s;
+};
+
+// IEEE half-precision bit pattern of the largest half <= v (up = false) or
+// the smallest half >= v (up = true), for |v| <= 2^15.
+static uint16_t half_bits(double v, bool up)
+{
+    if (v == 0.0) return 0;
+    int e = 0;
+    (void)std::frexp(std::fabs(v), &e);          // |v| in [2^(e-1), 2^e)
+    int ex = std::max(e - 1, -14);                // binade; below 2^-14 the subnormal spacing
+    double m = std::ldexp(v, 10 - ex);            // exact: in quanta of the binade
+    m = up ? std::ceil(m) : std::floor(m);
+    const uint16_t sign = m < 0 ? 0x8000 : 0;
+    double a = std::fabs(m);
+    if (a == 0.0) return sign;
+    if (a >= 2048.0) { a *= 0.5; ++ex; }          // rounded up into the next binade
+    if (a < 1024.0) return (uint16_t)(sign | (uint16_t)a);   // subnormal
+    return (uint16_t)(sign | (uint16_t)((ex + 15) << 10) | (uint16_t)((int)a - 1024));
+}
+
+// Value of a half-precision bit pattern (finite).
+static double half_value(uint16_t h)
+{
+    const int ex = (h >> 10) & 31, mt = h & 1023;
+    const double v = ex ? std::ldexp(1024.0 + mt, ex - 25) : std::ldexp((double)mt, -24);
+    return (h & 0x8000) ? -v : v;
+}
+
 // Hierarchy over items (traversal positions with their padded boxes and type
 // segments), split by the surface-area heuristic (the split of the centroid
 // bins that minimises count x surface area summed over both sides) while the
@@ -663,7 +696,7 @@ static bool prim_box(const rf_primitive &p, WBox &b)
 // but of mixed segments is split between segments.  Returns the root node
 // index and the depth (the most stack entries a traversal needs).
 static int32_t build_bvh(std::vector<int32_t> items, const std::vector<WBox> &box, const std::vector<int> &kind,
-                         std::vector<BvhNode> &nodes, std::vector<int32_t> &pos, int &depth)
+                         const BoxFrame &fr, std::vector<BvhNode> &nodes, std::vector<int32_t> &pos, int &depth)
 {
     struct Ref {
         int32_t c, n;
@@ -679,13 +712,24 @@ static int32_t build_bvh(std::vector<int32_t> items, const std::vector<WBox> &bo
             }
         return u;
     };
+    // a child's box in the frame, corners rounded outward; its reference word
+    const auto put_box = [&](BvhNode &nd, int k, const WBox &b) {
+        for (int a = 0; a < 3; ++a) {
+            nd.box[6 * k + a] = half_bits((b.lo[a] - fr.c[a]) / fr.s, false);
+            nd.box[6 * k + 3 + a] = half_bits((b.hi[a] - fr.c[a]) / fr.s, true);
+        }
+    };
+    const auto ref_of = [](const Ref &x) {
+        if (x.c < 0) return BVH_NO_CHILD;
+        const uint32_t cnt = (uint32_t)x.n & 255u, seg = (uint32_t)x.n >> 8;
+        return ((uint32_t)x.c & BVH_NO_CHILD) | cnt << BVH_INDEX_BITS | seg << (BVH_INDEX_BITS + 3);
+    };
     const auto node_of = [&](int32_t self, const Ref &l, const Ref &r) {
         BvhNode &nd = nodes[self];
-        for (int a = 0; a < 3; ++a) {
-            nd.lo0[a] = (float)l.b.lo[a]; nd.hi0[a] = (float)l.b.hi[a];
-            nd.lo1[a] = (float)r.b.lo[a]; nd.hi1[a] = (float)r.b.hi[a];
-        }
-        nd.c[0] = l.c; nd.n[0] = l.n; nd.c[1] = r.c; nd.n[1] = r.n;
+        put_box(nd, 0, l.b);
+        put_box(nd, 1, r.b);
+        nd.ref[0] = ref_of(l);
+        nd.ref[1] = ref_of(r);
         WBox u;
         for (int a = 0; a < 3; ++a) { u.lo[a] = std::min(l.b.lo[a], r.b.lo[a]); u.hi[a] = std::max(l.b.hi[a], r.b.hi[a]); }
         return Ref{self, 0, u};
@@ -797,11 +841,10 @@ static int32_t build_bvh(std::vector<int32_t> items, const std::vector<WBox> &bo
         nodes.push_back(BvhNode{});
         const Ref l = rec(items.data(), items.size(), 1);
         BvhNode &nd = nodes[self];
-        for (int a = 0; a < 3; ++a) {
-            nd.lo0[a] = (float)l.b.lo[a]; nd.hi0[a] = (float)l.b.hi[a];
-            nd.lo1[a] = 1.0f; nd.hi1[a] = -1.0f;
-        }
-        nd.c[0] = l.c; nd.n[0] = l.n; nd.c[1] = -1; nd.n[1] = 0;
+        put_box(nd, 0, l.b);
+        for (int a = 0; a < 3; ++a) { nd.box[6 + a] = 0x3C00; nd.box[9 + a] = 0xBC00; }   // +1, -1: empty
+        nd.ref[0] = ref_of(l);
+        nd.ref[1] = BVH_NO_CHILD;
         return self;
     }
     return rec(items.data(), items.size(), 0).c;
@@ -848,7 +891,7 @@ extern "C" int64_t rf_scene_footprint(int32_t n, int32_t nbv)
 {
     if (n < 0 || nbv < 0) return -1;
     return (int64_t)n * (sizeof(rf_primitive) + sizeof(Emitter) + 2 * sizeof(FastPrim) + 64 + 8 + 2 * sizeof(BandRec) +
-                         sizeof(BvhNode) / 2 + sizeof(FastPrim) +
+                         sizeof(BvhNode) + sizeof(FastPrim) +
                          2 * sizeof(CylRun) + 4) + 9 * 256 +
            (int64_t)nbv * (sizeof(rf_bounding_volume) + sizeof(FastPrim) + sizeof(BvMeta) + (N_SEG + 1) * 4 +
                            4 + 8) +
@@ -1049,12 +1092,14 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     // ulps of the coordinates) and of the kernel's slab entry and exit
     // distances (one FMA of a box face with the reciprocal direction, a few
     // ulps of the largest coordinate over |c|), so a pruned box holds no hit
-    // at or before the best one
+    // at or before the best one; the nodes store the padded boxes in half
+    // precision relative to the scene's centre, rounded outward
     std::vector<BvhNode> bvh_nodes;
     std::vector<int32_t> bvh_pos;
     std::vector<FastPrim> bvh_prims;
     int32_t bvh_root = -1;
     int bvh_depth = 0;
+    BoxFrame frame{{0.0f, 0.0f, 0.0f}, 1.0f};
     if (bvh_on) {
         std::vector<WBox> box(n);
         WBox all;
@@ -1087,10 +1132,30 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
                     for (int32_t p = seg_begin[k]; p < seg_end[k]; ++p) items.push_back(p);
                 }
             }
+            // the nodes' box frame: the centre of the scene's box, and a
+            // power-of-two scale that puts every padded corner within 2^14
+            // of it in half precision
+            double reach = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                frame.c[a] = (float)(0.5 * (all.lo[a] + all.hi[a]));
+                reach = std::max({reach, std::fabs(all.lo[a] - pad - frame.c[a]), std::fabs(all.hi[a] + pad - frame.c[a])});
+            }
+            int se = 0;
+            (void)std::frexp(reach / 16384.0, &se);
+            frame.s = std::ldexp(1.0f, se);
             int deepest = 0;
-            bvh_root = build_bvh(items, box, kind, bvh_nodes, bvh_pos, deepest);
-            // a traversal keeps at most one entry per level on its stack
-            if (deepest >= BVH_MAX_DEPTH) bvh_on = false;
+            bvh_root = build_bvh(items, box, kind, frame, bvh_nodes, bvh_pos, deepest);
+            // a traversal keeps at most one entry per level on its stack;
+            // child references hold 25-bit indices
+            if (deepest >= BVH_MAX_DEPTH || bvh_nodes.size() >= BVH_NO_CHILD / 2 || bvh_pos.size() >= BVH_NO_CHILD / 2)
+                bvh_on = false;
+            // every stored box holds its padded box (outward rounding)
+            for (size_t q = 0; q < bvh_nodes.size() && bvh_on; ++q)
+                for (int k = 0; k < 2; ++k)
+                    for (int a = 0; a < 3; ++a)
+                        if (!std::isfinite(half_value(bvh_nodes[q].box[6 * k + a])) ||
+                            !std::isfinite(half_value(bvh_nodes[q].box[6 * k + 3 + a])))
+                            bvh_on = false;
             bvh_depth = deepest;
             // leaf records in leaf order, each carrying its traversal position
             for (int32_t p : bvh_pos) {
@@ -1213,6 +1278,8 @@ extern "C" int32_t rf_scene_create(rf_ctx *ctx, const rf_primitive *prims, int32
     d.bvh_prims = s->d_bvh_prims;
     d.bvh_root = bvh_root;
     d.bvh_on = bvh_on ? (s->stack_prims == 0 ? 2 : 1) : 0;
+    for (int a = 0; a < 3; ++a) d.bvh_c[a] = frame.c[a];
+    d.bvh_s = frame.s;
     up((void **)&s->d_desc, &d, sizeof d);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
