@@ -1463,25 +1463,28 @@ static int walked_depth(const rf_scene *s, const rf_solve_opts *o)
     return o->math == RF_MATH_FAST && s->bvh_on ? s->bvh_depth : 0;
 }
 
-// Rays per thread of the hierarchy form (compiled for 1, 2, 4 and 8; 2 unless
-// the caller asks: with four sub-passes per tile, 2048-ray tiles measured
-// best on 300- to 2000-primitive soups, 1.2x 4 rays per thread, 1.4x 1).
-static constexpr int kBvhRpt = 2;
+// Rays per thread of the hierarchy form (compiled for 1, 2, 4 and 8) unless
+// the caller asks: 4 where no scene of the launch has band stacks (with
+// half-precision nodes and four sub-passes per tile, 4096-ray tiles measured
+// 1.03x to 1.3x 2 rays per thread on 300- to 2000-primitive soups); 2 where
+// one has, since band stacks are searched only at 1 and 2.
+static constexpr int kBvhRpt = 4;
+static constexpr int kBvhRptStacks = 2;
 
 // The hierarchy form of the fast plain scan (VALU transform, band stacks on,
 // the caller's hierarchy switch on) when some scene of the launch has a
 // hierarchy (depth: the deepest one's; the units of the others scan):
-// kBvhRpt rays per thread unless asked, no LDS band copy, the device counters
+// kBvhRpt (kBvhRptStacks) rays per thread unless asked, no LDS band copy, the device counters
 // as count_tests asks, and each lane's traversal stack (depth + 1 entries)
 // after the re-solve list (the LDS tally is dropped if they would not fit).
-static void bvh_switch(const rf_solve_opts *o, LaunchCfg &c, KParams &kp, int depth)
+static void bvh_switch(const rf_solve_opts *o, LaunchCfg &c, KParams &kp, int depth, bool stacks)
 {
     c.bvh = 0;
     if (depth <= 0 || o->hierarchy != 0 || (c.math != KM_FAST && c.math != KM_FP64) || c.use_bv || c.direct ||
         c.mfma || !kp.stacks)
         return;
     // (the FP64 walk's tiles are of 2 rays per thread whatever the option)
-    int nr = c.math == KM_FP64 ? 2 : (o->rays_per_thread ? o->rays_per_thread : kBvhRpt);
+    int nr = c.math == KM_FP64 ? 2 : (o->rays_per_thread ? o->rays_per_thread : stacks ? kBvhRptStacks : kBvhRpt);
     const size_t fixed = (size_t)(kPendK + depth + 1) * c.block * sizeof(uint32_t);
     // the kernel's own LDS: the tile sort's cell counts and two 16-bit words
     // per ray of a tile (four sub-passes of up to 256 x nr rays); fewer rays
@@ -1644,7 +1647,7 @@ static int launch_range(rf_ctx *ctx, rf_scene *s, int emitter, int64_t wi0, int6
     c.lds_bytes = ((c.trace || kp.bins == 0) ? 0 : (size_t)kp.bins * sizeof(uint32_t)) +
                   (size_t)kPendK * block * sizeof(uint32_t);
     band_switches(ctx, o, c, kp, cyl_records(s), s->has_search);
-    bvh_switch(o, c, kp, walked_depth(s, o));
+    bvh_switch(o, c, kp, walked_depth(s, o), s->stack_prims > 0);
     if (c.math == KM_FAST && c.use_bv && s->desc.bv_binned) {
         c.bvbin = 1;
         kp.bv_slots = s->n_bvs;
@@ -1862,8 +1865,12 @@ static int prepare_persistent(rf_ctx *ctx, rf_scene *const *scenes, int n_geoms,
     for (int g = 0; g < n_geoms; ++g) max_cyl = std::max(max_cyl, cyl_records(scenes[g]));
     band_switches(ctx, o, c, kp, max_cyl, searched);
     int bvh_depth = 0;
-    for (int g = 0; g < n_geoms; ++g) bvh_depth = std::max(bvh_depth, walked_depth(scenes[g], o));
-    bvh_switch(o, c, kp, bvh_depth);
+    bool any_stacks = false;
+    for (int g = 0; g < n_geoms; ++g) {
+        bvh_depth = std::max(bvh_depth, walked_depth(scenes[g], o));
+        any_stacks = any_stacks || scenes[g]->stack_prims > 0;
+    }
+    bvh_switch(o, c, kp, bvh_depth, any_stacks);
     if (c.math == KM_FAST && use_bv) {
         bool binned = true;
         int slots = 0;
@@ -2218,7 +2225,7 @@ extern "C" int32_t rf_solve_noalloc(rf_ctx *ctx, rf_scene *s, const rf_solve_opt
     c.grid = (int)std::min<int64_t>(std::max<int64_t>(n_units, 1), groups);
     c.lds_bytes = (kp.bins ? (size_t)kp.bins * sizeof(uint32_t) : 0) + (size_t)kPendK * block * sizeof(uint32_t);
     band_switches(ctx, o, c, kp, cyl_records(s), s->has_search);
-    bvh_switch(o, c, kp, walked_depth(s, o));
+    bvh_switch(o, c, kp, walked_depth(s, o), s->stack_prims > 0);
     if (c.math == KM_FAST && c.use_bv && s->desc.bv_binned) {
         c.bvbin = 1;
         kp.bv_slots = s->n_bvs;
